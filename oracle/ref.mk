@@ -1,0 +1,44 @@
+# Builds the reference engine from its own sources where they lie under
+# /root/reference (read-only) into oracle/_ref/.  Test infrastructure only:
+# used to pin the oracle restatement and as the CPU baseline in bench.py.
+#
+#   make -f oracle/ref.mk            (from the repository root)
+#
+# Nothing is copied from the reference; only compiled object code lands in
+# oracle/_ref/ (git-ignored).  Two ISA variants are produced: the AVX-512
+# build runs the reference's native W8 route, the AVX2 build its legacy W4
+# route; callers pick the one the host CPU supports.
+
+REF ?= /root/reference
+OUT := oracle/_ref
+CXX ?= g++
+CXXFLAGS_COMMON := -std=c++17 -O2 -fopenmp -fPIC -fvisibility=hidden -I$(REF)
+
+CORE_SRC := $(REF)/core/math/general_math.cpp \
+            $(REF)/core/shell/shell_math.cpp \
+            $(REF)/core/shell/shell_contract.cpp \
+            $(REF)/core/shell/cart2sph.cpp
+ONEE_SRC := $(REF)/integrals/one_electron/onee_integrals.cpp \
+            $(REF)/integrals/one_electron/onee_matrices.cpp
+TWOE_SRC := $(REF)/integrals/two_electron/twoe_integrals.cpp \
+            $(REF)/integrals/two_electron/fock/jk_direct.cpp
+
+ISAS := avx512 avx2
+march_avx512 := -march=x86-64-v4
+march_avx2 := -march=x86-64-v3
+
+API_LIBS := $(foreach i,$(ISAS),$(OUT)/libtlno_ref_$(i).so)
+AUDIT_LIBS := $(foreach i,$(ISAS),$(OUT)/libtlno_ref_audit_$(i).so)
+
+all: $(API_LIBS) $(AUDIT_LIBS)
+
+$(OUT):
+	mkdir -p $(OUT)
+
+$(OUT)/libtlno_ref_%.so: oracle/ref_api_shim.cpp $(CORE_SRC) $(ONEE_SRC) $(TWOE_SRC) | $(OUT)
+	$(CXX) $(CXXFLAGS_COMMON) $(march_$*) -shared -o $@ $^
+
+$(OUT)/libtlno_ref_audit_%.so: oracle/ref_audit_shim.cpp $(CORE_SRC) | $(OUT)
+	$(CXX) $(CXXFLAGS_COMMON) $(march_$*) -shared -o $@ $^
+
+.PHONY: all
