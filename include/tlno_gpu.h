@@ -1,0 +1,113 @@
+/*
+ * tlno_gpu.h — C ABI of the MI355X (gfx950) Gaussian-integral engine.
+ *
+ * Plain pointers and sizes only.  Every function returns 0 on success and a
+ * nonzero status on failure; tlno_gpu_last_error() then holds a message that
+ * uses the reference engine's wording (prefixes "eri_ao_tensor_hgp:" and
+ * "jk_direct:").  Host-pointer entry points copy in and out; *_device entry
+ * points take device pointers that are already resident in HBM.
+ *
+ * Basis input is the reference's component layout
+ * (core/basis/types.hpp:31-36, std::vector<Shell>): component i has centre
+ * centers[3i..3i+2] in bohr, Cartesian powers ang[3i..3i+2], nprim[i]
+ * primitives whose exponents and stored (normalized) coefficients are
+ * consecutive in exps/coefs.  Components are grouped into shells exactly as
+ * the reference does (integrals/two_electron/rys/rys_engine.ipp:1966-2062).
+ *
+ * AO conventions:
+ *   Cartesian mode: the AO index of a function is its component index.
+ *   Spherical mode: shells in first-appearance order, 2L+1 real solid
+ *   harmonics each in PySCF order (rys_engine.ipp:2365-2379), exactly the
+ *   M x M basis of the reference's J/K endpoints.
+ */
+#ifndef TLNO_GPU_H
+#define TLNO_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tlno_gpu_plan tlno_gpu_plan;
+
+#define TLNO_GPU_J 1
+#define TLNO_GPU_K 2
+
+/* Message of the last failing call on this thread. */
+const char* tlno_gpu_last_error(void);
+
+/* Selects the HIP device for subsequent plans of this thread and builds the
+ * device Boys/Rys tables (no reference counterpart: device selection). */
+int tlno_gpu_init(int device);
+
+/* Replaces build_direct_plan / build_eri_packed_plan_for_G_rys
+ * (integrals/two_electron/fock/jk_direct.hpp:54-56,
+ *  integrals/two_electron/twoe_integrals.hpp:104-106).
+ * spherical = 1 builds the reference's spherical AO basis, 0 the Cartesian
+ * component basis.  Schwarz bounds are computed here, on the GPU. */
+int tlno_gpu_plan_create(int n_comp, const double* centers, const int* ang,
+                         const int* nprim, const double* exps, const double* coefs,
+                         int spherical, tlno_gpu_plan** out);
+void tlno_gpu_plan_destroy(tlno_gpu_plan* plan);
+
+/* M (output AO count), shell groups, shell pairs, primitive pairs
+ * (EriPackedPlanProfile, twoe_integrals.hpp:68-89). */
+int tlno_gpu_plan_info(const tlno_gpu_plan* plan, int* M, int* n_groups,
+                       int* n_pairs, long long* n_prim_pairs);
+
+/* Shell groups of the plan in first-appearance order: L[g] and, for each
+ * group, the basis (component) index of its Cartesian components in
+ * descending-lex order, comps[15*g + k] (unused slots -1).  max_groups bounds
+ * both arrays.  Counterpart of the grouping in build_groups_h
+ * (rys_engine.ipp:1966-2062). */
+int tlno_gpu_plan_groups(const tlno_gpu_plan* plan, int max_groups, int* L, int* comps);
+
+/* Replaces build_coulomb_j / build_exchange_k / build_rhf_g
+ * (jk_direct.hpp:80-96) and build_jk_direct_for_G_rys (eri_direct_jk.hpp:36-39):
+ * J[a,b] = sum_cd P[c,d](ab|cd), K[a,c] = sum_bd P[b,d](ab|cd), G = J - K/2.
+ * density, J, K are M x M row-major host arrays; J or K may be NULL.
+ * schwarz: shell quartets with Q_ab Q_cd < schwarz are skipped (>= 0).
+ * The density must be finite and symmetric (128 eps), as in the reference
+ * (fock/detail/direct_build.ipp:29-97). */
+int tlno_gpu_build_jk(tlno_gpu_plan* plan, const double* density, double* J,
+                      double* K, double schwarz);
+
+/* Same build on device pointers (M x M doubles each, J or K may be NULL),
+ * enqueued on the plan's stream and completed before return.  No density
+ * validation is done on this path. */
+int tlno_gpu_build_jk_device(tlno_gpu_plan* plan, const double* d_density,
+                             double* d_J, double* d_K, double schwarz);
+
+/* Work statistics of the last build: shell quartets and primitive quartets
+ * evaluated (after Schwarz screening). */
+int tlno_gpu_last_build_stats(const tlno_gpu_plan* plan, long long* shell_quartets,
+                              long long* prim_quartets);
+
+/* Unit-diagonal Cartesian ERI blocks (ga gb|gc gd) for n group quadruples
+ * (groups4[4i..4i+3], groups numbered in first-appearance order); block i is
+ * written at out + offsets[i] as [a][b][c][d] in each group's component
+ * order (the order the components appear in the basis).  Test/audit entry
+ * point (audit-TU counterpart: compute_quartet_block_rys_impl,
+ * rys_engine.ipp:13754-13842). */
+int tlno_gpu_eri_cart_blocks(tlno_gpu_plan* plan, int n, const int* groups4,
+                             const long long* offsets, double* out);
+
+/* Boys F_0..F_mmax at n arguments: out[i*(mmax+1)+m] (mmax <= 16).
+ * Counterpart of boys_single / boys_r000_all (core/math/general_math.hpp:6-8)
+ * evaluated by the device evaluator used in the ERI kernels. */
+int tlno_gpu_boys(int n, const double* T, int mmax, double* out);
+
+/* Rys roots t^2 and weights for N = 1..5 at n arguments: roots[i*N+k].
+ * Counterpart of rys_roots_weights (rys_engine.ipp:3820-3837). */
+int tlno_gpu_rys(int N, int n, const double* T, double* roots, double* weights);
+
+/* One-electron S, T, V (n x n row-major each, n = M of the plan's basis in
+ * the plan's mode).  Replaces build_one_electron_matrices /
+ * build_one_electron_spherical_matrices (integrals/one_electron/onee_matrices.hpp:31-79).
+ * nuclear charges Z (n_nuc) and centres R (3*n_nuc, bohr). */
+int tlno_gpu_onee(tlno_gpu_plan* plan, int n_nuc, const double* Z, const double* R,
+                  double* S, double* T, double* V);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -69,7 +69,8 @@ static void eri_block(const orc_group* A, const orc_group* B, const orc_group* C
             const double Kab = exp(-al * be / p * rab2);
             const double cab = A->c_radial[i] * B->c_radial[j];
             double P[3];
-            for (int k = 0; k < 3; ++k) P[k] = (al * A->center[k] + be * B->center[k]) / p;
+            /* P = (a A + b B)/p, evaluated as A - (b/p)(A - B) (exact for one centre) */
+            for (int k = 0; k < 3; ++k) P[k] = A->center[k] - be / p * AB[k];
             for (int k2 = 0; k2 < C->nprim; ++k2)
                 for (int l = 0; l < D->nprim; ++l) {
                     const double ga = C->exps[k2], de = D->exps[l];
@@ -78,7 +79,7 @@ static void eri_block(const orc_group* A, const orc_group* B, const orc_group* C
                     const double ccd = C->c_radial[k2] * D->c_radial[l];
                     double Q[3], PQ[3];
                     for (int k = 0; k < 3; ++k) {
-                        Q[k] = (ga * C->center[k] + de * D->center[k]) / q;
+                        Q[k] = C->center[k] - de / q * CD[k];
                         PQ[k] = P[k] - Q[k];
                     }
                     const double pq = p + q;

@@ -3,8 +3,12 @@
  *
  * Follows integrals/one_electron/onee_integrals.cpp (reference):
  *  - overlap from 1D Gaussian-product tables (32-51, 283-448);
- *  - kinetic per axis T_x = b(2j+1) S(i,j) - 2 b^2 S(i,j+2) - j(j-1)/2 S(i,j-2),
- *    T = T_x S_y S_z + S_x T_y S_z + S_x S_y T_z (283-448);
+ *  - kinetic T = T_x S_y S_z + S_x T_y S_z + S_x S_y T_z (283-448).  The
+ *    reference applies -1/2 d^2/dx^2 to the ket,
+ *    T_x = b(2j+1) S(i,j) - 2 b^2 S(i,j+2) - j(j-1)/2 S(i,j-2), whose terms
+ *    cancel to a relative 1/b when a tight ket meets a diffuse bra; the
+ *    oracle uses the equal symmetric form 1/2 <d/dx a|d/dx b>,
+ *    T_x = 1/2 [ij S(i-1,j-1) - 2ib S(i-1,j+1) - 2ja S(i+1,j-1) + 4ab S(i+1,j+1)];
  *  - nuclear attraction by McMurchie-Davidson: Hermite E^{ij}_t per axis,
  *    R^{(n)}_{tuv} recursion from R^{(n)}_{000} = (-2p)^n F_n(p |PC|^2),
  *    V = -(2 pi / p) sum_C Z_C sum_tuv E_t E_u E_v R_tuv (53-157, 476-918).
@@ -52,11 +56,11 @@ static void prim_block(const orc_group* A, const orc_group* B, int ia, int ib, i
     double P[3], PA[3], PB[3];
     double E[3][8][8][16];
     for (int k = 0; k < 3; ++k) {
-        P[k] = (al * A->center[k] + be * B->center[k]) / p;
+        P[k] = A->center[k] - be / p * (A->center[k] - B->center[k]);
         PA[k] = P[k] - A->center[k];
         PB[k] = P[k] - B->center[k];
         const double X = A->center[k] - B->center[k];
-        hermite_E(A->L, B->L + 2, p, PA[k], PB[k], exp(-mu * X * X), E[k]);
+        hermite_E(A->L + 1, B->L + 1, p, PA[k], PB[k], exp(-mu * X * X), E[k]);
     }
     const double sp = sqrt(PI_D / p);
     const int na = A->ncomp, nb = B->ncomp;
@@ -67,9 +71,13 @@ static void prim_block(const orc_group* A, const orc_group* B, int ia, int ib, i
             for (int k = 0; k < 3; ++k) {
                 const int i = A->ang[a][k], j = B->ang[b][k];
                 s1[k] = E[k][i][j][0] * sp;
-                double tv = be * (2 * j + 1) * s1[k] - 2.0 * be * be * E[k][i][j + 2][0] * sp;
-                if (j >= 2) tv -= 0.5 * j * (j - 1) * E[k][i][j - 2][0] * sp;
-                t1[k] = tv;
+                /* symmetric (gradient) form, well conditioned for a tight
+                 * function against a diffuse one */
+                double tv = 4.0 * al * be * E[k][i + 1][j + 1][0];
+                if (i > 0) tv -= 2.0 * i * be * E[k][i - 1][j + 1][0];
+                if (j > 0) tv -= 2.0 * j * al * E[k][i + 1][j - 1][0];
+                if (i > 0 && j > 0) tv += (double)(i * j) * E[k][i - 1][j - 1][0];
+                t1[k] = 0.5 * tv * sp;
             }
             S[a * nb + b] = s1[0] * s1[1] * s1[2];
             Tk[a * nb + b] = t1[0] * s1[1] * s1[2] + s1[0] * t1[1] * s1[2] + s1[0] * s1[1] * t1[2];

@@ -1,0 +1,1149 @@
+// Host engine and C ABI: plan construction (shell grouping, primitive-pair
+// SoA, GPU Schwarz bounds, class/bound ordering, work lists), density and
+// result transforms, and the J/K build driver.
+//
+// Plan construction follows the reference's plan semantics
+// (integrals/two_electron/rys/rys_engine.ipp:1966-2062 grouping,
+// 2243-2277 primitive pairs, 2365-2460 AO numbering and pair orientation,
+// 16164-16258 Schwarz bounds) but lays everything out for the GPU:
+//   * internal Cartesian AO order = shells in first-appearance order, each
+//     shell's components contiguous in descending-lex order, so a shell's
+//     block is [base, base + ncart) in every M x M matrix;
+//   * shell pairs sorted by class (ss, ps, pp, ds, dp, dd) and, inside a
+//     class, by descending Schwarz bound, so the surviving kets of a bra are
+//     a prefix found by binary search;
+//   * primitive pairs as {Px, Py, Pz, p} double4 + weight arrays.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "eri_device.hpp"
+#include "onee.hpp"
+#include "tlno_gpu.h"
+
+namespace tlno_amd {
+
+thread_local std::string g_err;
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define HIP_OK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess)                                                              \
+            throw HipError(std::string("hip: ") + hipGetErrorString(e_) + " at " #x);      \
+    } while (0)
+
+// ------------------------------------------------------------- device ctx --
+struct DeviceCtx {
+    bool ready = false;
+    double* boys = nullptr;
+    double* rys = nullptr;
+    DevTables tb{};
+};
+
+ClassKernels g_kernels[6][6];
+std::once_flag g_kernels_once;
+std::mutex g_dev_mutex;
+DeviceCtx g_dev[16];
+thread_local int g_device = 0;
+
+void register_all_kernels() {
+    register_kernels_bra0(g_kernels);
+    register_kernels_bra1(g_kernels);
+    register_kernels_bra2(g_kernels);
+    register_kernels_bra3(g_kernels);
+    register_kernels_bra4(g_kernels);
+    register_kernels_bra4b(g_kernels);
+    register_kernels_bra5(g_kernels);
+    register_kernels_bra5b(g_kernels);
+    register_kernels_bra5c(g_kernels);
+}
+
+DeviceCtx& device_ctx(int dev) {
+    std::call_once(g_kernels_once, register_all_kernels);
+    if (dev < 0 || dev >= 16) throw std::runtime_error("tlno_gpu: device index out of range");
+    std::lock_guard<std::mutex> lk(g_dev_mutex);
+    DeviceCtx& c = g_dev[dev];
+    if (c.ready) return c;
+    HIP_OK(hipSetDevice(dev));
+    const HostTables& t = host_tables();
+    HIP_OK(hipMalloc(&c.boys, t.boys.size() * sizeof(double)));
+    HIP_OK(hipMemcpy(c.boys, t.boys.data(), t.boys.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_OK(hipMalloc(&c.rys, t.rys.size() * sizeof(double)));
+    HIP_OK(hipMemcpy(c.rys, t.rys.data(), t.rys.size() * sizeof(double), hipMemcpyHostToDevice));
+    c.tb.boys = c.boys;
+    c.tb.rys = c.rys;
+    for (int i = 0; i < RYS_NMAX + 2; ++i) c.tb.rys_off[i] = t.rys_off[i];
+    std::memcpy(c.tb.herm_r2, t.herm_r2, sizeof(c.tb.herm_r2));
+    std::memcpy(c.tb.herm_w, t.herm_w, sizeof(c.tb.herm_w));
+    c.ready = true;
+    return c;
+}
+
+// -------------------------------------------------- cartesian -> spherical --
+// PySCF-ordered real solid harmonics over unit-diagonal Cartesians, from the
+// solid-harmonic recursion (Helgaker/Jorgensen/Olsen eqs. 6.4.70-6.4.73),
+// normalized in the unit-diagonal metric.  Row s, column = local Cartesian
+// component (descending lex).
+struct C2S {
+    double c[3][5][6] = {};  // L <= 2
+};
+
+double dfact(int n) {
+    double v = 1.0;
+    for (int k = 2 * n - 1; k > 0; k -= 2) v *= k;
+    return v;
+}
+
+int local_index(int L, int lx, int ly) {
+    const int dx = L - lx;
+    return dx * (dx + 1) / 2 + (dx - ly);
+}
+
+void local_tuple(int L, int idx, int* t) {
+    int k = 0;
+    for (int x = L; x >= 0; --x)
+        for (int y = L - x; y >= 0; --y, ++k)
+            if (k == idx) {
+                t[0] = x;
+                t[1] = y;
+                t[2] = L - x - y;
+                return;
+            }
+}
+
+const C2S& c2s_table() {
+    static C2S tab = [] {
+        C2S out;
+        // polynomials over monomials of degree l, index local_index
+        std::vector<std::vector<std::vector<double>>> S(3);
+        for (int l = 0; l <= 2; ++l)
+            S[l].assign(2 * l + 1, std::vector<double>(static_cast<std::size_t>(ncart(l)), 0.0));
+        S[0][0][0] = 1.0;
+        auto mul_axis = [](const std::vector<double>& a, int L, int axis, double s, std::vector<double>& o) {
+            for (int i = 0; i < ncart(L); ++i) {
+                if (a[i] == 0.0) continue;
+                int t[3];
+                local_tuple(L, i, t);
+                t[axis] += 1;
+                o[local_index(L + 1, t[0], t[1])] += s * a[i];
+            }
+        };
+        auto mul_r2 = [](const std::vector<double>& a, int L, double s, std::vector<double>& o) {
+            for (int i = 0; i < ncart(L); ++i) {
+                if (a[i] == 0.0) continue;
+                int t[3];
+                local_tuple(L, i, t);
+                for (int k = 0; k < 3; ++k) {
+                    int u[3] = {t[0], t[1], t[2]};
+                    u[k] += 2;
+                    o[local_index(L + 2, u[0], u[1])] += s * a[i];
+                }
+            }
+        };
+        for (int l = 0; l < 2; ++l) {
+            const double f = std::sqrt((l == 0 ? 2.0 : 1.0) * (2.0 * l + 1.0) / (2.0 * l + 2.0));
+            mul_axis(S[l][2 * l], l, 0, f, S[l + 1][2 * l + 2]);
+            if (l > 0) mul_axis(S[l][0], l, 1, -f, S[l + 1][2 * l + 2]);
+            mul_axis(S[l][2 * l], l, 1, f, S[l + 1][0]);
+            if (l > 0) mul_axis(S[l][0], l, 0, f, S[l + 1][0]);
+            for (int m = -l; m <= l; ++m) {
+                const double den = std::sqrt(double(l + m + 1) * (l - m + 1));
+                mul_axis(S[l][m + l], l, 2, (2.0 * l + 1.0) / den, S[l + 1][m + l + 1]);
+                if (l > 0 && std::abs(m) <= l - 1)
+                    mul_r2(S[l - 1][m + l - 1], l - 1, -std::sqrt(double(l + m) * (l - m)) / den,
+                           S[l + 1][m + l + 1]);
+            }
+        }
+        for (int L = 0; L <= 2; ++L) {
+            const int nc = ncart(L);
+            for (int s = 0; s < 2 * L + 1; ++s) {
+                const int m = (L == 1) ? (s == 0 ? 1 : (s == 1 ? -1 : 0)) : s - L;
+                const auto& P = S[L][m + L];
+                double row[6];
+                for (int c = 0; c < nc; ++c) {
+                    int t[3];
+                    local_tuple(L, c, t);
+                    row[c] = P[c] * std::sqrt(dfact(t[0]) * dfact(t[1]) * dfact(t[2]));
+                }
+                double n2 = 0.0;
+                for (int c = 0; c < nc; ++c)
+                    for (int d = 0; d < nc; ++d) {
+                        int tc[3], td[3];
+                        local_tuple(L, c, tc);
+                        local_tuple(L, d, td);
+                        double g = 1.0;
+                        for (int k = 0; k < 3; ++k) {
+                            const int sm = tc[k] + td[k];
+                            if (sm % 2) {
+                                g = 0.0;
+                                break;
+                            }
+                            g *= dfact(sm / 2) / std::sqrt(dfact(tc[k]) * dfact(td[k]));
+                        }
+                        n2 += row[c] * row[d] * g;
+                    }
+                for (int c = 0; c < nc; ++c) out.c[L][s][c] = row[c] / std::sqrt(n2);
+            }
+        }
+        return out;
+    }();
+    return tab;
+}
+
+// ------------------------------------------------------------------ plan --
+struct GroupH {
+    int L = 0;
+    double C[3] = {0, 0, 0};
+    int nprim = 0;
+    std::vector<double> exps, cr;  // radial coefficients c / N_A
+    int comp[15];                  // basis index of each local canonical component
+    int first = 0;                 // first-appearance component
+    int int_base = 0;
+    int sph_base = 0;
+};
+
+struct TaskSet {
+    int4* d_tasks = nullptr;
+    long long off[6][6] = {};
+    long long cnt[6][6] = {};
+    long long shell_quartets = 0;
+    long long prim_quartets = 0;
+};
+
+}  // namespace tlno_amd
+
+struct tlno_gpu_plan {
+    int device = 0;
+    int spherical = 0;
+    int Mcart = 0, Msph = 0, Mout = 0;
+    std::vector<tlno_amd::GroupH> groups;
+    std::vector<int> int2basis;
+    int npair = 0;
+    int cls_begin[7] = {0};
+    std::vector<int> pgA, pgB;
+    std::vector<double> Q;
+    long long nprim_pairs = 0;
+    std::vector<int> pn_host;
+    // device pair data
+    double *dA = nullptr, *dAB = nullptr, *dpw = nullptr;
+    int *dao = nullptr, *dpoff = nullptr, *dpn = nullptr, *ddiag = nullptr;
+    double4* dpP = nullptr;
+    double* dpexp = nullptr;  // [nprim][2] alpha, beta (one-electron kinetic term)
+    int* dpL = nullptr;       // [npair][2] LA, LB
+    // transform metadata (per internal Cartesian index / per output index)
+    int *d_int2basis = nullptr, *d_intL = nullptr, *d_intloc = nullptr, *d_intsb = nullptr;
+    int *d_sphL = nullptr, *d_sphloc = nullptr, *d_sphib = nullptr;
+    // work buffers
+    double *dDin = nullptr, *dDint = nullptr, *dJacc = nullptr, *dKacc = nullptr;
+    double *dJout = nullptr, *dKout = nullptr;
+    hipStream_t stream = nullptr;
+    std::map<double, tlno_amd::TaskSet> tasks;
+    long long last_shell_quartets = 0, last_prim_quartets = 0;
+
+    tlno_amd::PairData pair_data() const {
+        tlno_amd::PairData p;
+        p.A = dA;
+        p.AB = dAB;
+        p.ao = dao;
+        p.poff = dpoff;
+        p.pn = dpn;
+        p.pP = dpP;
+        p.pw = dpw;
+        p.diag = ddiag;
+        return p;
+    }
+    ~tlno_gpu_plan();
+};
+
+namespace tlno_amd {
+
+template <class T>
+void dev_upload(T*& dst, const std::vector<T>& src) {
+    if (dst) HIP_OK(hipFree(dst));
+    dst = nullptr;
+    if (src.empty()) return;
+    HIP_OK(hipMalloc(&dst, src.size() * sizeof(T)));
+    HIP_OK(hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+bool finite(double v) { return std::isfinite(v); }
+
+int class_of(int LA, int LB) {
+    // LA >= LB
+    static const int t[3][3] = {{0, -1, -1}, {1, 2, -1}, {3, 4, 5}};
+    return t[LA][LB];
+}
+
+// Group components as the reference does (rys_engine.ipp:1925-2062).
+std::vector<GroupH> build_groups(int n, const double* centers, const int* ang, const int* nprim,
+                                 const double* exps, const double* coefs) {
+    std::vector<GroupH> groups;
+    std::size_t off = 0;
+    for (int i = 0; i < n; ++i) {
+        const double* c = centers + 3 * i;
+        const int* a = ang + 3 * i;
+        for (int k = 0; k < 3; ++k)
+            if (!finite(c[k])) throw std::runtime_error("eri_ao_tensor_hgp: shell centers must be finite.");
+        if (a[0] < 0 || a[1] < 0 || a[2] < 0)
+            throw std::runtime_error("eri_ao_tensor_hgp: Cartesian angular powers must be nonnegative.");
+        const int np = nprim[i];
+        if (np <= 0)
+            throw std::runtime_error("eri_ao_tensor_hgp: contracted shells need at least one primitive.");
+        if (np > 16) throw std::runtime_error("eri_ao_tensor_hgp: at most 16 primitives per shell.");
+        const double* e = exps + off;
+        const double* cf = coefs + off;
+        off += static_cast<std::size_t>(np);
+        for (int p = 0; p < np; ++p)
+            if (!finite(e[p]) || e[p] <= 0.0 || !finite(cf[p]))
+                throw std::runtime_error(
+                    "eri_ao_tensor_hgp: primitive exponents must be finite and positive and coefficients "
+                    "must be finite.");
+        const int L = a[0] + a[1] + a[2];
+        if (L > 2)
+            throw std::runtime_error(
+                "eri_ao_tensor_hgp: this build supports shell L=0..2 (s, p, d).");
+        const double na = 1.0 / std::sqrt(dfact(a[0]) * dfact(a[1]) * dfact(a[2]));
+        int gi = -1;
+        for (int g = 0; g < static_cast<int>(groups.size()) && gi < 0; ++g) {
+            const GroupH& G = groups[g];
+            if (G.L != L || G.C[0] != c[0] || G.C[1] != c[1] || G.C[2] != c[2] || G.nprim != np) continue;
+            bool same = true;
+            for (int p = 0; p < np && same; ++p) {
+                if (G.exps[p] != e[p]) same = false;
+                const double x = G.cr[p], y = cf[p] / na;
+                if (x == y) continue;
+                const double sc = std::max(std::abs(x), std::abs(y));
+                if (sc != 0.0 && std::abs(x - y) > 32.0 * 2.220446049250313e-16 * sc) same = false;
+            }
+            if (same) gi = g;
+        }
+        if (gi < 0) {
+            GroupH G;
+            G.L = L;
+            for (int k = 0; k < 3; ++k) G.C[k] = c[k];
+            G.nprim = np;
+            G.exps.assign(e, e + np);
+            G.cr.resize(np);
+            for (int p = 0; p < np; ++p) G.cr[p] = cf[p] / na;
+            for (int k = 0; k < 15; ++k) G.comp[k] = -1;
+            G.first = i;
+            gi = static_cast<int>(groups.size());
+            groups.push_back(std::move(G));
+        }
+        GroupH& G = groups[gi];
+        const int loc = local_index(L, a[0], a[1]);
+        if (G.comp[loc] >= 0)
+            throw std::runtime_error(
+                "eri_ao_tensor_hgp: duplicate or invalid Cartesian component inside a contracted shell.");
+        G.comp[loc] = i;
+    }
+    for (auto& G : groups)
+        for (int k = 0; k < ncart(G.L); ++k)
+            if (G.comp[k] < 0)
+                throw std::runtime_error(
+                    "eri_ao_tensor_hgp: each radial contraction must contain exactly one complete Cartesian "
+                    "shell.");
+    return groups;
+}
+
+// ------------------------------------------------------ transform kernels --
+struct C2SArgs {
+    double c[3][5][6];
+};
+
+// D_int[i][j] = D_user[perm[i]][perm[j]]  (Cartesian mode)
+__global__ void k_prep_cart(const double* __restrict__ Din, double* __restrict__ Dint, const int* perm, int M) {
+    const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (idx >= (long)M * M) return;
+    const int i = static_cast<int>(idx / M), j = static_cast<int>(idx % M);
+    Dint[idx] = Din[(long)perm[i] * M + perm[j]];
+}
+
+// D_int[i][j] = sum_st C[L_i][s][loc_i] D_sph[sb_i + s][sb_j + t] C[L_j][t][loc_j]
+__global__ void k_prep_sph(const double* __restrict__ Dsph, double* __restrict__ Dint, const int* iL,
+                           const int* iloc, const int* isb, int Mc, int Ms, C2SArgs c2s) {
+    const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (idx >= (long)Mc * Mc) return;
+    const int i = static_cast<int>(idx / Mc), j = static_cast<int>(idx % Mc);
+    const int Li = iL[i], Lj = iL[j], li = iloc[i], lj = iloc[j];
+    const double* row = Dsph + (long)isb[i] * Ms + isb[j];
+    double v = 0.0;
+    for (int s = 0; s < 2 * Li + 1; ++s) {
+        double t2 = 0.0;
+        for (int t = 0; t < 2 * Lj + 1; ++t) t2 += row[(long)s * Ms + t] * c2s.c[Lj][t][lj];
+        v += c2s.c[Li][s][li] * t2;
+    }
+    Dint[idx] = v;
+}
+
+// X_user[perm[i]][perm[j]] = Xacc[i][j] + Xacc[j][i]
+__global__ void k_fin_cart(const double* __restrict__ Xacc, double* __restrict__ Xout, const int* perm, int M) {
+    const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (idx >= (long)M * M) return;
+    const int i = static_cast<int>(idx / M), j = static_cast<int>(idx % M);
+    Xout[(long)perm[i] * M + perm[j]] = Xacc[idx] + Xacc[(long)j * M + i];
+}
+
+// X_sph[u][v] = sum_ij C[u][i] (Xacc[i][j] + Xacc[j][i]) C[v][j]
+__global__ void k_fin_sph(const double* __restrict__ Xacc, double* __restrict__ Xout, const int* sL,
+                          const int* sloc, const int* sib, int Mc, int Ms, C2SArgs c2s) {
+    const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (idx >= (long)Ms * Ms) return;
+    const int u = static_cast<int>(idx / Ms), v = static_cast<int>(idx % Ms);
+    const int Lu = sL[u], Lv = sL[v], su = sloc[u], sv = sloc[v];
+    const int iu = sib[u], iv = sib[v];
+    double acc = 0.0;
+    for (int a = 0; a < ncart(Lu); ++a) {
+        const double ca = c2s.c[Lu][su][a];
+        if (ca == 0.0) continue;
+        double t = 0.0;
+        for (int b = 0; b < ncart(Lv); ++b) {
+            const long ij = (long)(iu + a) * Mc + (iv + b), ji = (long)(iv + b) * Mc + (iu + a);
+            t += c2s.c[Lv][sv][b] * (Xacc[ij] + Xacc[ji]);
+        }
+        acc += ca * t;
+    }
+    Xout[idx] = acc;
+}
+
+__global__ void k_fill(double* p, long n, double v) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+template <int MMAX>
+__global__ void k_boys(DevTables tb, const double* T, double* out, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double F[MMAX + 1];
+    boys_all<MMAX>(tb, T[i], F);
+    for (int m = 0; m <= MMAX; ++m) out[(long)i * (MMAX + 1) + m] = F[m];
+}
+
+template <int N>
+__global__ void k_rys(DevTables tb, const double* T, double* r, double* w, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double rr[N], ww[N];
+    rys_roots<N>(tb, T[i], rr, ww);
+    for (int k = 0; k < N; ++k) {
+        r[(long)i * N + k] = rr[k];
+        w[(long)i * N + k] = ww[k];
+    }
+}
+
+inline unsigned grid_for(long n, int block) { return static_cast<unsigned>((n + block - 1) / block); }
+
+void launch_jk(const ClassKernels& k, int chunk, JKArgs& args, long long ntask, hipStream_t st) {
+    void* params[] = {&args};
+    const unsigned grid = grid_for(ntask * 64, 256);
+    HIP_OK(hipLaunchKernel(reinterpret_cast<const void*>(k.jk[chunk]), dim3(grid), dim3(256), params, 0, st));
+}
+
+void launch_store(const ClassKernels& k, int chunk, StoreArgs& args, hipStream_t st) {
+    void* params[] = {&args};
+    HIP_OK(hipLaunchKernel(reinterpret_cast<const void*>(k.store[chunk]), dim3(grid_for(args.n, 128)),
+                           dim3(128), params, 0, st));
+}
+
+// Evaluates full Cartesian pair-orientation blocks for a list of (P, Q)
+// quartets that all belong to class pair (X, Y), X >= Y.
+void run_store(tlno_gpu_plan* pl, int X, int Y, const std::vector<int2>& qs, const std::vector<long long>& off,
+               double* d_out) {
+    if (qs.empty()) return;
+    DeviceCtx& ctx = device_ctx(pl->device);
+    int2* dq = nullptr;
+    long long* doff = nullptr;
+    HIP_OK(hipMalloc(&dq, qs.size() * sizeof(int2)));
+    HIP_OK(hipMalloc(&doff, off.size() * sizeof(long long)));
+    HIP_OK(hipMemcpyAsync(dq, qs.data(), qs.size() * sizeof(int2), hipMemcpyHostToDevice, pl->stream));
+    HIP_OK(hipMemcpyAsync(doff, off.data(), off.size() * sizeof(long long), hipMemcpyHostToDevice, pl->stream));
+    StoreArgs a;
+    a.pd = pl->pair_data();
+    a.tb = ctx.tb;
+    a.quartets = dq;
+    a.off = doff;
+    a.n = static_cast<int>(qs.size());
+    a.out = d_out;
+    const ClassKernels& k = g_kernels[X][Y];
+    for (int ch = 0; ch < k.nchunk; ++ch) launch_store(k, ch, a, pl->stream);
+    HIP_OK(hipStreamSynchronize(pl->stream));
+    HIP_OK(hipFree(dq));
+    HIP_OK(hipFree(doff));
+}
+
+void upload_pairs(tlno_gpu_plan* pl, const std::vector<int>& order,
+                  const std::vector<std::vector<double4>>& prims_P, const std::vector<std::vector<double>>& prims_w,
+                  const std::vector<std::vector<double>>& prims_ab) {
+    const int np = static_cast<int>(order.size());
+    std::vector<double> A(3 * np), AB(3 * np), pw, pab;
+    std::vector<int> ao(2 * np), poff(np), pn(np), diag(np), pL(2 * np);
+    std::vector<double4> pP;
+    std::vector<int> gA(np), gB(np);
+    for (int k = 0; k < np; ++k) {
+        const int src = order[k];
+        const GroupH& Ga = pl->groups[pl->pgA[src]];
+        const GroupH& Gb = pl->groups[pl->pgB[src]];
+        for (int x = 0; x < 3; ++x) {
+            A[3 * k + x] = Ga.C[x];
+            AB[3 * k + x] = Ga.C[x] - Gb.C[x];
+        }
+        ao[2 * k] = Ga.int_base;
+        ao[2 * k + 1] = Gb.int_base;
+        poff[k] = static_cast<int>(pP.size());
+        pn[k] = static_cast<int>(prims_P[src].size());
+        diag[k] = pl->pgA[src] == pl->pgB[src];
+        pP.insert(pP.end(), prims_P[src].begin(), prims_P[src].end());
+        pw.insert(pw.end(), prims_w[src].begin(), prims_w[src].end());
+        pab.insert(pab.end(), prims_ab[src].begin(), prims_ab[src].end());
+        pL[2 * k] = Ga.L;
+        pL[2 * k + 1] = Gb.L;
+        gA[k] = pl->pgA[src];
+        gB[k] = pl->pgB[src];
+    }
+    pl->pgA = gA;
+    pl->pgB = gB;
+    pl->pn_host = pn;
+    pl->nprim_pairs = static_cast<long long>(pP.size());
+    dev_upload(pl->dA, A);
+    dev_upload(pl->dAB, AB);
+    dev_upload(pl->dao, ao);
+    dev_upload(pl->dpoff, poff);
+    dev_upload(pl->dpn, pn);
+    dev_upload(pl->ddiag, diag);
+    dev_upload(pl->dpP, pP);
+    dev_upload(pl->dpw, pw);
+    dev_upload(pl->dpexp, pab);
+    dev_upload(pl->dpL, pL);
+}
+
+void build_plan(tlno_gpu_plan* pl, int n, const double* centers, const int* ang, const int* nprim,
+                const double* exps, const double* coefs) {
+    DeviceCtx& ctx = device_ctx(pl->device);
+    (void)ctx;
+    HIP_OK(hipSetDevice(pl->device));
+    HIP_OK(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking));
+    pl->groups = build_groups(n, centers, ang, nprim, exps, coefs);
+    const int ng = static_cast<int>(pl->groups.size());
+    int ib = 0, sb = 0;
+    for (auto& G : pl->groups) {
+        G.int_base = ib;
+        G.sph_base = sb;
+        ib += ncart(G.L);
+        sb += 2 * G.L + 1;
+    }
+    pl->Mcart = ib;
+    pl->Msph = sb;
+    if (n != ib) throw std::runtime_error("eri_ao_tensor_hgp: inconsistent component count.");
+    if (sb > 65535 || ib > 65535)
+        throw std::runtime_error("eri_ao_tensor_hgp: direct J/K pair tables assume M <= 65535.");
+    pl->Mout = pl->spherical ? pl->Msph : pl->Mcart;
+    pl->int2basis.resize(ib);
+    std::vector<int> iL(ib), iloc(ib), isb(ib);
+    for (const auto& G : pl->groups)
+        for (int k = 0; k < ncart(G.L); ++k) {
+            pl->int2basis[G.int_base + k] = G.comp[k];
+            iL[G.int_base + k] = G.L;
+            iloc[G.int_base + k] = k;
+            isb[G.int_base + k] = G.sph_base;
+        }
+    std::vector<int> sL(sb), sloc(sb), sib(sb);
+    for (const auto& G : pl->groups)
+        for (int s = 0; s < 2 * G.L + 1; ++s) {
+            sL[G.sph_base + s] = G.L;
+            sloc[G.sph_base + s] = s;
+            sib[G.sph_base + s] = G.int_base;
+        }
+    dev_upload(pl->d_int2basis, pl->int2basis);
+    dev_upload(pl->d_intL, iL);
+    dev_upload(pl->d_intloc, iloc);
+    dev_upload(pl->d_intsb, isb);
+    dev_upload(pl->d_sphL, sL);
+    dev_upload(pl->d_sphloc, sloc);
+    dev_upload(pl->d_sphib, sib);
+
+    // shell pairs, oriented with the higher L first (rys_engine.ipp:2443-2460)
+    std::vector<int> pcls;
+    std::vector<std::vector<double4>> prims_P;
+    std::vector<std::vector<double>> prims_w, prims_ab;
+    for (int lg = 0; lg < ng; ++lg)
+        for (int rg = 0; rg <= lg; ++rg) {
+            int a = lg, b = rg;
+            if (pl->groups[b].L > pl->groups[a].L) std::swap(a, b);
+            const GroupH& Ga = pl->groups[a];
+            const GroupH& Gb = pl->groups[b];
+            pl->pgA.push_back(a);
+            pl->pgB.push_back(b);
+            pcls.push_back(class_of(Ga.L, Gb.L));
+            double AB[3], r2 = 0.0;
+            for (int x = 0; x < 3; ++x) {
+                AB[x] = Ga.C[x] - Gb.C[x];
+                r2 += AB[x] * AB[x];
+            }
+            std::vector<double4> P;
+            std::vector<double> w, ab;
+            for (int i = 0; i < Ga.nprim; ++i)
+                for (int j = 0; j < Gb.nprim; ++j) {
+                    const double al = Ga.exps[i], be = Gb.exps[j];
+                    const double cab = Ga.cr[i] * Gb.cr[j];
+                    if (cab == 0.0) continue;
+                    const double p = al + be, ip = 1.0 / p;
+                    const double K = std::exp(-(al * be * ip) * r2);
+                    // P = (a A + b B)/p written as A - (b/p)(A - B): exact for
+                    // one-centre pairs, so parity-forbidden integrals vanish.
+                    double4 v;
+                    v.x = Ga.C[0] - be * ip * AB[0];
+                    v.y = Ga.C[1] - be * ip * AB[1];
+                    v.z = Ga.C[2] - be * ip * AB[2];
+                    v.w = p;
+                    P.push_back(v);
+                    w.push_back(K * cab * ip);
+                    ab.push_back(al);
+                    ab.push_back(be);
+                }
+            prims_P.push_back(std::move(P));
+            prims_w.push_back(std::move(w));
+            prims_ab.push_back(std::move(ab));
+        }
+    const int np = static_cast<int>(pcls.size());
+    pl->npair = np;
+    // order by class (stable)
+    std::vector<int> order(np);
+    for (int i = 0; i < np; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return pcls[x] < pcls[y]; });
+    std::vector<int> cls_sorted(np);
+    for (int k = 0; k < np; ++k) cls_sorted[k] = pcls[order[k]];
+    const std::vector<int> pgA0 = pl->pgA, pgB0 = pl->pgB;
+    upload_pairs(pl, order, prims_P, prims_w, prims_ab);
+    for (int c = 0; c <= 6; ++c)
+        pl->cls_begin[c] =
+            static_cast<int>(std::lower_bound(cls_sorted.begin(), cls_sorted.end(), c) - cls_sorted.begin());
+
+    // Schwarz bounds from the diagonal blocks (ab|ab), on the GPU
+    std::vector<double> Q(np, 0.0);
+    for (int X = 0; X < 6; ++X) {
+        const int b0 = pl->cls_begin[X], b1 = pl->cls_begin[X + 1];
+        if (b1 <= b0) continue;
+        const int LA = (X == 0 ? 0 : (X <= 2 ? 1 : 2)), LB = (X == 0 || X == 1 || X == 3) ? 0 : (X == 2 || X == 4 ? 1 : 2);
+        const int na = ncart(LA), nb = ncart(LB), nab = na * nb;
+        std::vector<int2> qs;
+        std::vector<long long> off;
+        for (int P = b0; P < b1; ++P) {
+            qs.push_back(make_int2(P, P));
+            off.push_back(static_cast<long long>(P - b0) * nab * nab);
+        }
+        double* dout = nullptr;
+        const std::size_t total = static_cast<std::size_t>(b1 - b0) * nab * nab;
+        HIP_OK(hipMalloc(&dout, total * sizeof(double)));
+        run_store(pl, X, X, qs, off, dout);
+        std::vector<double> h(total);
+        HIP_OK(hipMemcpy(h.data(), dout, total * sizeof(double), hipMemcpyDeviceToHost));
+        HIP_OK(hipFree(dout));
+        const C2S& cs = c2s_table();
+        for (int P = b0; P < b1; ++P) {
+            const double* blk = h.data() + static_cast<std::size_t>(P - b0) * nab * nab;
+            double mx = 0.0;
+            if (!pl->spherical) {
+                for (int ab = 0; ab < nab; ++ab) mx = std::max(mx, std::abs(blk[ab * nab + ab]));
+            } else {
+                const int sa = 2 * LA + 1, sbn = 2 * LB + 1;
+                for (int s = 0; s < sa; ++s)
+                    for (int t = 0; t < sbn; ++t) {
+                        double v = 0.0;
+                        for (int a = 0; a < na; ++a)
+                            for (int b = 0; b < nb; ++b) {
+                                const double wab = cs.c[LA][s][a] * cs.c[LB][t][b];
+                                if (wab == 0.0) continue;
+                                for (int c = 0; c < na; ++c)
+                                    for (int d = 0; d < nb; ++d) {
+                                        const double wcd = cs.c[LA][s][c] * cs.c[LB][t][d];
+                                        if (wcd == 0.0) continue;
+                                        v += wab * wcd * blk[(a * nb + b) * nab + c * nb + d];
+                                    }
+                            }
+                        mx = std::max(mx, std::abs(v));
+                    }
+            }
+            Q[P] = std::sqrt(mx);
+        }
+    }
+    // sort inside each class by descending bound
+    std::vector<int> order2(np);
+    for (int i = 0; i < np; ++i) order2[i] = i;
+    for (int c = 0; c < 6; ++c)
+        std::stable_sort(order2.begin() + pl->cls_begin[c], order2.begin() + pl->cls_begin[c + 1],
+                         [&](int x, int y) { return Q[x] > Q[y]; });
+    // compose with the first ordering
+    std::vector<int> final_order(np);
+    for (int k = 0; k < np; ++k) final_order[k] = order[order2[k]];
+    pl->pgA = pgA0;
+    pl->pgB = pgB0;
+    upload_pairs(pl, final_order, prims_P, prims_w, prims_ab);
+    pl->Q.resize(np);
+    for (int k = 0; k < np; ++k) pl->Q[k] = Q[order2[k]];
+
+    const std::size_t mc2 = static_cast<std::size_t>(pl->Mcart) * pl->Mcart;
+    const std::size_t mo2 = static_cast<std::size_t>(pl->Mout) * pl->Mout;
+    HIP_OK(hipMalloc(&pl->dDin, mo2 * sizeof(double)));
+    HIP_OK(hipMalloc(&pl->dDint, mc2 * sizeof(double)));
+    HIP_OK(hipMalloc(&pl->dJacc, mc2 * sizeof(double)));
+    HIP_OK(hipMalloc(&pl->dKacc, mc2 * sizeof(double)));
+    HIP_OK(hipMalloc(&pl->dJout, mo2 * sizeof(double)));
+    HIP_OK(hipMalloc(&pl->dKout, mo2 * sizeof(double)));
+}
+
+const TaskSet& get_tasks(tlno_gpu_plan* pl, double thr) {
+    auto it = pl->tasks.find(thr);
+    if (it != pl->tasks.end()) return it->second;
+    TaskSet ts;
+    std::vector<int4> all;
+    std::vector<long long> pref(static_cast<std::size_t>(pl->npair) + 1, 0);
+    for (int k = 0; k < pl->npair; ++k) pref[k + 1] = pref[k] + pl->pn_host[k];
+    for (int X = 0; X < 6; ++X)
+        for (int Y = 0; Y <= X; ++Y) {
+            ts.off[X][Y] = static_cast<long long>(all.size());
+            const int b0 = pl->cls_begin[X], b1 = pl->cls_begin[X + 1];
+            const int k0 = pl->cls_begin[Y], k1 = pl->cls_begin[Y + 1];
+            if (b1 > b0 && k1 > k0) {
+                for (int P = b0; P < b1; ++P) {
+                    int end = k1;
+                    if (thr > 0.0) {
+                        // largest prefix with Q_P * Q_Q >= thr (bounds sorted descending)
+                        int lo = k0, hi = k1;
+                        while (lo < hi) {
+                            const int mid = (lo + hi) / 2;
+                            if (pl->Q[P] * pl->Q[mid] >= thr) lo = mid + 1;
+                            else hi = mid;
+                        }
+                        end = lo;
+                    }
+                    if (X == Y) end = std::min(end, P + 1);
+                    for (int q0 = k0; q0 < end; q0 += 64) all.push_back(make_int4(P, q0, end, 0));
+                    const long long nq = std::max(0, end - k0);
+                    ts.shell_quartets += nq;
+                    if (end > k0) ts.prim_quartets += (pref[end] - pref[k0]) * pl->pn_host[P];
+                }
+            }
+            ts.cnt[X][Y] = static_cast<long long>(all.size()) - ts.off[X][Y];
+        }
+    if (!all.empty()) {
+        HIP_OK(hipMalloc(&ts.d_tasks, all.size() * sizeof(int4)));
+        HIP_OK(hipMemcpy(ts.d_tasks, all.data(), all.size() * sizeof(int4), hipMemcpyHostToDevice));
+    }
+    return pl->tasks.emplace(thr, ts).first->second;
+}
+
+void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, double thr) {
+    DeviceCtx& ctx = device_ctx(pl->device);
+    HIP_OK(hipSetDevice(pl->device));
+    if (!std::isfinite(thr) || thr < 0.0)
+        throw std::runtime_error("eri_ao_tensor_hgp: schwarz_threshold must be finite and >= 0.");
+    const TaskSet& ts = get_tasks(pl, thr);
+    const hipStream_t st = pl->stream;
+    const long mc2 = static_cast<long>(pl->Mcart) * pl->Mcart;
+    const long mo2 = static_cast<long>(pl->Mout) * pl->Mout;
+    C2SArgs c2s;
+    std::memcpy(c2s.c, c2s_table().c, sizeof(c2s.c));
+    if (pl->spherical)
+        k_prep_sph<<<grid_for(mc2, 256), 256, 0, st>>>(dD, pl->dDint, pl->d_intL, pl->d_intloc, pl->d_intsb,
+                                                       pl->Mcart, pl->Msph, c2s);
+    else
+        k_prep_cart<<<grid_for(mc2, 256), 256, 0, st>>>(dD, pl->dDint, pl->d_int2basis, pl->Mcart);
+    HIP_OK(hipGetLastError());
+    if (dJ) k_fill<<<grid_for(mc2, 256), 256, 0, st>>>(pl->dJacc, mc2, 0.0);
+    if (dK) k_fill<<<grid_for(mc2, 256), 256, 0, st>>>(pl->dKacc, mc2, 0.0);
+    JKArgs a;
+    a.pd = pl->pair_data();
+    a.tb = ctx.tb;
+    a.D = pl->dDint;
+    a.J = pl->dJacc;
+    a.K = pl->dKacc;
+    a.M = pl->Mcart;
+    a.do_j = dJ != nullptr;
+    a.do_k = dK != nullptr;
+    if (a.do_j || a.do_k) {
+        // heaviest classes first
+        for (int X = 5; X >= 0; --X)
+            for (int Y = X; Y >= 0; --Y) {
+                const long long nt = ts.cnt[X][Y];
+                if (nt == 0) continue;
+                a.tasks = ts.d_tasks + ts.off[X][Y];
+                a.ntasks = static_cast<int>(nt);
+                const ClassKernels& k = g_kernels[X][Y];
+                for (int ch = 0; ch < k.nchunk; ++ch) launch_jk(k, ch, a, nt, st);
+            }
+    }
+    if (dJ) {
+        if (pl->spherical)
+            k_fin_sph<<<grid_for(mo2, 256), 256, 0, st>>>(pl->dJacc, dJ, pl->d_sphL, pl->d_sphloc, pl->d_sphib,
+                                                          pl->Mcart, pl->Msph, c2s);
+        else
+            k_fin_cart<<<grid_for(mo2, 256), 256, 0, st>>>(pl->dJacc, dJ, pl->d_int2basis, pl->Mcart);
+    }
+    if (dK) {
+        if (pl->spherical)
+            k_fin_sph<<<grid_for(mo2, 256), 256, 0, st>>>(pl->dKacc, dK, pl->d_sphL, pl->d_sphloc, pl->d_sphib,
+                                                          pl->Mcart, pl->Msph, c2s);
+        else
+            k_fin_cart<<<grid_for(mo2, 256), 256, 0, st>>>(pl->dKacc, dK, pl->d_int2basis, pl->Mcart);
+    }
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(st));
+    pl->last_shell_quartets = ts.shell_quartets;
+    pl->last_prim_quartets = ts.prim_quartets;
+}
+
+void validate_density(const double* D, int M) {
+    const double tol = 128.0 * 2.220446049250313e-16;
+    for (int i = 0; i < M; ++i)
+        for (int j = 0; j <= i; ++j) {
+            const double a = D[static_cast<std::size_t>(i) * M + j], b = D[static_cast<std::size_t>(j) * M + i];
+            if (!std::isfinite(a) || !std::isfinite(b))
+                throw std::runtime_error("eri_ao_tensor_hgp: density contains a non-finite value.");
+            const double sc = std::max({1.0, std::abs(a), std::abs(b)});
+            if (std::abs(a - b) > tol * sc) throw std::runtime_error("eri_ao_tensor_hgp: density must be symmetric.");
+        }
+}
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    } catch (...) {
+        g_err = "tlno_gpu: unknown error";
+        return 1;
+    }
+}
+
+}  // namespace tlno_amd
+
+tlno_gpu_plan::~tlno_gpu_plan() {
+    auto f = [](void* p) {
+        if (p) (void)hipFree(p);
+    };
+    f(dA); f(dAB); f(dpw); f(dao); f(dpoff); f(dpn); f(ddiag); f(dpP); f(dpexp); f(dpL);
+    f(d_int2basis); f(d_intL); f(d_intloc); f(d_intsb); f(d_sphL); f(d_sphloc); f(d_sphib);
+    f(dDin); f(dDint); f(dJacc); f(dKacc); f(dJout); f(dKout);
+    for (auto& kv : tasks) f(kv.second.d_tasks);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+using namespace tlno_amd;
+
+extern "C" {
+
+const char* tlno_gpu_last_error(void) { return g_err.c_str(); }
+
+int tlno_gpu_init(int device) {
+    return guarded([&] {
+        int n = 0;
+        HIP_OK(hipGetDeviceCount(&n));
+        if (device < 0 || device >= n) throw std::runtime_error("tlno_gpu: no such HIP device");
+        g_device = device;
+        device_ctx(device);
+    });
+}
+
+int tlno_gpu_plan_create(int n_comp, const double* centers, const int* ang, const int* nprim,
+                         const double* exps, const double* coefs, int spherical, tlno_gpu_plan** out) {
+    return guarded([&] {
+        if (!out) throw std::runtime_error("tlno_gpu: null output handle");
+        *out = nullptr;
+        if (n_comp <= 0 || !centers || !ang || !nprim || !exps || !coefs)
+            throw std::runtime_error("eri_ao_tensor_hgp: empty or null basis.");
+        auto pl = std::make_unique<tlno_gpu_plan>();
+        pl->device = g_device;
+        pl->spherical = spherical ? 1 : 0;
+        build_plan(pl.get(), n_comp, centers, ang, nprim, exps, coefs);
+        *out = pl.release();
+    });
+}
+
+void tlno_gpu_plan_destroy(tlno_gpu_plan* plan) { delete plan; }
+
+int tlno_gpu_plan_info(const tlno_gpu_plan* pl, int* M, int* n_groups, int* n_pairs, long long* n_prim_pairs) {
+    return guarded([&] {
+        if (!pl) throw std::runtime_error("jk_direct: null J/K plan backend.");
+        if (M) *M = pl->Mout;
+        if (n_groups) *n_groups = static_cast<int>(pl->groups.size());
+        if (n_pairs) *n_pairs = pl->npair;
+        if (n_prim_pairs) *n_prim_pairs = pl->nprim_pairs;
+    });
+}
+
+int tlno_gpu_plan_groups(const tlno_gpu_plan* pl, int max_groups, int* L, int* comps) {
+    return guarded([&] {
+        if (!pl) throw std::runtime_error("jk_direct: null J/K plan backend.");
+        const int ng = static_cast<int>(pl->groups.size());
+        if (ng > max_groups) throw std::runtime_error("tlno_gpu: group buffer too small");
+        for (int g = 0; g < ng; ++g) {
+            if (L) L[g] = pl->groups[g].L;
+            if (comps)
+                for (int k = 0; k < 15; ++k) comps[15 * g + k] = k < ncart(pl->groups[g].L) ? pl->groups[g].comp[k] : -1;
+        }
+    });
+}
+
+int tlno_gpu_build_jk(tlno_gpu_plan* pl, const double* density, double* J, double* K, double schwarz) {
+    return guarded([&] {
+        if (!pl) throw std::runtime_error("jk_direct: null J/K plan backend.");
+        if (!density) throw std::runtime_error("eri_ao_tensor_hgp: density must be spherical M x M row-major.");
+        const int M = pl->Mout;
+        validate_density(density, M);
+        const std::size_t bytes = static_cast<std::size_t>(M) * M * sizeof(double);
+        HIP_OK(hipSetDevice(pl->device));
+        HIP_OK(hipMemcpyAsync(pl->dDin, density, bytes, hipMemcpyHostToDevice, pl->stream));
+        jk_device(pl, pl->dDin, J ? pl->dJout : nullptr, K ? pl->dKout : nullptr, schwarz);
+        if (J) HIP_OK(hipMemcpy(J, pl->dJout, bytes, hipMemcpyDeviceToHost));
+        if (K) HIP_OK(hipMemcpy(K, pl->dKout, bytes, hipMemcpyDeviceToHost));
+    });
+}
+
+int tlno_gpu_build_jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, double schwarz) {
+    return guarded([&] {
+        if (!pl) throw std::runtime_error("jk_direct: null J/K plan backend.");
+        if (!dD) throw std::runtime_error("eri_ao_tensor_hgp: null density.");
+        jk_device(pl, dD, dJ, dK, schwarz);
+    });
+}
+
+int tlno_gpu_last_build_stats(const tlno_gpu_plan* pl, long long* sq, long long* pq) {
+    return guarded([&] {
+        if (!pl) throw std::runtime_error("jk_direct: null J/K plan backend.");
+        if (sq) *sq = pl->last_shell_quartets;
+        if (pq) *pq = pl->last_prim_quartets;
+    });
+}
+
+int tlno_gpu_eri_cart_blocks(tlno_gpu_plan* pl, int n, const int* g4, const long long* offsets, double* out) {
+    return guarded([&] {
+        if (!pl) throw std::runtime_error("jk_direct: null J/K plan backend.");
+        const int ng = static_cast<int>(pl->groups.size());
+        // pair index of each unordered group pair
+        std::map<std::pair<int, int>, int> pidx;
+        for (int k = 0; k < pl->npair; ++k) pidx[{pl->pgA[k], pl->pgB[k]}] = k;
+        auto cls_of_pair = [&](int P) {
+            int c = 0;
+            while (!(P >= pl->cls_begin[c] && P < pl->cls_begin[c + 1])) ++c;
+            return c;
+        };
+        struct Item {
+            int i, P, Q;
+            bool swap_bra, swap_ket, swap_pq;
+        };
+        std::vector<Item> items;
+        for (int i = 0; i < n; ++i) {
+            int g[4] = {g4[4 * i], g4[4 * i + 1], g4[4 * i + 2], g4[4 * i + 3]};
+            for (int k = 0; k < 4; ++k)
+                if (g[k] < 0 || g[k] >= ng) throw std::runtime_error("tlno_gpu: group index out of range");
+            auto find = [&](int x, int y, bool& sw) {
+                auto it = pidx.find({x, y});
+                if (it != pidx.end()) {
+                    sw = false;
+                    return it->second;
+                }
+                sw = true;
+                return pidx.at({y, x});
+            };
+            Item it;
+            it.i = i;
+            int P = find(g[0], g[1], it.swap_bra);
+            int Q = find(g[2], g[3], it.swap_ket);
+            it.swap_pq = cls_of_pair(P) < cls_of_pair(Q);
+            if (it.swap_pq) std::swap(P, Q);
+            it.P = P;
+            it.Q = Q;
+            items.push_back(it);
+        }
+        // evaluate per class pair
+        std::map<std::pair<int, int>, std::vector<int>> by_class;
+        for (int k = 0; k < static_cast<int>(items.size()); ++k)
+            by_class[{cls_of_pair(items[k].P), cls_of_pair(items[k].Q)}].push_back(k);
+        for (auto& kv : by_class) {
+            const int X = kv.first.first, Y = kv.first.second;
+            std::vector<int2> qs;
+            std::vector<long long> off;
+            long long total = 0;
+            for (int k : kv.second) {
+                qs.push_back(make_int2(items[k].P, items[k].Q));
+                off.push_back(total);
+                const int LA = pl->groups[pl->pgA[items[k].P]].L, LB = pl->groups[pl->pgB[items[k].P]].L;
+                const int LC = pl->groups[pl->pgA[items[k].Q]].L, LD = pl->groups[pl->pgB[items[k].Q]].L;
+                total += static_cast<long long>(ncart(LA)) * ncart(LB) * ncart(LC) * ncart(LD);
+            }
+            double* dout = nullptr;
+            HIP_OK(hipMalloc(&dout, static_cast<std::size_t>(total) * sizeof(double)));
+            run_store(pl, X, Y, qs, off, dout);
+            std::vector<double> h(static_cast<std::size_t>(total));
+            HIP_OK(hipMemcpy(h.data(), dout, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+            HIP_OK(hipFree(dout));
+            for (std::size_t j = 0; j < kv.second.size(); ++j) {
+                const Item& it = items[kv.second[j]];
+                const GroupH* G[4] = {&pl->groups[pl->pgA[it.P]], &pl->groups[pl->pgB[it.P]],
+                                      &pl->groups[pl->pgA[it.Q]], &pl->groups[pl->pgB[it.Q]]};
+                const int n0 = ncart(G[0]->L), n1 = ncart(G[1]->L), n2 = ncart(G[2]->L), n3 = ncart(G[3]->L);
+                const double* src = h.data() + off[j];
+                // requested order of the four shells: map back through the swaps
+                // stored block axes: (A_P, B_P, A_Q, B_Q)
+                // requested axes r0..r3 correspond to stored axes:
+                int map[4];
+                {
+                    int bra[2] = {0, 1}, ket[2] = {2, 3};
+                    if (it.swap_pq) {
+                        bra[0] = 2; bra[1] = 3; ket[0] = 0; ket[1] = 1;
+                    }
+                    map[0] = it.swap_bra ? bra[1] : bra[0];
+                    map[1] = it.swap_bra ? bra[0] : bra[1];
+                    map[2] = it.swap_ket ? ket[1] : ket[0];
+                    map[3] = it.swap_ket ? ket[0] : ket[1];
+                }
+                const int dims[4] = {n0, n1, n2, n3};
+                const int req_g[4] = {g4[4 * it.i], g4[4 * it.i + 1], g4[4 * it.i + 2], g4[4 * it.i + 3]};
+                int rd[4];
+                for (int k = 0; k < 4; ++k) rd[k] = dims[map[k]];
+                // component order: basis-appearance order within each group
+                std::vector<int> ord[4];
+                for (int k = 0; k < 4; ++k) {
+                    const GroupH& Gr = pl->groups[req_g[k]];
+                    std::vector<std::pair<int, int>> v;
+                    for (int c = 0; c < ncart(Gr.L); ++c) v.push_back({Gr.comp[c], c});
+                    std::sort(v.begin(), v.end());
+                    for (auto& x : v) ord[k].push_back(x.second);  // local index of k-th appearing component
+                }
+                double* dst = out + offsets[it.i];
+                int e[4];
+                for (e[0] = 0; e[0] < rd[0]; ++e[0])
+                    for (e[1] = 0; e[1] < rd[1]; ++e[1])
+                        for (e[2] = 0; e[2] < rd[2]; ++e[2])
+                            for (e[3] = 0; e[3] < rd[3]; ++e[3]) {
+                                int s[4];
+                                for (int k = 0; k < 4; ++k) s[map[k]] = ord[k][e[k]];
+                                dst[((static_cast<long long>(e[0]) * rd[1] + e[1]) * rd[2] + e[2]) * rd[3] + e[3]] =
+                                    src[((static_cast<long long>(s[0]) * n1 + s[1]) * n2 + s[2]) * n3 + s[3]];
+                            }
+            }
+        }
+    });
+}
+
+int tlno_gpu_onee(tlno_gpu_plan* pl, int n_nuc, const double* Z, const double* R, double* S, double* T,
+                  double* V) {
+    return guarded([&] {
+        if (!pl) throw std::runtime_error("jk_direct: null J/K plan backend.");
+        if (n_nuc < 0 || (n_nuc > 0 && (!Z || !R))) throw std::runtime_error("tlno_gpu: invalid nuclei");
+        for (int c = 0; c < n_nuc; ++c)
+            if (!std::isfinite(Z[c]) || !std::isfinite(R[3 * c]) || !std::isfinite(R[3 * c + 1]) ||
+                !std::isfinite(R[3 * c + 2]))
+                throw std::runtime_error("tlno_gpu: nuclear charges and centres must be finite");
+        DeviceCtx& ctx = device_ctx(pl->device);
+        HIP_OK(hipSetDevice(pl->device));
+        const hipStream_t st = pl->stream;
+        const long mc2 = static_cast<long>(pl->Mcart) * pl->Mcart;
+        const long mo2 = static_cast<long>(pl->Mout) * pl->Mout;
+        double *dR = nullptr, *dZ = nullptr, *dX = nullptr, *dO = nullptr;
+        HIP_OK(hipMalloc(&dR, std::max(1, 3 * n_nuc) * sizeof(double)));
+        HIP_OK(hipMalloc(&dZ, std::max(1, n_nuc) * sizeof(double)));
+        HIP_OK(hipMalloc(&dX, 3 * mc2 * sizeof(double)));
+        HIP_OK(hipMalloc(&dO, 3 * mo2 * sizeof(double)));
+        if (n_nuc > 0) {
+            HIP_OK(hipMemcpyAsync(dR, R, 3 * n_nuc * sizeof(double), hipMemcpyHostToDevice, st));
+            HIP_OK(hipMemcpyAsync(dZ, Z, n_nuc * sizeof(double), hipMemcpyHostToDevice, st));
+        }
+        k_fill<<<grid_for(3 * mc2, 256), 256, 0, st>>>(dX, 3 * mc2, 0.0);
+        launch_onee(pl->pair_data(), ctx.tb, pl->dpL, pl->dpexp, dR, dZ, n_nuc, pl->npair, pl->Mcart, dX, dX + mc2,
+                    dX + 2 * mc2, st);
+        C2SArgs c2s;
+        std::memcpy(c2s.c, c2s_table().c, sizeof(c2s.c));
+        for (int m = 0; m < 3; ++m) {
+            if (pl->spherical)
+                k_fin_sph<<<grid_for(mo2, 256), 256, 0, st>>>(dX + m * mc2, dO + m * mo2, pl->d_sphL, pl->d_sphloc,
+                                                              pl->d_sphib, pl->Mcart, pl->Msph, c2s);
+            else
+                k_fin_cart<<<grid_for(mo2, 256), 256, 0, st>>>(dX + m * mc2, dO + m * mo2, pl->d_int2basis,
+                                                               pl->Mcart);
+        }
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipStreamSynchronize(st));
+        double* outs[3] = {S, T, V};
+        for (int m = 0; m < 3; ++m)
+            if (outs[m]) HIP_OK(hipMemcpy(outs[m], dO + m * mo2, mo2 * sizeof(double), hipMemcpyDeviceToHost));
+        HIP_OK(hipFree(dR));
+        HIP_OK(hipFree(dZ));
+        HIP_OK(hipFree(dX));
+        HIP_OK(hipFree(dO));
+    });
+}
+
+int tlno_gpu_boys(int n, const double* T, int mmax, double* out) {
+    return guarded([&] {
+        if (n <= 0) return;
+        if (mmax < 0 || mmax > 16) throw std::runtime_error("tlno_gpu: boys order must be 0..16");
+        DeviceCtx& ctx = device_ctx(g_device);
+        HIP_OK(hipSetDevice(g_device));
+        double *dT = nullptr, *dO = nullptr;
+        HIP_OK(hipMalloc(&dT, n * sizeof(double)));
+        HIP_OK(hipMalloc(&dO, static_cast<std::size_t>(n) * (mmax + 1) * sizeof(double)));
+        HIP_OK(hipMemcpy(dT, T, n * sizeof(double), hipMemcpyHostToDevice));
+        const unsigned g = grid_for(n, 128);
+        switch (mmax) {
+#define TLNO_BOYS_CASE(m) \
+    case m: k_boys<m><<<g, 128>>>(ctx.tb, dT, dO, n); break;
+            TLNO_BOYS_CASE(0) TLNO_BOYS_CASE(1) TLNO_BOYS_CASE(2) TLNO_BOYS_CASE(3) TLNO_BOYS_CASE(4)
+            TLNO_BOYS_CASE(5) TLNO_BOYS_CASE(6) TLNO_BOYS_CASE(7) TLNO_BOYS_CASE(8) TLNO_BOYS_CASE(9)
+            TLNO_BOYS_CASE(10) TLNO_BOYS_CASE(11) TLNO_BOYS_CASE(12) TLNO_BOYS_CASE(13) TLNO_BOYS_CASE(14)
+            TLNO_BOYS_CASE(15) TLNO_BOYS_CASE(16)
+#undef TLNO_BOYS_CASE
+        }
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpy(out, dO, static_cast<std::size_t>(n) * (mmax + 1) * sizeof(double), hipMemcpyDeviceToHost));
+        HIP_OK(hipFree(dT));
+        HIP_OK(hipFree(dO));
+    });
+}
+
+int tlno_gpu_rys(int N, int n, const double* T, double* roots, double* weights) {
+    return guarded([&] {
+        if (n <= 0) return;
+        if (N < 1 || N > RYS_NMAX) throw std::runtime_error("tlno_gpu: Rys order must be 1..5");
+        DeviceCtx& ctx = device_ctx(g_device);
+        HIP_OK(hipSetDevice(g_device));
+        double *dT = nullptr, *dR = nullptr, *dW = nullptr;
+        HIP_OK(hipMalloc(&dT, n * sizeof(double)));
+        HIP_OK(hipMalloc(&dR, static_cast<std::size_t>(n) * N * sizeof(double)));
+        HIP_OK(hipMalloc(&dW, static_cast<std::size_t>(n) * N * sizeof(double)));
+        HIP_OK(hipMemcpy(dT, T, n * sizeof(double), hipMemcpyHostToDevice));
+        const unsigned g = grid_for(n, 128);
+        switch (N) {
+            case 1: k_rys<1><<<g, 128>>>(ctx.tb, dT, dR, dW, n); break;
+            case 2: k_rys<2><<<g, 128>>>(ctx.tb, dT, dR, dW, n); break;
+            case 3: k_rys<3><<<g, 128>>>(ctx.tb, dT, dR, dW, n); break;
+            case 4: k_rys<4><<<g, 128>>>(ctx.tb, dT, dR, dW, n); break;
+            case 5: k_rys<5><<<g, 128>>>(ctx.tb, dT, dR, dW, n); break;
+        }
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpy(roots, dR, static_cast<std::size_t>(n) * N * sizeof(double), hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(weights, dW, static_cast<std::size_t>(n) * N * sizeof(double), hipMemcpyDeviceToHost));
+        HIP_OK(hipFree(dT));
+        HIP_OK(hipFree(dR));
+        HIP_OK(hipFree(dW));
+    });
+}
+
+}  // extern "C"

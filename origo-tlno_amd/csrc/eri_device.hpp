@@ -1,0 +1,514 @@
+// Device side of the Rys-quadrature ERI engine for CDNA4 (gfx950).
+//
+// One wave (64 lanes) owns one task: a fixed bra shell pair P and up to 64
+// consecutive ket shell pairs Q of one class, Schwarz-sorted so that the
+// surviving kets of P form a prefix.  Each lane evaluates one contracted
+// shell quartet (P|Q) entirely in registers:
+//
+//   for bra primitive pair (wave-uniform) x ket primitive pair (lane):
+//     T = rho |PQ|^2, Rys roots/weights (Boys for N = 1, piecewise
+//     polynomials + Hermite asymptote for N >= 2),
+//     per root and axis: 2D VRR G[e][f], ket HRR, bra HRR (all unrolled at
+//     compile time for the class), then acc[o] += Ix * Iy * (w Iz).
+//
+// and then digests its block against the density: the J contribution of the
+// shared bra pair is reduced across the wave with cross-lane shuffles before
+// one atomic per component; the ket-side J and the four exchange blocks go
+// out as FP64 atomics into pair-space accumulators Jacc/Kacc, which a
+// finalize kernel symmetrizes (J = Jacc + Jacc^T, K = Kacc + Kacc^T).
+//
+// Classes whose full Cartesian block would not fit in registers are split
+// into compile-time "chunks": a fixed bra component AC and/or ket component
+// CC per kernel instance, so a lane accumulates at most 81 outputs.
+//
+// The per-primitive-quartet formulas follow the reference's generic Rys path
+// (integrals/two_electron/rys/rys_engine.ipp:13540-13628, HRR 3155-3363):
+//   B00 = t^2/(2(p+q)), B10 = (1 - q t^2/(p+q))/(2p), B01 = (1 - p t^2/(p+q))/(2q),
+//   C00 = (P-A) - q/(p+q) PQ t^2, D00 = (Q-C) + p/(p+q) PQ t^2,
+//   prefactor 2 pi^{5/2} w_ab w_cd / sqrt(p+q), w = K c_a c_b / p.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "tables.hpp"
+
+namespace tlno_amd {
+
+__host__ __device__ constexpr int ncart(int L) { return (L + 1) * (L + 2) / 2; }
+
+template <int L>
+struct CartT {
+    int e[3][ncart(L)];
+    double nang[ncart(L)];  // 1/sqrt((2lx-1)!!(2ly-1)!!(2lz-1)!!)
+};
+
+__host__ __device__ constexpr int dfact2m1(int n) {
+    int v = 1;
+    for (int k = 2 * n - 1; k > 0; k -= 2) v *= k;
+    return v;
+}
+
+// constexpr 1/sqrt for the few values that occur ((2l-1)!! products, l <= 4)
+__host__ __device__ constexpr double inv_sqrt_small(int v) {
+    return v == 1 ? 1.0
+         : v == 3 ? 0.57735026918962576451
+         : v == 9 ? 1.0 / 3.0
+         : v == 15 ? 0.25819888974716112568
+         : v == 105 ? 0.09759000729485332666
+         : v == 45 ? 0.14907119849998597976
+         : v == 225 ? 1.0 / 15.0
+         : 0.0;
+}
+
+template <int L>
+__host__ __device__ constexpr CartT<L> make_cart() {
+    CartT<L> t{};
+    int k = 0;
+    for (int x = L; x >= 0; --x)
+        for (int y = L - x; y >= 0; --y) {
+            t.e[0][k] = x;
+            t.e[1][k] = y;
+            t.e[2][k] = L - x - y;
+            t.nang[k] = inv_sqrt_small(dfact2m1(x) * dfact2m1(y) * dfact2m1(L - x - y));
+            ++k;
+        }
+    return t;
+}
+
+// ---------------------------------------------------------------- tables ---
+struct DevTables {
+    const double* boys;   // [BOYS_NGRID][BOYS_MMAX+1]
+    const double* rys;    // polynomial pieces, see tables.hpp
+    int rys_off[RYS_NMAX + 2];
+    double herm_r2[RYS_NMAX + 1][RYS_NMAX];
+    double herm_w[RYS_NMAX + 1][RYS_NMAX];
+};
+
+// F_0..F_MMAX at T
+template <int MMAX>
+__device__ __forceinline__ void boys_all(const DevTables& tb, double T, double* F) {
+    if (T < BOYS_TCUT) {
+        const int k = static_cast<int>(T * 8.0 + 0.5);
+        const double dd = k * 0.125 - T;  // F_m(T) = sum_j F_{m+j}(T_k) dd^j / j!
+        const double* row = tb.boys + k * (BOYS_MMAX + 1);
+#pragma unroll
+        for (int m = 0; m <= MMAX; ++m) {
+            double v = row[m + BOYS_TAYLOR - 1];
+#pragma unroll
+            for (int j = BOYS_TAYLOR - 1; j >= 1; --j) v = row[m + j - 1] + v * dd * (1.0 / j);
+            F[m] = v;
+        }
+    } else {
+        const double iT = 1.0 / T;
+        double f = 0.88622692545275801365 * sqrt(iT);  // sqrt(pi)/2
+        F[0] = f;
+        if (MMAX > 0) {
+            const double ex = exp(-T);
+            const double h = 0.5 * iT;
+#pragma unroll
+            for (int m = 0; m < MMAX; ++m) {
+                f = ((2 * m + 1) * f - ex) * h;
+                F[m + 1] = f;
+            }
+        }
+    }
+}
+
+template <int N>
+__device__ __forceinline__ void rys_roots(const DevTables& tb, double T, double* r, double* w) {
+    if constexpr (N == 1) {
+        double F[2];
+        boys_all<1>(tb, T, F);
+        r[0] = F[1] / F[0];
+        w[0] = F[0];
+    } else {
+        constexpr int ttab[6] = {0, 41, 49, 55, 61, 67};
+        if (T < ttab[N]) {
+            const int k = static_cast<int>(T);
+            const double u = T - k - 0.5;
+            const double* c = tb.rys + tb.rys_off[N] + k * (2 * N * RYS_NCOEF);
+#pragma unroll
+            for (int v = 0; v < 2 * N; ++v) {
+                const double* cv = c + v * RYS_NCOEF;
+                double s = cv[RYS_NCOEF - 1];
+#pragma unroll
+                for (int p = RYS_NCOEF - 2; p >= 0; --p) s = fma(s, u, cv[p]);
+                if (v < N) r[v] = s;
+                else w[v - N] = s;
+            }
+        } else {
+            const double iT = 1.0 / T;
+            const double isT = sqrt(iT);
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                r[i] = tb.herm_r2[N][i] * iT;
+                w[i] = tb.herm_w[N][i] * isT;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------ pair data ---
+struct PairData {
+    const double* A;        // [npair][3] centre of the bra-side (higher-L) shell
+    const double* AB;       // [npair][3] A - B
+    const int* ao;          // [npair][2] internal Cartesian AO base of A and B
+    const int* poff;        // [npair] first primitive pair
+    const int* pn;          // [npair] primitive pair count
+    const double4* pP;      // [nprim] {Px, Py, Pz, p}
+    const double* pw;       // [nprim] K c_a c_b / p
+    const int* diag;        // [npair] 1 if A and B are the same shell
+};
+
+// ----------------------------------------------------- one shell quartet ---
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void axis_table(double C00, double D00, double B00, double B10, double B01,
+                                           double g00, double AB, double CD,
+                                           double (&I)[LA + 1][LB + 1][LC + 1][LD + 1]) {
+    constexpr int LAB = LA + LB, LCD = LC + LD;
+    double G[LAB + 1][LCD + 1];
+    G[0][0] = g00;
+    if constexpr (LAB > 0) G[1][0] = C00 * g00;
+#pragma unroll
+    for (int e = 1; e < LAB; ++e) G[e + 1][0] = C00 * G[e][0] + (e * B10) * G[e - 1][0];
+#pragma unroll
+    for (int f = 0; f < LCD; ++f) {
+#pragma unroll
+        for (int e = 0; e <= LAB; ++e) {
+            double v = D00 * G[e][f];
+            if (e > 0) v = fma(e * B00, G[e - 1][f], v);
+            if (f > 0) v = fma(f * B01, G[e][f - 1], v);
+            G[e][f + 1] = v;
+        }
+    }
+    // ket HRR: H[e][c][d], (c, d+1) = (c+1, d) + CD (c, d)
+    double H[LAB + 1][LCD + 1][LD + 1];
+#pragma unroll
+    for (int e = 0; e <= LAB; ++e) {
+#pragma unroll
+        for (int c = 0; c <= LCD; ++c) H[e][c][0] = G[e][c];
+#pragma unroll
+        for (int d = 0; d < LD; ++d)
+#pragma unroll
+            for (int c = 0; c + d + 1 <= LCD; ++c) H[e][c][d + 1] = fma(CD, H[e][c][d], H[e][c + 1][d]);
+    }
+    // bra HRR: (a, b+1) = (a+1, b) + AB (a, b)
+#pragma unroll
+    for (int c = 0; c <= LC; ++c)
+#pragma unroll
+        for (int d = 0; d <= LD; ++d) {
+            double Jb[LAB + 1][LB + 1];
+#pragma unroll
+            for (int e = 0; e <= LAB; ++e) Jb[e][0] = H[e][c][d];
+#pragma unroll
+            for (int b = 0; b < LB; ++b)
+#pragma unroll
+                for (int a = 0; a + b + 1 <= LAB; ++a) Jb[a][b + 1] = fma(AB, Jb[a][b], Jb[a + 1][b]);
+#pragma unroll
+            for (int a = 0; a <= LA; ++a)
+#pragma unroll
+                for (int b = 0; b <= LB; ++b) I[a][b][c][d] = Jb[a][b];
+        }
+}
+
+template <int LA, int LB, int LC, int LD, int AC, int CC>
+struct Shape {
+    static constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    static constexpr int LAB = LA + LB, LCD = LC + LD;
+    static constexpr int NR = (LAB + LCD) / 2 + 1;
+    static constexpr int NAL = AC < 0 ? NA : 1;
+    static constexpr int NCL = CC < 0 ? NC : 1;
+    static constexpr int NOUT = NAL * NB * NCL * ND;
+    __device__ static constexpr int acomp(int ai) { return AC < 0 ? ai : AC; }
+    __device__ static constexpr int ccomp(int ci) { return CC < 0 ? ci : CC; }
+};
+
+constexpr double kTwoPi52 = 34.986836655249725693;  // 2 pi^{5/2}
+
+// Accumulates the lane's (unnormalized-component) block slice into acc.
+template <int LA, int LB, int LC, int LD, int AC, int CC>
+__device__ __forceinline__ void quartet(const DevTables& tb, const PairData& pd, int P, int Q,
+                                        double (&acc)[Shape<LA, LB, LC, LD, AC, CC>::NOUT]) {
+    using S = Shape<LA, LB, LC, LD, AC, CC>;
+    constexpr CartT<LA> ca = make_cart<LA>();
+    constexpr CartT<LB> cb = make_cart<LB>();
+    constexpr CartT<LC> cc = make_cart<LC>();
+    constexpr CartT<LD> cd = make_cart<LD>();
+#pragma unroll
+    for (int o = 0; o < S::NOUT; ++o) acc[o] = 0.0;
+
+    const double Ax = pd.A[3 * P], Ay = pd.A[3 * P + 1], Az = pd.A[3 * P + 2];
+    const double ABx = pd.AB[3 * P], ABy = pd.AB[3 * P + 1], ABz = pd.AB[3 * P + 2];
+    const double Cx = pd.A[3 * Q], Cy = pd.A[3 * Q + 1], Cz = pd.A[3 * Q + 2];
+    const double CDx = pd.AB[3 * Q], CDy = pd.AB[3 * Q + 1], CDz = pd.AB[3 * Q + 2];
+    const int pb0 = pd.poff[P], pbn = pd.pn[P];
+    const int pk0 = pd.poff[Q], pkn = pd.pn[Q];
+
+    for (int ib = 0; ib < pbn; ++ib) {
+        const double4 Pp = pd.pP[pb0 + ib];
+        const double wab = pd.pw[pb0 + ib];
+        const double p = Pp.w;
+        for (int ik = 0; ik < pkn; ++ik) {
+            const double4 Qp = pd.pP[pk0 + ik];
+            const double wcd = pd.pw[pk0 + ik];
+            const double q = Qp.w;
+            const double pq = p + q;
+            const double ipq = 1.0 / pq;
+            const double PQx = Pp.x - Qp.x, PQy = Pp.y - Qp.y, PQz = Pp.z - Qp.z;
+            const double T = p * q * ipq * (PQx * PQx + PQy * PQy + PQz * PQz);
+            const double pref = kTwoPi52 * wab * wcd * sqrt(ipq);
+            double rt[S::NR], wt[S::NR];
+            rys_roots<S::NR>(tb, T, rt, wt);
+            const double qo = q * ipq, po = p * ipq;
+            const double i2p = 0.5 / p, i2q = 0.5 / q;
+            const double PAx = Pp.x - Ax, PAy = Pp.y - Ay, PAz = Pp.z - Az;
+            const double QCx = Qp.x - Cx, QCy = Qp.y - Cy, QCz = Qp.z - Cz;
+#pragma unroll
+            for (int r = 0; r < S::NR; ++r) {
+                const double t2 = rt[r];
+                const double B00 = 0.5 * ipq * t2;
+                const double B10 = i2p * (1.0 - qo * t2);
+                const double B01 = i2q * (1.0 - po * t2);
+                const double qt = qo * t2, pt = po * t2;
+                double Ix[LA + 1][LB + 1][LC + 1][LD + 1];
+                double Iy[LA + 1][LB + 1][LC + 1][LD + 1];
+                double Iz[LA + 1][LB + 1][LC + 1][LD + 1];
+                axis_table<LA, LB, LC, LD>(PAx - qt * PQx, QCx + pt * PQx, B00, B10, B01, 1.0, ABx, CDx, Ix);
+                axis_table<LA, LB, LC, LD>(PAy - qt * PQy, QCy + pt * PQy, B00, B10, B01, 1.0, ABy, CDy, Iy);
+                axis_table<LA, LB, LC, LD>(PAz - qt * PQz, QCz + pt * PQz, B00, B10, B01, pref * wt[r], ABz,
+                                           CDz, Iz);
+                int o = 0;
+#pragma unroll
+                for (int ai = 0; ai < S::NAL; ++ai)
+#pragma unroll
+                    for (int b = 0; b < S::NB; ++b)
+#pragma unroll
+                        for (int ci = 0; ci < S::NCL; ++ci)
+#pragma unroll
+                            for (int d = 0; d < S::ND; ++d, ++o) {
+                                constexpr int dummy = 0;
+                                (void)dummy;
+                                const int a = S::acomp(ai), c = S::ccomp(ci);
+                                const double xy = Ix[ca.e[0][a]][cb.e[0][b]][cc.e[0][c]][cd.e[0][d]] *
+                                                  Iy[ca.e[1][a]][cb.e[1][b]][cc.e[1][c]][cd.e[1][d]];
+                                acc[o] = fma(xy, Iz[ca.e[2][a]][cb.e[2][b]][cc.e[2][c]][cd.e[2][d]], acc[o]);
+                            }
+            }
+        }
+    }
+    // unit-diagonal normalization of each component
+    int o = 0;
+#pragma unroll
+    for (int ai = 0; ai < S::NAL; ++ai)
+#pragma unroll
+        for (int b = 0; b < S::NB; ++b)
+#pragma unroll
+            for (int ci = 0; ci < S::NCL; ++ci)
+#pragma unroll
+                for (int d = 0; d < S::ND; ++d, ++o) {
+                    const int a = S::acomp(ai), c = S::ccomp(ci);
+                    acc[o] *= ca.nang[a] * cb.nang[b] * cc.nang[c] * cd.nang[d];
+                }
+}
+
+// ------------------------------------------------------------ kernels ----
+struct JKArgs {
+    PairData pd;
+    DevTables tb;
+    const int4* tasks;  // {P, q_begin, q_end, 0}
+    int ntasks;
+    const double* D;    // internal Cartesian density, M x M
+    double* J;          // Jacc
+    double* K;          // Kacc
+    int M;
+    int do_j, do_k;
+};
+
+struct StoreArgs {
+    PairData pd;
+    DevTables tb;
+    const int2* quartets;   // {P, Q}
+    const long long* off;   // output offset of each quartet's full block
+    int n;
+    double* out;
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+template <int LA, int LB, int LC, int LD, int AC, int CC>
+__global__ __launch_bounds__(256) void k_eri_jk(JKArgs a) {
+    using S = Shape<LA, LB, LC, LD, AC, CC>;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (wave >= a.ntasks) return;  // whole wave leaves together
+    const int4 t = a.tasks[wave];
+    const int P = t.x;
+    const int qraw = t.y + lane;
+    const bool active = qraw < t.z;
+    const int Q = active ? qraw : t.z - 1;
+
+    double acc[S::NOUT];
+    quartet<LA, LB, LC, LD, AC, CC>(a.tb, a.pd, P, Q, acc);
+
+    // degeneracy scale: 1/2 per diagonal pair, 1/2 if P == Q
+    double s = 1.0;
+    if (a.pd.diag[P]) s *= 0.5;
+    if (a.pd.diag[Q]) s *= 0.5;
+    if (P == Q) s *= 0.5;
+    if (!active) s = 0.0;
+
+    const long M = a.M;
+    const int A0 = a.pd.ao[2 * P], B0 = a.pd.ao[2 * P + 1];
+    const int C0 = a.pd.ao[2 * Q], D0 = a.pd.ao[2 * Q + 1];
+    const double* __restrict__ Dm = a.D;
+
+    if (a.do_j) {
+        // ket side: Jacc[c][d] += 2 s sum_ab (ab|cd) D[a][b]
+        // bra side: Jacc[a][b] += 2 s sum_cd (ab|cd) D[c][d], reduced over the wave
+        double dab[S::NAL][S::NB];
+#pragma unroll
+        for (int ai = 0; ai < S::NAL; ++ai)
+#pragma unroll
+            for (int b = 0; b < S::NB; ++b) dab[ai][b] = Dm[(A0 + S::acomp(ai)) * M + B0 + b];
+        double jab[S::NAL][S::NB];
+#pragma unroll
+        for (int ai = 0; ai < S::NAL; ++ai)
+#pragma unroll
+            for (int b = 0; b < S::NB; ++b) jab[ai][b] = 0.0;
+#pragma unroll
+        for (int ci = 0; ci < S::NCL; ++ci)
+#pragma unroll
+            for (int d = 0; d < S::ND; ++d) {
+                const long row = (C0 + S::ccomp(ci)) * M + D0 + d;
+                const double dcd = Dm[row];
+                double jcd = 0.0;
+#pragma unroll
+                for (int ai = 0; ai < S::NAL; ++ai)
+#pragma unroll
+                    for (int b = 0; b < S::NB; ++b) {
+                        const double v = acc[((ai * S::NB + b) * S::NCL + ci) * S::ND + d];
+                        jcd = fma(v, dab[ai][b], jcd);
+                        jab[ai][b] = fma(v, dcd, jab[ai][b]);
+                    }
+                if (active) unsafeAtomicAdd(a.J + row, 2.0 * s * jcd);
+            }
+#pragma unroll
+        for (int ai = 0; ai < S::NAL; ++ai)
+#pragma unroll
+            for (int b = 0; b < S::NB; ++b) {
+                const double v = wave_sum(2.0 * s * jab[ai][b]);
+                if (lane == 0) unsafeAtomicAdd(a.J + (A0 + S::acomp(ai)) * M + B0 + b, v);
+            }
+    }
+    if (a.do_k && active) {
+        // Kacc[a][c] += s sum_{b,d} v D[b][d]; Kacc[b][c] += s sum_{a,d} v D[a][d]
+        // Kacc[a][d] += s sum_{b,c} v D[b][c]; Kacc[b][d] += s sum_{a,c} v D[a][c]
+#pragma unroll
+        for (int ai = 0; ai < S::NAL; ++ai) {
+            const int ga = A0 + S::acomp(ai);
+#pragma unroll
+            for (int ci = 0; ci < S::NCL; ++ci) {
+                const int gc = C0 + S::ccomp(ci);
+                double kac = 0.0;
+#pragma unroll
+                for (int b = 0; b < S::NB; ++b)
+#pragma unroll
+                    for (int d = 0; d < S::ND; ++d)
+                        kac = fma(acc[((ai * S::NB + b) * S::NCL + ci) * S::ND + d], Dm[(B0 + b) * M + D0 + d], kac);
+                unsafeAtomicAdd(a.K + ga * M + gc, s * kac);
+            }
+#pragma unroll
+            for (int d = 0; d < S::ND; ++d) {
+                const int gd = D0 + d;
+                double kad = 0.0;
+#pragma unroll
+                for (int b = 0; b < S::NB; ++b)
+#pragma unroll
+                    for (int ci = 0; ci < S::NCL; ++ci)
+                        kad = fma(acc[((ai * S::NB + b) * S::NCL + ci) * S::ND + d],
+                                  Dm[(B0 + b) * M + C0 + S::ccomp(ci)], kad);
+                unsafeAtomicAdd(a.K + ga * M + gd, s * kad);
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < S::NB; ++b) {
+            const int gb = B0 + b;
+#pragma unroll
+            for (int ci = 0; ci < S::NCL; ++ci) {
+                const int gc = C0 + S::ccomp(ci);
+                double kbc = 0.0;
+#pragma unroll
+                for (int ai = 0; ai < S::NAL; ++ai)
+#pragma unroll
+                    for (int d = 0; d < S::ND; ++d)
+                        kbc = fma(acc[((ai * S::NB + b) * S::NCL + ci) * S::ND + d],
+                                  Dm[(A0 + S::acomp(ai)) * M + D0 + d], kbc);
+                unsafeAtomicAdd(a.K + gb * M + gc, s * kbc);
+            }
+#pragma unroll
+            for (int d = 0; d < S::ND; ++d) {
+                const int gd = D0 + d;
+                double kbd = 0.0;
+#pragma unroll
+                for (int ai = 0; ai < S::NAL; ++ai)
+#pragma unroll
+                    for (int ci = 0; ci < S::NCL; ++ci)
+                        kbd = fma(acc[((ai * S::NB + b) * S::NCL + ci) * S::ND + d],
+                                  Dm[(A0 + S::acomp(ai)) * M + C0 + S::ccomp(ci)], kbd);
+                unsafeAtomicAdd(a.K + gb * M + gd, s * kbd);
+            }
+        }
+    }
+}
+
+// Writes the lane's slice of the unit-diagonal Cartesian block of (P|Q) into
+// out[off + ((a*NB + b)*NC + c)*ND + d] (bra/ket shells in pair orientation).
+template <int LA, int LB, int LC, int LD, int AC, int CC>
+__global__ __launch_bounds__(256) void k_eri_store(StoreArgs a) {
+    using S = Shape<LA, LB, LC, LD, AC, CC>;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const int2 pq = a.quartets[i];
+    double acc[S::NOUT];
+    quartet<LA, LB, LC, LD, AC, CC>(a.tb, a.pd, pq.x, pq.y, acc);
+    double* out = a.out + a.off[i];
+    int o = 0;
+#pragma unroll
+    for (int ai = 0; ai < S::NAL; ++ai)
+#pragma unroll
+        for (int b = 0; b < S::NB; ++b)
+#pragma unroll
+            for (int ci = 0; ci < S::NCL; ++ci)
+#pragma unroll
+                for (int d = 0; d < S::ND; ++d, ++o)
+                    out[((S::acomp(ai) * S::NB + b) * S::NC + S::ccomp(ci)) * S::ND + d] = acc[o];
+}
+
+// ------------------------------------------------------------ dispatch ---
+using JKKernel = void (*)(JKArgs);
+using StoreKernel = void (*)(StoreArgs);
+
+struct ClassKernels {
+    int nchunk = 0;
+    JKKernel jk[36] = {};
+    StoreKernel store[36] = {};
+};
+
+// class index: 0 ss, 1 ps, 2 pp, 3 ds, 4 dp, 5 dd (pair orientation: higher L first)
+#define TLNO_AMD_REG(tag) void register_kernels_bra##tag(ClassKernels (&tab)[6][6]);
+TLNO_AMD_REG(0)
+TLNO_AMD_REG(1)
+TLNO_AMD_REG(2)
+TLNO_AMD_REG(3)
+TLNO_AMD_REG(4)
+TLNO_AMD_REG(4b)
+TLNO_AMD_REG(5)
+TLNO_AMD_REG(5b)
+TLNO_AMD_REG(5c)
+#undef TLNO_AMD_REG
+
+}  // namespace tlno_amd

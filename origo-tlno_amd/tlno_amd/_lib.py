@@ -1,0 +1,61 @@
+"""Loads the HIP library (libtlno_amd.so) and declares its C ABI (include/tlno_gpu.h).
+
+There is no fallback: if the library is missing or no HIP device is usable,
+every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtlno_amd.so")
+
+D = C.POINTER(C.c_double)
+I = C.POINTER(C.c_int)
+LL = C.POINTER(C.c_longlong)
+
+# every symbol the header declares, with (restype, argtypes)
+SIGNATURES = {
+    "tlno_gpu_last_error": (C.c_char_p, []),
+    "tlno_gpu_init": (C.c_int, [C.c_int]),
+    "tlno_gpu_plan_create": (C.c_int, [C.c_int, D, I, I, D, D, C.c_int, C.POINTER(C.c_void_p)]),
+    "tlno_gpu_plan_destroy": (None, [C.c_void_p]),
+    "tlno_gpu_plan_info": (C.c_int, [C.c_void_p, I, I, I, LL]),
+    "tlno_gpu_plan_groups": (C.c_int, [C.c_void_p, C.c_int, I, I]),
+    "tlno_gpu_build_jk": (C.c_int, [C.c_void_p, D, D, D, C.c_double]),
+    "tlno_gpu_build_jk_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double]),
+    "tlno_gpu_last_build_stats": (C.c_int, [C.c_void_p, LL, LL]),
+    "tlno_gpu_eri_cart_blocks": (C.c_int, [C.c_void_p, C.c_int, I, LL, D]),
+    "tlno_gpu_boys": (C.c_int, [C.c_int, D, C.c_int, D]),
+    "tlno_gpu_rys": (C.c_int, [C.c_int, C.c_int, D, D, D]),
+    "tlno_gpu_onee": (C.c_int, [C.c_void_p, C.c_int, D, D, D, D, D]),
+}
+
+_lib = None
+
+
+class TlnoGpuError(RuntimeError):
+    pass
+
+
+def load():
+    """Return the loaded library (symbols declared); raise if it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise TlnoGpuError(
+            f"HIP library not built: {LIB_PATH} (run __graft_entry__.build() or make -C origo-tlno_amd/csrc)")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != 0:
+        raise TlnoGpuError(load().tlno_gpu_last_error().decode())

@@ -1,0 +1,257 @@
+"""Direct J/K/G builders over the HIP engine, mirroring the reference's
+``tlno::integrals::twoe::fock`` and ``tlno::integrals::twoe`` interfaces
+(integrals/two_electron/fock/jk_direct.hpp:18-96,
+integrals/two_electron/fock/eri_direct_jk.hpp:17-39,
+integrals/two_electron/twoe_integrals.hpp:41-113).
+
+Same names, argument meaning and error behaviour:
+  * the plan is spherical by default (the reference's only basis); pass
+    ``cartesian=True`` for the Cartesian component basis (AO index =
+    component index);
+  * densities are M x M, finite and symmetric within 128 eps (else
+    ``RuntimeError("eri_ao_tensor_hgp: density ...")``);
+  * thresholds must be finite and >= 0.
+
+Options the reference exposes whose screens are not built on the GPU yet
+(primitive FASTSCHWARZ, decay, density/CVHF) raise NotImplementedError when
+nonzero instead of being silently ignored.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .basis import Basis
+
+DEFAULT_SCHWARZ = 1.0e-12  # kEriPackedDefaultSchwarzThreshold (twoe_integrals.hpp:41)
+
+
+@dataclass
+class ScreeningOptions:
+    integral_threshold: float = DEFAULT_SCHWARZ
+    primitive_fastschwarz_threshold: float = 0.0
+    decay_threshold: float = 0.0
+    density_threshold: float = 0.0
+
+
+@dataclass
+class EriPackedBuildOptions:
+    schwarz_threshold: float = DEFAULT_SCHWARZ
+    primitive_fastschwarz_threshold: float = 0.0
+    decay_threshold: float = 0.0
+    density_threshold: float = 0.0
+
+
+def screening_from_eri_options(o: EriPackedBuildOptions) -> ScreeningOptions:
+    return ScreeningOptions(o.schwarz_threshold, o.primitive_fastschwarz_threshold, o.decay_threshold,
+                            o.density_threshold)
+
+
+def screening_to_eri_options(o: ScreeningOptions) -> EriPackedBuildOptions:
+    return EriPackedBuildOptions(o.integral_threshold, o.primitive_fastschwarz_threshold, o.decay_threshold,
+                                 o.density_threshold)
+
+
+def _check_threshold(name, v):
+    if not (isinstance(v, (int, float)) and math.isfinite(v) and v >= 0.0):
+        raise RuntimeError(f"eri_ao_tensor_hgp: {name} must be finite and >= 0.")
+
+
+def _validate_options(o: ScreeningOptions):
+    _check_threshold("schwarz_threshold", o.integral_threshold)
+    _check_threshold("primitive_fastschwarz_threshold", o.primitive_fastschwarz_threshold)
+    _check_threshold("decay_threshold", o.decay_threshold)
+    _check_threshold("density_threshold", o.density_threshold)
+    for name in ("primitive_fastschwarz_threshold", "decay_threshold", "density_threshold"):
+        if getattr(o, name) != 0.0:
+            raise NotImplementedError(f"{name} screening is not available in the GPU engine")
+
+
+@dataclass
+class PlanProfile:
+    """Subset of EriPackedPlanProfile (twoe_integrals.hpp:68-89)."""
+    M: int = 0
+    shell_groups: int = 0
+    pair_groups: int = 0
+    primitive_pairs: int = 0
+
+
+class DirectPlan:
+    """GPU-resident plan: grouped shells, primitive-pair SoA and Schwarz bounds."""
+
+    def __init__(self, basis: Basis, options: ScreeningOptions | None = None, cartesian: bool = False,
+                 device: int = 0):
+        lib = _lib.load()
+        _lib.check(lib.tlno_gpu_init(int(device)))
+        self.basis = basis
+        self.cartesian = bool(cartesian)
+        self.options = options or ScreeningOptions()
+        _validate_options(self.options)
+        self._arrays = (np.ascontiguousarray(basis.centers, dtype=np.float64).reshape(-1),
+                        np.ascontiguousarray(basis.ang, dtype=np.int32).reshape(-1),
+                        np.ascontiguousarray(basis.nprim, dtype=np.int32),
+                        np.ascontiguousarray(basis.exps, dtype=np.float64),
+                        np.ascontiguousarray(basis.coefs, dtype=np.float64))
+        c, a, n, e, f = self._arrays
+        h = C.c_void_p()
+        _lib.check(lib.tlno_gpu_plan_create(len(n), c.ctypes.data_as(_lib.D), a.ctypes.data_as(_lib.I),
+                                            n.ctypes.data_as(_lib.I), e.ctypes.data_as(_lib.D),
+                                            f.ctypes.data_as(_lib.D), 0 if cartesian else 1, C.byref(h)))
+        self._h = h
+        M = C.c_int()
+        ng = C.c_int()
+        npair = C.c_int()
+        npp = C.c_longlong()
+        _lib.check(lib.tlno_gpu_plan_info(h, C.byref(M), C.byref(ng), C.byref(npair), C.byref(npp)))
+        self.profile = PlanProfile(M.value, ng.value, npair.value, npp.value)
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def M(self):
+        return self.profile.M
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            _lib.load().tlno_gpu_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def jk(self, density, want_j=True, want_k=True, schwarz=None):
+        """Raw J and K (either may be skipped) on host arrays."""
+        thr = self.options.integral_threshold if schwarz is None else schwarz
+        _check_threshold("schwarz_threshold", thr)
+        M = self.M
+        D = np.ascontiguousarray(density, dtype=np.float64)
+        if D.shape != (M, M):
+            raise RuntimeError("eri_ao_tensor_hgp: density must be spherical M x M row-major.")
+        J = np.empty((M, M)) if want_j else None
+        K = np.empty((M, M)) if want_k else None
+        _lib.check(_lib.load().tlno_gpu_build_jk(self._h, D.ctypes.data_as(_lib.D),
+                                                 J.ctypes.data_as(_lib.D) if want_j else None,
+                                                 K.ctypes.data_as(_lib.D) if want_k else None, float(thr)))
+        return J, K
+
+    def last_build_stats(self):
+        sq = C.c_longlong()
+        pq = C.c_longlong()
+        _lib.check(_lib.load().tlno_gpu_last_build_stats(self._h, C.byref(sq), C.byref(pq)))
+        return sq.value, pq.value
+
+    def eri_cart_blocks(self, quads):
+        """Unit-diagonal Cartesian blocks for group quadruples [(ga, gb, gc, gd), ...]."""
+        from .basis import ncart
+        quads = np.ascontiguousarray(quads, dtype=np.int32).reshape(-1, 4)
+        Ls = self.group_L()
+        sizes = [int(np.prod([ncart(Ls[g]) for g in q])) for q in quads]
+        offs = np.zeros(len(quads), dtype=np.int64)
+        if len(quads):
+            offs[1:] = np.cumsum(sizes)[:-1]
+        out = np.zeros(int(sum(sizes)))
+        _lib.check(_lib.load().tlno_gpu_eri_cart_blocks(self._h, len(quads), quads.ctypes.data_as(_lib.I),
+                                                        offs.ctypes.data_as(_lib.LL), out.ctypes.data_as(_lib.D)))
+        res = []
+        for q, o, s in zip(quads, offs, sizes):
+            res.append(out[o:o + s].reshape([ncart(Ls[g]) for g in q]))
+        return res
+
+    def group_L(self):
+        """L of each shell group of the plan, in first-appearance order."""
+        self._load_groups()
+        return self._group_L
+
+    def group_components(self):
+        """basis index of each group's components in descending-lex order."""
+        self._load_groups()
+        return self._group_comps
+
+    def _load_groups(self):
+        if hasattr(self, "_group_L"):
+            return
+        ng = self.profile.shell_groups
+        L = np.zeros(ng, np.int32)
+        comps = np.zeros(15 * ng, np.int32)
+        _lib.check(_lib.load().tlno_gpu_plan_groups(self._h, ng, L.ctypes.data_as(_lib.I),
+                                                    comps.ctypes.data_as(_lib.I)))
+        self._group_L = [int(x) for x in L]
+        self._group_comps = [comps[15 * g:15 * g + (l + 1) * (l + 2) // 2].copy() for g, l in enumerate(self._group_L)]
+
+
+@dataclass
+class CoulombJResult:
+    j: np.ndarray = field(default_factory=lambda: np.zeros((0, 0)))
+    M: int = 0
+
+
+@dataclass
+class ExchangeKResult:
+    k: np.ndarray = field(default_factory=lambda: np.zeros((0, 0)))
+    M: int = 0
+
+
+@dataclass
+class DirectResult:
+    g: np.ndarray = field(default_factory=lambda: np.zeros((0, 0)))
+    M: int = 0
+
+
+def build_direct_plan(basis: Basis, options: ScreeningOptions | None = None, cartesian=False) -> DirectPlan:
+    return DirectPlan(basis, options, cartesian=cartesian)
+
+
+def _opts(plan, options):
+    o = options or plan.options
+    _validate_options(o)
+    return o
+
+
+def build_coulomb_j(plan: DirectPlan, density, options: ScreeningOptions | None = None) -> CoulombJResult:
+    o = _opts(plan, options)
+    J, _ = plan.jk(density, True, False, o.integral_threshold)
+    return CoulombJResult(J, plan.M)
+
+
+def build_exchange_k(plan: DirectPlan, density, options: ScreeningOptions | None = None) -> ExchangeKResult:
+    """K[a,c] = sum_(b,d) P[b,d] (ab|cd), for a real symmetric density."""
+    o = _opts(plan, options)
+    _, K = plan.jk(density, False, True, o.integral_threshold)
+    return ExchangeKResult(K, plan.M)
+
+
+def build_rhf_g(plan: DirectPlan, density, options: ScreeningOptions | None = None) -> DirectResult:
+    """G = J - K/2 from one traversal."""
+    o = _opts(plan, options)
+    J, K = plan.jk(density, True, True, o.integral_threshold)
+    return DirectResult(J - 0.5 * K, plan.M)
+
+
+# --- the twoe:: names (EriPackedPlan is the same GPU plan object) ---
+def build_eri_packed_plan_for_G_rys(basis: Basis, options: EriPackedBuildOptions | None = None) -> DirectPlan:
+    return DirectPlan(basis, screening_from_eri_options(options or EriPackedBuildOptions()))
+
+
+def adopt_eri_plan_for_transition(plan: DirectPlan) -> DirectPlan:
+    return plan
+
+
+@dataclass
+class EriDirectJKResult:
+    g: np.ndarray = field(default_factory=lambda: np.zeros((0, 0)))
+    M: int = 0
+
+
+def build_jk_direct_for_G_rys(plan: DirectPlan, density, options: EriPackedBuildOptions | None = None):
+    o = screening_from_eri_options(options or EriPackedBuildOptions())
+    r = build_rhf_g(plan, density, o)
+    return EriDirectJKResult(r.g, r.M)

@@ -1,0 +1,164 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle on the same
+seeded inputs.
+
+Tolerances (stated per test):
+  ERIs   |d| <= 1e-14 + 1e-12 |ref| + 1e-13 B, with B = Q_ab Q_cd the
+         Schwarz bound of the block.  The last term is the floor of entries
+         that vanish by symmetry or cancel: binary64 rounding of the product
+         centres alone perturbs them by ~eps |A| sqrt(p) B (tight exponents,
+         centres ~10 bohr), in every implementation including the reference.
+  S/T/V  |d| <= 1e-14 + 1e-12 |ref|.
+  J/K    max-abs <= 1e-10.
+  Boys and Rys rules: relative 1e-13.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tlno_amd import configs, core_math
+from tlno_amd.fock import DirectPlan, ScreeningOptions, build_coulomb_j, build_exchange_k, build_rhf_g
+from tlno_amd import onee
+
+pytestmark = pytest.mark.gpu
+
+ERI_ABS, ERI_REL = 1e-14, 1e-12
+JK_ABS = 1e-10
+
+
+def _close(x, ref, abs_tol=ERI_ABS, rel_tol=ERI_REL, floor=0.0):
+    x = np.asarray(x)
+    ref = np.asarray(ref)
+    tol = abs_tol + rel_tol * np.abs(ref) + floor
+    bad = np.abs(x - ref) > tol
+    return not bad.any(), float(np.max(np.abs(x - ref) / tol))
+
+
+def _schwarz_floor(ob):
+    Q = ob.schwarz(False)
+
+    def floor(q):
+        def qi(i, j):
+            i, j = max(i, j), min(i, j)
+            return Q[i * (i + 1) // 2 + j]
+        return 1e-13 * qi(q[0], q[1]) * qi(q[2], q[3])
+    return floor
+
+
+@pytest.fixture(scope="module")
+def small_organic():
+    return configs.organic30(natoms=5)
+
+
+def test_boys_matches_oracle():
+    rng = np.random.default_rng(1)
+    T = np.concatenate([[0.0, 1e-12, 1e-6, 0.0625, 0.125, 39.999, 40.0, 40.001, 55.0, 120.0, 1e4],
+                        rng.uniform(0, 45, 400), rng.uniform(0, 2, 100)])
+    for mmax in (0, 1, 4, 8, 16):
+        F = core_math.boys(T, mmax)
+        ref = np.array([[O.boys(m, t) for m in range(mmax + 1)] for t in T])
+        rel = np.abs(F - ref) / np.abs(ref)
+        assert rel.max() < 1e-13, (mmax, rel.max(), T[np.unravel_index(rel.argmax(), rel.shape)[0]])
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5])
+def test_rys_matches_oracle(N):
+    rng = np.random.default_rng(N)
+    edges = [0.0, 1e-9, 0.5, 0.999999, 1.0, 1.000001, 12.5, 29.0, 37.0, 41.0, 48.99, 49.0, 54.99, 55.0,
+             60.99, 61.0, 66.99, 67.0, 80.0, 200.0, 5000.0]
+    T = np.concatenate([edges, rng.uniform(0, 70, 300)])
+    r, w = core_math.rys(N, T)
+    for i, t in enumerate(T):
+        rr, ww = O.rys(N, t)
+        assert np.allclose(r[i], rr, rtol=1e-13, atol=0), (N, t, r[i], rr)
+        assert np.allclose(w[i], ww, rtol=1e-13, atol=0), (N, t, w[i], ww)
+
+
+def test_eri_blocks_water7_all():
+    b, _ = configs.water7()
+    plan = DirectPlan(b, ScreeningOptions(integral_threshold=0.0), cartesian=True)
+    ob = O.OracleBasis(b)
+    ng = ob.ngroups
+    quads = [(a, bb, c, d) for a in range(ng) for bb in range(ng) for c in range(ng) for d in range(ng)]
+    blocks = plan.eri_cart_blocks(quads)
+    fl = _schwarz_floor(ob)
+    for q, blk in zip(quads, blocks):
+        ok, worst = _close(blk, ob.cart_block(*q), floor=fl(q))
+        assert ok, (q, worst)
+
+
+def test_eri_blocks_with_d_shells(small_organic):
+    b, _ = small_organic
+    plan = DirectPlan(b, ScreeningOptions(integral_threshold=0.0), cartesian=True)
+    ob = O.OracleBasis(b)
+    Ls = plan.group_L()
+    ng = len(Ls)
+    rng = np.random.default_rng(7)
+    quads = set()
+    # every class combination at least once, plus random quadruples
+    byL = {L: [g for g in range(ng) if Ls[g] == L] for L in set(Ls)}
+    for la in byL:
+        for lb in byL:
+            for lc in byL:
+                for ld in byL:
+                    quads.add((rng.choice(byL[la]), rng.choice(byL[lb]), rng.choice(byL[lc]), rng.choice(byL[ld])))
+    for _ in range(200):
+        quads.add(tuple(int(x) for x in rng.integers(0, ng, 4)))
+    quads = sorted(quads)
+    blocks = plan.eri_cart_blocks(quads)
+    fl = _schwarz_floor(ob)
+    for q, blk in zip(quads, blocks):
+        ok, worst = _close(blk, ob.cart_block(*q), floor=fl(q))
+        assert ok, (q, [Ls[g] for g in q], worst)
+
+
+@pytest.mark.parametrize("cartesian", [True, False])
+def test_jk_water7(cartesian):
+    b, D = configs.water7()
+    plan = DirectPlan(b, ScreeningOptions(integral_threshold=0.0), cartesian=cartesian)
+    ob = O.OracleBasis(b)
+    Jr, Kr = ob.jk(D, spherical=not cartesian)
+    J = build_coulomb_j(plan, D).j
+    K = build_exchange_k(plan, D).k
+    G = build_rhf_g(plan, D).g
+    assert np.abs(J - Jr).max() < JK_ABS
+    assert np.abs(K - Kr).max() < JK_ABS
+    assert np.abs(G - (Jr - 0.5 * Kr)).max() < JK_ABS
+
+
+@pytest.mark.parametrize("cartesian", [True, False])
+def test_jk_small_organic(small_organic, cartesian):
+    b, Dc = small_organic
+    plan = DirectPlan(b, ScreeningOptions(integral_threshold=0.0), cartesian=cartesian)
+    ob = O.OracleBasis(b)
+    if cartesian:
+        D = Dc
+    else:
+        rng = np.random.default_rng(3)
+        C = rng.standard_normal((ob.nsph, 10))
+        D = C @ C.T
+        D /= np.linalg.norm(D)
+    Jr, Kr = ob.jk(D, spherical=not cartesian)
+    J, K = plan.jk(D)
+    assert np.abs(J - Jr).max() < JK_ABS, np.abs(J - Jr).max()
+    assert np.abs(K - Kr).max() < JK_ABS, np.abs(K - Kr).max()
+
+
+def test_jk_schwarz_screened_close_to_unscreened(small_organic):
+    b, D = small_organic
+    plan = DirectPlan(b, ScreeningOptions(integral_threshold=1e-12), cartesian=True)
+    J0, K0 = plan.jk(D, schwarz=0.0)
+    J1, K1 = plan.jk(D, schwarz=1e-12)
+    assert np.abs(J1 - J0).max() < JK_ABS
+    assert np.abs(K1 - K0).max() < JK_ABS
+
+
+@pytest.mark.parametrize("cartesian", [True, False])
+def test_onee(small_organic, cartesian):
+    b, _ = small_organic
+    plan = DirectPlan(b, ScreeningOptions(), cartesian=cartesian)
+    m = onee.evaluate(plan, b.nuclear_charges, b.nuclear_centers)
+    ob = O.OracleBasis(b)
+    S, T, V = ob.onee(b.nuclear_charges, b.nuclear_centers, spherical=not cartesian)
+    for x, r in ((m.overlap, S), (m.kinetic, T), (m.nuclear, V)):
+        ok, worst = _close(x, r)
+        assert ok, worst
