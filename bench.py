@@ -1,0 +1,173 @@
+#!/usr/bin/env python3
+"""Benchmark: direct J/K Fock builds per second on one MI355X.
+
+Workload (default): organic30 — 30 atoms uniform in a 12 A cube (minimum
+separation 1 A, C/N/O/H .3/.1/.1/.5), heavy atoms 3s2p1d and H 2s1p
+Cartesian shells with 1-6 primitives (exponents log-uniform in [0.08, 5000]),
+340 Cartesian functions; density D = C C^T (C 340 x 60, N(0,1), ||D||_F = 1);
+Schwarz threshold 1e-12; FP64 throughout.
+
+A step is one J+K build from a density already resident in HBM (one ERI
+traversal producing both matrices), through the C ABI's device entry point.
+The plan (shell pairs, Schwarz bounds, work lists) is built before timing, as
+in the reference (plan and build are separate API calls).
+
+Reported besides the headline value:
+  * ERI throughput: shell quartets/s, primitive quartets/s, Cartesian
+    integrals/s after screening;
+  * roofline of the dominant class kernel (bra/ket class pair with the most
+    time): algorithmic flops per launch (tlno_amd.flops model) over its
+    average HIP-event-timed duration on the plan's stream, against the
+    MI355X FP64 peak (78.6 TFLOP/s; vector and matrix FP64 peaks are equal);
+  * cpu_baseline: the reference engine built from its own sources
+    (oracle/_ref) doing the same build (G = J - K/2, spherical d, its only
+    basis) on this host, one thread — the reference's K and G endpoints
+    refuse more than one OpenMP thread.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "origo-tlno_amd"), ROOT]
+
+import numpy as np  # noqa: E402
+
+from tlno_amd import _lib, configs, flops, hipmem  # noqa: E402
+from tlno_amd.fock import DirectPlan, ScreeningOptions  # noqa: E402
+
+FP64_PEAK_TFLOPS = 78.6
+
+
+def gpu_run(b, D, steps, warmup, thr):
+    lib = _lib.load()
+    plan = DirectPlan(b, ScreeningOptions(integral_threshold=thr), cartesian=True)
+    M = plan.M
+    dD = hipmem.DeviceArray.from_host(D)
+    dJ = hipmem.DeviceArray((M, M))
+    dK = hipmem.DeviceArray((M, M))
+
+    def one():
+        _lib.check(lib.tlno_gpu_build_jk_device(plan.handle, dD.ptr, dJ.ptr, dK.ptr, thr))
+
+    for _ in range(warmup):
+        one()
+    hipmem.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        one()
+    hipmem.synchronize()
+    wall = time.perf_counter() - t0
+    # per-class kernel timing (separate pass, so event records do not sit in the timed loop)
+    _lib.check(lib.tlno_gpu_set_timing(plan.handle, 1))
+    for _ in range(max(3, min(steps, 10))):
+        one()
+    sq = (C.c_longlong * 36)()
+    pq = (C.c_longlong * 36)()
+    ms = (C.c_double * 36)()
+    nb = C.c_longlong()
+    _lib.check(lib.tlno_gpu_class_stats(plan.handle, sq, pq, ms, C.byref(nb)))
+    _lib.check(lib.tlno_gpu_set_timing(plan.handle, 0))
+    J = dJ.to_host()
+    K = dK.to_host()
+    classes = []
+    for X in range(6):
+        for Y in range(X + 1):
+            i = 6 * X + Y
+            if sq[i] == 0:
+                continue
+            fl = flops.class_flops(X, Y, pq[i], sq[i])
+            t = ms[i] / nb.value * 1e-3
+            classes.append({"class": f"({flops.CLASS_NAME[X]}|{flops.CLASS_NAME[Y]})", "shell_quartets": sq[i],
+                            "prim_quartets": pq[i], "ms": ms[i] / nb.value, "gflop": fl / 1e9,
+                            "tflops": fl / t / 1e12 if t > 0 else 0.0,
+                            "integrals": sq[i] * flops.n_out(*flops.CLASS_L[X], *flops.CLASS_L[Y])})
+    return plan, wall, classes, J, K
+
+
+def cpu_baseline(b, seconds_cap=60.0):
+    try:
+        from oracle import oracle as O
+        ref = O.Reference()
+    except Exception as e:  # reference not built on this host
+        return {"value": None, "unit": "Fock builds/s", "cores": 0, "kind": "reference",
+                "sample": f"unavailable: {e}"}
+    M = ref.plan_M(b)
+    D = configs.spherical_density(M, 5, 60)
+    _, sec = ref.direct(b, D, "G", schwarz=1e-12, threads=1, repeats=1)
+    ncpu = os.cpu_count()
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": 1.0 / sec, "unit": "Fock builds/s", "cores": 1, "kind": "reference",
+            "sample": (f"one full G = J - K/2 build of the same molecule and basis in the reference's spherical "
+                       f"basis (M={M}), Schwarz 1e-12, 1 thread ({sec:.2f} s); host {model}, {ncpu} cpus")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="organic30", choices=["organic30", "water40"])
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the J/K build shards by "
+              "quartet tasks with one all-reduce of J and K; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    thr = 1e-12
+    b, D = configs.organic30() if args.workload == "organic30" else configs.water40()
+    plan, wall, classes, J, K = gpu_run(b, D, args.steps, args.warmup, thr)
+    ms_per = wall / args.steps * 1e3
+    sq_tot = sum(c["shell_quartets"] for c in classes)
+    pq_tot = sum(c["prim_quartets"] for c in classes)
+    ints_tot = sum(c["integrals"] for c in classes)
+    dom = max(classes, key=lambda c: c["ms"])
+    total_gflop = sum(c["gflop"] for c in classes)
+    result = {
+        "metric": "J/K Fock builds/s",
+        "value": args.steps / wall,
+        "unit": "Fock builds/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded generator, tlno_amd.configs)",
+        "config": {"workload": f"{args.workload}: direct J+K build, Cartesian d, Schwarz 1e-12",
+                   "n_basis": plan.M, "shell_pairs": plan.profile.pair_groups,
+                   "primitive_pairs": plan.profile.primitive_pairs, "parallelism": "single GPU"},
+        "eri_throughput": {"shell_quartets_per_s": sq_tot / (wall / args.steps),
+                           "primitive_quartets_per_s": pq_tot / (wall / args.steps),
+                           "cartesian_integrals_per_s": ints_tot / (wall / args.steps),
+                           "shell_quartets_per_build": sq_tot, "primitive_quartets_per_build": pq_tot},
+        "roofline": {"bound": "mfma", "pipe": "fp64 VALU (fp64 vector peak = fp64 matrix peak)",
+                     "kernel": f"k_eri_jk {dom['class']} (all chunks)", "achieved": dom["tflops"],
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": dom["tflops"] / FP64_PEAK_TFLOPS,
+                     "traffic": None, "algorithmic_gflop_per_launch": dom["gflop"],
+                     "avg_launch_ms": dom["ms"],
+                     "whole_build_tflops": total_gflop / (ms_per * 1e-3) / 1e3},
+        "classes": [{k: (round(v, 6) if isinstance(v, float) else v) for k, v in c.items()} for c in classes],
+    }
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(b)
+        if result["cpu_baseline"]["value"]:
+            result["gpu_over_cpu_baseline"] = result["value"] / result["cpu_baseline"]["value"]
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -82,6 +82,17 @@ int tlno_gpu_build_jk_device(tlno_gpu_plan* plan, const double* d_density,
 int tlno_gpu_last_build_stats(const tlno_gpu_plan* plan, long long* shell_quartets,
                               long long* prim_quartets);
 
+/* Per-class-pair kernel timing (no reference counterpart; measurement hook):
+ * enable = 1 records HIP events on the plan's stream around the kernels of
+ * each bra/ket class pair of every following build and resets the
+ * accumulators.  tlno_gpu_class_stats returns, for the last build's
+ * threshold, 36 entries each (index 6*X + Y, classes ss ps pp ds dp dd):
+ * shell quartets, primitive quartets and accumulated milliseconds over
+ * `builds` timed builds. */
+int tlno_gpu_set_timing(tlno_gpu_plan* plan, int enable);
+int tlno_gpu_class_stats(const tlno_gpu_plan* plan, long long* shell_quartets,
+                         long long* prim_quartets, double* ms, long long* builds);
+
 /* Unit-diagonal Cartesian ERI blocks (ga gb|gc gd) for n group quadruples
  * (groups4[4i..4i+3], groups numbered in first-appearance order); block i is
  * written at out + offsets[i] as [a][b][c][d] in each group's component

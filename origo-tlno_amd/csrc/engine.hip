@@ -215,12 +215,17 @@ struct GroupH {
     int sph_base = 0;
 };
 
+#define tlno_amd_side_streams 4
+constexpr int kSideStreams = tlno_amd_side_streams;
+
 struct TaskSet {
     int4* d_tasks = nullptr;
     long long off[6][6] = {};
     long long cnt[6][6] = {};
     long long shell_quartets = 0;
     long long prim_quartets = 0;
+    long long cls_sq[6][6] = {};  // shell quartets per class pair
+    long long cls_pq[6][6] = {};  // primitive quartets per class pair
 };
 
 }  // namespace tlno_amd
@@ -250,8 +255,16 @@ struct tlno_gpu_plan {
     double *dDin = nullptr, *dDint = nullptr, *dJacc = nullptr, *dKacc = nullptr;
     double *dJout = nullptr, *dKout = nullptr;
     hipStream_t stream = nullptr;
+    hipStream_t side[tlno_amd_side_streams] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[tlno_amd_side_streams] = {};
     std::map<double, tlno_amd::TaskSet> tasks;
     long long last_shell_quartets = 0, last_prim_quartets = 0;
+    double last_thr = -1.0;
+    // optional per-class kernel timing with HIP events on the plan stream
+    int timing = 0;
+    hipEvent_t ev[6][6][2] = {};
+    double cls_ms[6][6] = {};
+    long long timed_builds = 0;
 
     tlno_amd::PairData pair_data() const {
         tlno_amd::PairData p;
@@ -447,15 +460,16 @@ __global__ void k_rys(DevTables tb, const double* T, double* r, double* w, int n
 
 inline unsigned grid_for(long n, int block) { return static_cast<unsigned>((n + block - 1) / block); }
 
-void launch_jk(const ClassKernels& k, int chunk, JKArgs& args, long long ntask, hipStream_t st) {
+// grid: x covers the tasks (one wave each, 4 waves per block), y the chunks
+void launch_jk(const ClassKernels& k, JKArgs& args, long long ntask, hipStream_t st) {
     void* params[] = {&args};
     const unsigned grid = grid_for(ntask * 64, 256);
-    HIP_OK(hipLaunchKernel(reinterpret_cast<const void*>(k.jk[chunk]), dim3(grid), dim3(256), params, 0, st));
+    HIP_OK(hipLaunchKernel(reinterpret_cast<const void*>(k.jk), dim3(grid, k.nchunk), dim3(256), params, 0, st));
 }
 
-void launch_store(const ClassKernels& k, int chunk, StoreArgs& args, hipStream_t st) {
+void launch_store(const ClassKernels& k, StoreArgs& args, hipStream_t st) {
     void* params[] = {&args};
-    HIP_OK(hipLaunchKernel(reinterpret_cast<const void*>(k.store[chunk]), dim3(grid_for(args.n, 128)),
+    HIP_OK(hipLaunchKernel(reinterpret_cast<const void*>(k.store), dim3(grid_for(args.n, 128), k.nchunk),
                            dim3(128), params, 0, st));
 }
 
@@ -479,7 +493,7 @@ void run_store(tlno_gpu_plan* pl, int X, int Y, const std::vector<int2>& qs, con
     a.n = static_cast<int>(qs.size());
     a.out = d_out;
     const ClassKernels& k = g_kernels[X][Y];
-    for (int ch = 0; ch < k.nchunk; ++ch) launch_store(k, ch, a, pl->stream);
+    launch_store(k, a, pl->stream);
     HIP_OK(hipStreamSynchronize(pl->stream));
     HIP_OK(hipFree(dq));
     HIP_OK(hipFree(doff));
@@ -532,11 +546,17 @@ void upload_pairs(tlno_gpu_plan* pl, const std::vector<int>& order,
 
 void build_plan(tlno_gpu_plan* pl, int n, const double* centers, const int* ang, const int* nprim,
                 const double* exps, const double* coefs) {
+    // input validation first: invalid bases fail the same way with or without a device
+    pl->groups = build_groups(n, centers, ang, nprim, exps, coefs);
     DeviceCtx& ctx = device_ctx(pl->device);
     (void)ctx;
     HIP_OK(hipSetDevice(pl->device));
     HIP_OK(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking));
-    pl->groups = build_groups(n, centers, ang, nprim, exps, coefs);
+    for (int s = 0; s < kSideStreams; ++s) {
+        HIP_OK(hipStreamCreateWithFlags(&pl->side[s], hipStreamNonBlocking));
+        HIP_OK(hipEventCreateWithFlags(&pl->ev_join[s], hipEventDisableTiming));
+    }
+    HIP_OK(hipEventCreateWithFlags(&pl->ev_fork, hipEventDisableTiming));
     const int ng = static_cast<int>(pl->groups.size());
     int ib = 0, sb = 0;
     for (auto& G : pl->groups) {
@@ -718,6 +738,7 @@ const TaskSet& get_tasks(tlno_gpu_plan* pl, double thr) {
             const int b0 = pl->cls_begin[X], b1 = pl->cls_begin[X + 1];
             const int k0 = pl->cls_begin[Y], k1 = pl->cls_begin[Y + 1];
             if (b1 > b0 && k1 > k0) {
+                std::vector<int4> base;
                 for (int P = b0; P < b1; ++P) {
                     int end = k1;
                     if (thr > 0.0) {
@@ -731,10 +752,31 @@ const TaskSet& get_tasks(tlno_gpu_plan* pl, double thr) {
                         end = lo;
                     }
                     if (X == Y) end = std::min(end, P + 1);
-                    for (int q0 = k0; q0 < end; q0 += 64) all.push_back(make_int4(P, q0, end, 0));
+                    for (int q0 = k0; q0 < end; q0 += 64) base.push_back(make_int4(P, q0, end, 0));
                     const long long nq = std::max(0, end - k0);
                     ts.shell_quartets += nq;
-                    if (end > k0) ts.prim_quartets += (pref[end] - pref[k0]) * pl->pn_host[P];
+                    ts.cls_sq[X][Y] += nq;
+                    if (end > k0) {
+                        const long long pq = (pref[end] - pref[k0]) * pl->pn_host[P];
+                        ts.prim_quartets += pq;
+                        ts.cls_pq[X][Y] += pq;
+                    }
+                }
+                // Too few waves to fill 256 CUs (rare, heavy classes): split the
+                // bra primitive pairs of each task into slices of their own.
+                const long long waves = static_cast<long long>(base.size()) * g_kernels[X][Y].nchunk;
+                constexpr long long kTargetWaves = 16384;
+                int split = 1;
+                if (waves > 0 && waves < kTargetWaves)
+                    split = static_cast<int>(std::min<long long>(256, (kTargetWaves + waves - 1) / waves));
+                for (const int4& t : base) {
+                    const int pbn = pl->pn_host[t.x];
+                    const int s = std::min(split, pbn);
+                    const int step = (pbn + s - 1) / s;
+                    for (int i0 = 0; i0 < pbn; i0 += step) {
+                        const int i1 = std::min(pbn, i0 + step);
+                        all.push_back(make_int4(t.x, t.y, t.z, i0 | (i1 << 16)));
+                    }
                 }
             }
             ts.cnt[X][Y] = static_cast<long long>(all.size()) - ts.off[X][Y];
@@ -775,7 +817,16 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
     a.do_j = dJ != nullptr;
     a.do_k = dK != nullptr;
     if (a.do_j || a.do_k) {
-        // heaviest classes first
+        // Class-pair kernels are independent (their J/K contributions are
+        // atomic adds), so they run concurrently on the plan's side streams,
+        // heaviest first; timing mode keeps them on one stream so each
+        // class's events bracket only its own kernel.
+        const bool concurrent = !pl->timing;
+        if (concurrent) {
+            HIP_OK(hipEventRecord(pl->ev_fork, st));
+            for (int s = 0; s < kSideStreams; ++s) HIP_OK(hipStreamWaitEvent(pl->side[s], pl->ev_fork, 0));
+        }
+        int rr = 0;
         for (int X = 5; X >= 0; --X)
             for (int Y = X; Y >= 0; --Y) {
                 const long long nt = ts.cnt[X][Y];
@@ -783,7 +834,15 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
                 a.tasks = ts.d_tasks + ts.off[X][Y];
                 a.ntasks = static_cast<int>(nt);
                 const ClassKernels& k = g_kernels[X][Y];
-                for (int ch = 0; ch < k.nchunk; ++ch) launch_jk(k, ch, a, nt, st);
+                const hipStream_t cs = concurrent ? pl->side[rr++ % kSideStreams] : st;
+                if (pl->timing) HIP_OK(hipEventRecord(pl->ev[X][Y][0], cs));
+                launch_jk(k, a, nt, cs);
+                if (pl->timing) HIP_OK(hipEventRecord(pl->ev[X][Y][1], cs));
+            }
+        if (concurrent)
+            for (int s = 0; s < kSideStreams; ++s) {
+                HIP_OK(hipEventRecord(pl->ev_join[s], pl->side[s]));
+                HIP_OK(hipStreamWaitEvent(st, pl->ev_join[s], 0));
             }
     }
     if (dJ) {
@@ -804,6 +863,17 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
     HIP_OK(hipStreamSynchronize(st));
     pl->last_shell_quartets = ts.shell_quartets;
     pl->last_prim_quartets = ts.prim_quartets;
+    pl->last_thr = thr;
+    if (pl->timing && (a.do_j || a.do_k)) {
+        for (int X = 0; X < 6; ++X)
+            for (int Y = 0; Y <= X; ++Y)
+                if (ts.cnt[X][Y]) {
+                    float ms = 0.0f;
+                    HIP_OK(hipEventElapsedTime(&ms, pl->ev[X][Y][0], pl->ev[X][Y][1]));
+                    pl->cls_ms[X][Y] += ms;
+                }
+        ++pl->timed_builds;
+    }
 }
 
 void validate_density(const double* D, int M) {
@@ -842,6 +912,16 @@ tlno_gpu_plan::~tlno_gpu_plan() {
     f(d_int2basis); f(d_intL); f(d_intloc); f(d_intsb); f(d_sphL); f(d_sphloc); f(d_sphib);
     f(dDin); f(dDint); f(dJacc); f(dKacc); f(dJout); f(dKout);
     for (auto& kv : tasks) f(kv.second.d_tasks);
+    if (timing)
+        for (int X = 0; X < 6; ++X)
+            for (int Y = 0; Y < 6; ++Y)
+                for (int e = 0; e < 2; ++e)
+                    if (ev[X][Y][e]) (void)hipEventDestroy(ev[X][Y][e]);
+    for (int s = 0; s < tlno_amd::kSideStreams; ++s) {
+        if (side[s]) (void)hipStreamDestroy(side[s]);
+        if (ev_join[s]) (void)hipEventDestroy(ev_join[s]);
+    }
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -885,6 +965,36 @@ int tlno_gpu_plan_info(const tlno_gpu_plan* pl, int* M, int* n_groups, int* n_pa
         if (n_groups) *n_groups = static_cast<int>(pl->groups.size());
         if (n_pairs) *n_pairs = pl->npair;
         if (n_prim_pairs) *n_prim_pairs = pl->nprim_pairs;
+    });
+}
+
+int tlno_gpu_set_timing(tlno_gpu_plan* pl, int enable) {
+    return guarded([&] {
+        if (!pl) throw std::runtime_error("jk_direct: null J/K plan backend.");
+        HIP_OK(hipSetDevice(pl->device));
+        if (enable && !pl->timing)
+            for (int X = 0; X < 6; ++X)
+                for (int Y = 0; Y < 6; ++Y)
+                    for (int e = 0; e < 2; ++e) HIP_OK(hipEventCreate(&pl->ev[X][Y][e]));
+        pl->timing = enable ? 1 : 0;
+        std::memset(pl->cls_ms, 0, sizeof(pl->cls_ms));
+        pl->timed_builds = 0;
+    });
+}
+
+int tlno_gpu_class_stats(const tlno_gpu_plan* pl, long long* shellq, long long* primq, double* ms,
+                         long long* builds) {
+    return guarded([&] {
+        if (!pl) throw std::runtime_error("jk_direct: null J/K plan backend.");
+        auto it = pl->tasks.find(pl->last_thr);
+        for (int X = 0; X < 6; ++X)
+            for (int Y = 0; Y < 6; ++Y) {
+                const int i = 6 * X + Y;
+                if (shellq) shellq[i] = it != pl->tasks.end() ? it->second.cls_sq[X][Y] : 0;
+                if (primq) primq[i] = it != pl->tasks.end() ? it->second.cls_pq[X][Y] : 0;
+                if (ms) ms[i] = pl->cls_ms[X][Y];
+            }
+        if (builds) *builds = pl->timed_builds;
     });
 }
 

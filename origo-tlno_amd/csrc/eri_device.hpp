@@ -227,7 +227,7 @@ constexpr double kTwoPi52 = 34.986836655249725693;  // 2 pi^{5/2}
 
 // Accumulates the lane's (unnormalized-component) block slice into acc.
 template <int LA, int LB, int LC, int LD, int AC, int CC>
-__device__ __forceinline__ void quartet(const DevTables& tb, const PairData& pd, int P, int Q,
+__device__ __forceinline__ void quartet(const DevTables& tb, const PairData& pd, int P, int Q, int ib0, int ib1,
                                         double (&acc)[Shape<LA, LB, LC, LD, AC, CC>::NOUT]) {
     using S = Shape<LA, LB, LC, LD, AC, CC>;
     constexpr CartT<LA> ca = make_cart<LA>();
@@ -241,17 +241,22 @@ __device__ __forceinline__ void quartet(const DevTables& tb, const PairData& pd,
     const double ABx = pd.AB[3 * P], ABy = pd.AB[3 * P + 1], ABz = pd.AB[3 * P + 2];
     const double Cx = pd.A[3 * Q], Cy = pd.A[3 * Q + 1], Cz = pd.A[3 * Q + 2];
     const double CDx = pd.AB[3 * Q], CDy = pd.AB[3 * Q + 1], CDz = pd.AB[3 * Q + 2];
-    const int pb0 = pd.poff[P], pbn = pd.pn[P];
+    // bra primitive pairs [ib0, ib1) of P (a task may own a slice of them;
+    // the block is linear in the primitives, so slices digest independently)
+    const int pb0 = pd.poff[P];
     const int pk0 = pd.poff[Q], pkn = pd.pn[Q];
 
-    for (int ib = 0; ib < pbn; ++ib) {
-        const double4 Pp = pd.pP[pb0 + ib];
-        const double wab = pd.pw[pb0 + ib];
-        const double p = Pp.w;
-        for (int ik = 0; ik < pkn; ++ik) {
-            const double4 Qp = pd.pP[pk0 + ik];
-            const double wcd = pd.pw[pk0 + ik];
-            const double q = Qp.w;
+    // ket primitives (per lane) outside, bra primitives inside: in the J/K
+    // kernel P is wave-uniform, so the inner loads are scalar-cache loads and
+    // each lane reads its ket primitive data once
+    for (int ik = 0; ik < pkn; ++ik) {
+        const double4 Qp = pd.pP[pk0 + ik];
+        const double wcd = pd.pw[pk0 + ik];
+        const double q = Qp.w;
+        for (int ib = ib0; ib < ib1; ++ib) {
+            const double4 Pp = pd.pP[pb0 + ib];
+            const double wab = pd.pw[pb0 + ib];
+            const double p = Pp.w;
             const double pq = p + q;
             const double ipq = 1.0 / pq;
             const double PQx = Pp.x - Qp.x, PQy = Pp.y - Qp.y, PQz = Pp.z - Qp.z;
@@ -339,20 +344,26 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// Task = {P, q_begin, q_end, bra primitive slice ib0 | ib1 << 16}.
 template <int LA, int LB, int LC, int LD, int AC, int CC>
-__global__ __launch_bounds__(256) void k_eri_jk(JKArgs a) {
+__device__ __forceinline__ void eri_jk_body(const JKArgs& a) {
     using S = Shape<LA, LB, LC, LD, AC, CC>;
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     if (wave >= a.ntasks) return;  // whole wave leaves together
     const int4 t = a.tasks[wave];
-    const int P = t.x;
-    const int qraw = t.y + lane;
-    const bool active = qraw < t.z;
-    const int Q = active ? qraw : t.z - 1;
+    // the task is wave-uniform: keep it in scalar registers
+    const int P = __builtin_amdgcn_readfirstlane(t.x);
+    const int qb = __builtin_amdgcn_readfirstlane(t.y);
+    const int qe = __builtin_amdgcn_readfirstlane(t.z);
+    const int tw = __builtin_amdgcn_readfirstlane(t.w);
+    const int qraw = qb + lane;
+    const bool active = qraw < qe;
+    const int Q = active ? qraw : qe - 1;
+    const int ib0 = tw & 0xffff, ib1 = (tw >> 16) & 0xffff;
 
     double acc[S::NOUT];
-    quartet<LA, LB, LC, LD, AC, CC>(a.tb, a.pd, P, Q, acc);
+    quartet<LA, LB, LC, LD, AC, CC>(a.tb, a.pd, P, Q, ib0, ib1, acc);
 
     // degeneracy scale: 1/2 per diagonal pair, 1/2 if P == Q
     double s = 1.0;
@@ -468,13 +479,13 @@ __global__ __launch_bounds__(256) void k_eri_jk(JKArgs a) {
 // Writes the lane's slice of the unit-diagonal Cartesian block of (P|Q) into
 // out[off + ((a*NB + b)*NC + c)*ND + d] (bra/ket shells in pair orientation).
 template <int LA, int LB, int LC, int LD, int AC, int CC>
-__global__ __launch_bounds__(256) void k_eri_store(StoreArgs a) {
+__device__ __forceinline__ void eri_store_body(const StoreArgs& a) {
     using S = Shape<LA, LB, LC, LD, AC, CC>;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     const int2 pq = a.quartets[i];
     double acc[S::NOUT];
-    quartet<LA, LB, LC, LD, AC, CC>(a.tb, a.pd, pq.x, pq.y, acc);
+    quartet<LA, LB, LC, LD, AC, CC>(a.tb, a.pd, pq.x, pq.y, 0, a.pd.pn[pq.x], acc);
     double* out = a.out + a.off[i];
     int o = 0;
 #pragma unroll
@@ -492,10 +503,11 @@ __global__ __launch_bounds__(256) void k_eri_store(StoreArgs a) {
 using JKKernel = void (*)(JKArgs);
 using StoreKernel = void (*)(StoreArgs);
 
+// One kernel per class pair; the output chunk is blockIdx.y.
 struct ClassKernels {
     int nchunk = 0;
-    JKKernel jk[36] = {};
-    StoreKernel store[36] = {};
+    JKKernel jk = nullptr;
+    StoreKernel store = nullptr;
 };
 
 // class index: 0 ss, 1 ps, 2 pp, 3 ds, 4 dp, 5 dd (pair orientation: higher L first)

@@ -71,21 +71,29 @@ __device__ void onee_pair(const OneeArgs& a, int P) {
         for (int x = 0; x < 3; ++x) {
             const double PA = Pv[x] - A[x];
             const double PB = PA + AB[x];  // P - B = P - A + (A - B)
-            double W[LA + LB + 3][LB + 3];
-            // vertical on i at j = 0, then transfer to j
-            W[0][0] = sp;  // exp(-mu AB^2) is folded into wK
+            // two-index Obara-Saika recursion (no transfer step: the transfer
+            // S(i, j+1) = S(i+1, j) + AB S(i, j) cancels badly for a tight
+            // function far from a diffuse one)
+            //   S(i+1, j) = PA S(i,j) + (i S(i-1,j) + j S(i,j-1)) / (2p)
+            //   S(i, j+1) = PB S(i,j) + (i S(i-1,j) + j S(i,j-1)) / (2p)
+            double (&W)[LA + 2][LB + 2] = S1[x];
 #pragma unroll
-            for (int i = 0; i < LA + LB + 2; ++i)
-                W[i + 1][0] = PA * W[i][0] + (i > 0 ? i * i2p * W[i - 1][0] : 0.0);
+            for (int j = 0; j <= LB + 1; ++j)
 #pragma unroll
-            for (int j = 0; j < LB + 2; ++j)
-#pragma unroll
-                for (int i = 0; i + j + 1 <= LA + LB + 2; ++i) W[i][j + 1] = W[i + 1][j] + AB[x] * W[i][j];
-            (void)PB;
-#pragma unroll
-            for (int i = 0; i <= LA + 1; ++i)
-#pragma unroll
-                for (int j = 0; j <= LB + 1; ++j) S1[x][i][j] = W[i][j];
+                for (int i = 0; i <= LA + 1; ++i) {
+                    double v;
+                    if (i == 0 && j == 0) {
+                        v = sp;  // exp(-mu AB^2) is folded into wK
+                    } else if (i > 0) {
+                        v = PA * W[i - 1][j];
+                        if (i > 1) v += (i - 1) * i2p * W[i > 1 ? i - 2 : 0][j];
+                        if (j > 0) v += j * i2p * W[i - 1][j > 0 ? j - 1 : 0];
+                    } else {
+                        v = PB * W[0][j - 1];
+                        if (j > 1) v += (j - 1) * i2p * W[0][j > 1 ? j - 2 : 0];
+                    }
+                    W[i][j] = v;
+                }
         }
 #pragma unroll
         for (int ia = 0; ia < NA; ++ia)

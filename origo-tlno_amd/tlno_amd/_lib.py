@@ -26,6 +26,8 @@ SIGNATURES = {
     "tlno_gpu_build_jk": (C.c_int, [C.c_void_p, D, D, D, C.c_double]),
     "tlno_gpu_build_jk_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double]),
     "tlno_gpu_last_build_stats": (C.c_int, [C.c_void_p, LL, LL]),
+    "tlno_gpu_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "tlno_gpu_class_stats": (C.c_int, [C.c_void_p, LL, LL, D, LL]),
     "tlno_gpu_eri_cart_blocks": (C.c_int, [C.c_void_p, C.c_int, I, LL, D]),
     "tlno_gpu_boys": (C.c_int, [C.c_int, D, C.c_int, D]),
     "tlno_gpu_rys": (C.c_int, [C.c_int, C.c_int, D, D, D]),

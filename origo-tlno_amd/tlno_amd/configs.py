@@ -89,6 +89,15 @@ def _random_atoms(rng, natoms, box_A, min_sep_A, probs=(0.3, 0.1, 0.1, 0.5)):
     return [(str(e), p * BOHR_PER_ANGSTROM) for e, p in zip(el, pos)]
 
 
+def spherical_density(M, seed, nocc):
+    """D = C C^T / ||C C^T||_F with C an M x nocc N(0,1) matrix (for the
+    spherical-basis builds of a workload)."""
+    rng = np.random.default_rng(seed)
+    Cm = rng.standard_normal((M, nocc))
+    D = Cm @ Cm.T
+    return D / np.linalg.norm(D)
+
+
 def water7(seed=SEED0 + 1):
     rng = np.random.default_rng(seed)
     eb = {"O": _draw_shells(rng, [0, 0, 1], (3, 3), (0.1, 150.0)),

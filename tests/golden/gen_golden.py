@@ -1,0 +1,138 @@
+"""Writes the golden fixtures in tests/golden/ from the reference engine compiled
+from its own sources (oracle/_ref, built by oracle/ref.mk; AVX-512 build =
+the reference's native W8 route).  Run in the build container only:
+
+    python tests/golden/gen_golden.py
+
+Fixtures hold inputs (seeded generators, re-derived and hash-checked by the
+tests) and the reference's outputs:
+
+  boys_rys.npz     boys_single(m, x) and boys_eval_hgp(m, x) samples;
+                   rys_roots_weights (production table route) and
+                   rys_roots_weights_accurate for N = 1..5
+  water7.npz       water, 7 functions: S/T/V (Cartesian and spherical),
+                   J/K/G at Schwarz 0 and 1e-12, every canonical ERI of the
+                   production spherical stream (lhs pair, rhs pair, value),
+                   the stream checksum
+  organic5.npz     5-atom organic fragment with Cartesian d shells: raw
+                   unit-diagonal Cartesian blocks of 60 group quadruples
+                   (audit route), spherical J/K/G at Schwarz 0, S/T/V,
+                   stream checksum
+  organic30.npz    the 340-function organic30 workload in the reference's
+                   spherical basis (321 functions), density
+                   configs.spherical_density(321, 5, 60): G = J - K/2 at
+                   Schwarz 1e-12 in full, and 16 bilinear probes u_i^T X v_i
+                   of J and K (python gen_golden.py organic30 rewrites only it)
+"""
+import hashlib
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path[:0] = [os.path.join(ROOT, "origo-tlno_amd"), ROOT]
+
+from tlno_amd import configs  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+
+def basis_hash(b):
+    h = hashlib.sha256()
+    for a in (b.centers, b.ang, b.nprim, b.exps, b.coefs):
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+sph_density = configs.spherical_density
+
+
+def main(only=None):
+    ref = O.Reference()
+    if only == "organic30":
+        return organic30(ref)
+    t0 = time.time()
+    # ---------------------------------------------------------- Boys / Rys
+    xs = np.concatenate([[0.0, 1e-10, 1e-3, 0.5, 5.0, 11.99, 12.0, 12.5, 25.0, 39.99, 40.0, 45.0, 80.0, 200.0],
+                         np.random.default_rng(11).uniform(0, 60, 200)])
+    boys = np.array([[ref.boys(m, x) for m in range(17)] for x in xs])
+    boys_hgp = np.array([[ref.boys_hgp(m, x) for m in range(17)] for x in xs])
+    Ts = np.concatenate([[0.0, 1e-8, 0.5, 1.0, 7.3, 28.99, 29.0, 36.99, 37.0, 42.99, 43.0, 48.99, 49.0, 54.99,
+                          55.0, 70.0, 150.0], np.random.default_rng(12).uniform(0, 80, 120)])
+    rys_tab = {}
+    for N in range(1, 6):
+        rys_tab[f"table_r{N}"] = np.array([ref.rys(N, T, 0)[0] for T in Ts])
+        rys_tab[f"table_w{N}"] = np.array([ref.rys(N, T, 0)[1] for T in Ts])
+        rys_tab[f"accurate_r{N}"] = np.array([ref.rys(N, T, 1)[0] for T in Ts])
+        rys_tab[f"accurate_w{N}"] = np.array([ref.rys(N, T, 1)[1] for T in Ts])
+    np.savez_compressed(os.path.join(HERE, "boys_rys.npz"), xs=xs, boys=boys, boys_hgp=boys_hgp, Ts=Ts, **rys_tab)
+
+    # ---------------------------------------------------------- water7
+    b, D = configs.water7()
+    out = {"basis_sha": basis_hash(b), "D": D}
+    for sph in (0, 1):
+        S, T, V = ref.onee(b, b.nuclear_charges, b.nuclear_centers, spherical=bool(sph))
+        out[f"S{sph}"], out[f"T{sph}"], out[f"V{sph}"] = S, T, V
+    for thr, tag in ((0.0, "0"), (1e-12, "12")):
+        for w in "JKG":
+            out[f"{w}_{tag}"] = ref.direct(b, D, w, schwarz=thr, threads=1)[0]
+    lhs, rhs, val = ref.sph_stream(b, 0.0)
+    out.update(stream_lhs=lhs, stream_rhs=rhs, stream_val=val, checksum=ref.checksum(b, 0.0))
+    np.savez_compressed(os.path.join(HERE, "water7.npz"), **out)
+
+    # ---------------------------------------------------------- organic5
+    b, _ = configs.organic30(natoms=5)
+    out = {"basis_sha": basis_hash(b)}
+    groups = ref.groups(b)
+    ng = len(groups)
+    rng = np.random.default_rng(21)
+    Ls = [g[0] for g in groups]
+    quads = []
+    byL = {L: [g for g in range(ng) if Ls[g] == L] for L in set(Ls)}
+    for la in sorted(byL):
+        for lb in sorted(byL):
+            for lc in sorted(byL):
+                for ld in sorted(byL):
+                    if len(quads) < 81:
+                        quads.append([int(rng.choice(byL[x])) for x in (la, lb, lc, ld)])
+    quads = quads[:60]
+    blocks = [ref.cart_block(b, *q, groups=groups).ravel() for q in quads]
+    out["quads"] = np.array(quads, dtype=np.int32)
+    out["block_off"] = np.concatenate([[0], np.cumsum([len(x) for x in blocks])]).astype(np.int64)
+    out["blocks"] = np.concatenate(blocks)
+    M = ref.plan_M(b)
+    Ds = sph_density(M, 31, 10)
+    out["Dsph"] = Ds
+    for w in "JKG":
+        out[f"{w}_0"] = ref.direct(b, Ds, w, schwarz=0.0, threads=1)[0]
+    for sph in (0, 1):
+        S, T, V = ref.onee(b, b.nuclear_charges, b.nuclear_centers, spherical=bool(sph), screening=False)
+        out[f"S{sph}"], out[f"T{sph}"], out[f"V{sph}"] = S, T, V
+    out["checksum"] = ref.checksum(b, 0.0)
+    np.savez_compressed(os.path.join(HERE, "organic5.npz"), **out)
+
+    organic30(ref)
+    print(f"golden fixtures written in {time.time() - t0:.1f} s")
+
+
+def organic30(ref):
+    # ---------------------------------------------------------- organic30
+    b, _ = configs.organic30()
+    M = ref.plan_M(b)
+    Ds = sph_density(M, 5, 60)
+    G, tg = ref.direct(b, Ds, "G", schwarz=1e-12, threads=1)
+    J, tj = ref.direct(b, Ds, "J", schwarz=1e-12, threads=8)
+    K, tk = ref.direct(b, Ds, "K", schwarz=1e-12, threads=1)
+    prng = np.random.default_rng(41)
+    U = prng.standard_normal((16, M))
+    Vv = prng.standard_normal((16, M))
+    np.savez_compressed(os.path.join(HERE, "organic30.npz"), basis_sha=basis_hash(b), G_12=G,
+                        U=U, Vp=Vv, J_probe=np.einsum("im,mn,in->i", U, J, Vv),
+                        K_probe=np.einsum("im,mn,in->i", U, K, Vv), J_fro=np.linalg.norm(J),
+                        K_fro=np.linalg.norm(K), ref_seconds=np.array([tg, tj, tk]))
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else None)

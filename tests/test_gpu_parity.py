@@ -7,7 +7,7 @@ Tolerances (stated per test):
          that vanish by symmetry or cancel: binary64 rounding of the product
          centres alone perturbs them by ~eps |A| sqrt(p) B (tight exponents,
          centres ~10 bohr), in every implementation including the reference.
-  S/T/V  |d| <= 1e-14 + 1e-12 |ref|.
+  S/T/V  |d| <= 1e-14 + 1e-12 |ref| + 1e-13 sqrt|X_aa X_bb|.
   J/K    max-abs <= 1e-10.
   Boys and Rys rules: relative 1e-13.
 """
@@ -160,5 +160,9 @@ def test_onee(small_organic, cartesian):
     ob = O.OracleBasis(b)
     S, T, V = ob.onee(b.nuclear_charges, b.nuclear_centers, spherical=not cartesian)
     for x, r in ((m.overlap, S), (m.kinetic, T), (m.nuclear, V)):
-        ok, worst = _close(x, r)
+        # S and T are positive and V negative definite, so sqrt|X_aa X_bb|
+        # is the scale of X_ab; 1e-13 of it is the rounding floor of
+        # entries that cancel (spherical combinations, distant pairs)
+        d = np.sqrt(np.abs(np.diag(r)))
+        ok, worst = _close(x, r, floor=1e-13 * np.outer(d, d))
         assert ok, worst

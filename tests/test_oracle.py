@@ -1,0 +1,346 @@
+"""CPU: pin the oracle against the reference's golden vectors (tests/golden/,
+written by gen_golden.py from the reference compiled from its own sources)
+and against what the mathematics fixes (closed forms, moment identities,
+permutational symmetry, translation invariance, exact high-precision values).
+"""
+import hashlib
+import os
+
+import mpmath as mp
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tlno_amd import configs
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    return np.load(os.path.join(GOLD, name))
+
+
+def basis_hash(b):
+    h = hashlib.sha256()
+    for a in (b.centers, b.ang, b.nprim, b.exps, b.coefs):
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def sph_tensor(ob, b):
+    """full spherical ERI tensor from the oracle's Cartesian blocks"""
+    M = ob.nsph
+    out = np.zeros((M, M, M, M))
+    ng = ob.ngroups
+    c2s = []
+    for (L, comps, sb) in ob.groups:
+        T = O.cart2sph(L)  # local canonical order
+        # component order -> local canonical order
+        loc = []
+        for c in comps:
+            lx, ly, lz = b.ang[c]
+            dx = L - lx
+            loc.append(dx * (dx + 1) // 2 + (dx - ly))
+        c2s.append(T[:, loc])
+    for a in range(ng):
+        for bb in range(ng):
+            for c in range(ng):
+                for d in range(ng):
+                    blk = ob.cart_block(a, bb, c, d)
+                    s = np.einsum("ia,jb,kc,ld,abcd->ijkl", c2s[a], c2s[bb], c2s[c], c2s[d], blk)
+                    sa, sbb, sc, sd = (ob.groups[g][2] for g in (a, bb, c, d))
+                    out[sa:sa + s.shape[0], sbb:sbb + s.shape[1], sc:sc + s.shape[2], sd:sd + s.shape[3]] = s
+    return out
+
+
+def pair_of(idx):
+    hi = int((np.sqrt(8 * idx + 1) - 1) // 2)
+    while hi * (hi + 1) // 2 > idx:
+        hi -= 1
+    while (hi + 1) * (hi + 2) // 2 <= idx:
+        hi += 1
+    return hi, idx - hi * (hi + 1) // 2
+
+
+# ----------------------------------------------------------------- inputs
+def test_generators_match_fixtures():
+    assert basis_hash(configs.water7()[0]) == str(load("water7.npz")["basis_sha"])
+    assert basis_hash(configs.organic30(natoms=5)[0]) == str(load("organic5.npz")["basis_sha"])
+    assert basis_hash(configs.organic30()[0]) == str(load("organic30.npz")["basis_sha"])
+
+
+# ------------------------------------------------------------------- Boys
+def test_boys_vs_reference():
+    g = load("boys_rys.npz")
+    ours = np.array([[O.boys(m, x) for m in range(17)] for x in g["xs"]])
+    # reference 1e table: 96-point Gauss-Legendre + Newton interpolation
+    assert np.max(np.abs(ours - g["boys"]) / g["boys"]) < 2e-13
+    # reference 2e table (5-point Newton, asymptote from x = 40 without the
+    # exp(-x) term): agrees to its own interpolation error
+    rel = np.abs(ours - g["boys_hgp"]) / g["boys"]
+    assert np.max(rel[g["xs"] < 40.0]) < 5e-12  # its interpolation error reaches 2.2e-12
+
+
+def test_boys_closed_forms():
+    for n in range(0, 17):
+        assert O.boys(n, 0.0) == pytest.approx(1.0 / (2 * n + 1), rel=1e-15)
+    mp.mp.dps = 40
+    rng = np.random.default_rng(3)
+    for _ in range(60):
+        n = int(rng.integers(0, 17))
+        x = float(rng.uniform(0, 80))
+        # F_n(x) = gamma(n+1/2) P(n+1/2, x) / (2 x^{n+1/2})
+        exact = mp.gammainc(n + mp.mpf(0.5), 0, x) / (2 * mp.mpf(x) ** (n + mp.mpf(0.5)))
+        assert abs(O.boys(n, x) - float(exact)) <= 1e-14 * float(exact)
+    # downward recursion identity F_n = (2x F_{n+1} + e^-x)/(2n+1)
+    for x in (0.3, 7.0, 33.0):
+        for n in range(10):
+            lhs = O.boys(n, x)
+            rhs = (2 * x * O.boys(n + 1, x) + np.exp(-x)) / (2 * n + 1)
+            assert abs(lhs - rhs) <= 1e-14 * lhs
+
+
+# -------------------------------------------------------------------- Rys
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5])
+def test_rys_vs_reference(N):
+    g = load("boys_rys.npz")
+    for i, T in enumerate(g["Ts"]):
+        r, w = O.rys(N, T)
+        # the reference's Stieltjes generator (same construction); from
+        # T_asym(N) on it returns the Hermite asymptote, good to ~1.1e-12
+        assert np.allclose(r, g[f"accurate_r{N}"][i], rtol=2e-12, atol=0)
+        assert np.allclose(w, g[f"accurate_w{N}"][i], rtol=2e-12, atol=0)
+        # production table route: 5-point Newton at dT = 0.01 and the
+        # Hermite asymptote from T_asym(N) on, both good to ~2e-12
+        assert np.allclose(r, g[f"table_r{N}"][i], rtol=5e-12, atol=0)
+        assert np.allclose(w, g[f"table_w{N}"][i], rtol=5e-12, atol=0)
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5])
+def test_rys_moment_identities(N):
+    for T in (0.0, 0.37, 3.0, 11.0, 28.0, 45.0, 66.0, 120.0):
+        r, w = O.rys(N, T)
+        assert np.all(np.diff(r) > 0) and r[0] >= 0 and r[-1] < 1 and np.all(w > 0)
+        for k in range(2 * N):
+            mom = float(np.sum(w * r ** k))
+            assert abs(mom - O.boys(k, T)) <= 1e-13 * O.boys(k, T), (N, T, k)
+    # N = 1 closed form: root F1/F0, weight F0
+    r, w = O.rys(1, 2.5)
+    assert r[0] == pytest.approx(O.boys(1, 2.5) / O.boys(0, 2.5), rel=1e-15)
+    assert w[0] == pytest.approx(O.boys(0, 2.5), rel=1e-15)
+
+
+def test_rys_table_route_matches_exact():
+    for N in range(1, 6):
+        for T in np.random.default_rng(N).uniform(0, 90, 50):
+            r0, w0 = O.rys(N, T)
+            r1, w1 = O.rys(N, T, fast=True)
+            assert np.allclose(r1, r0, rtol=2e-14, atol=0) and np.allclose(w1, w0, rtol=2e-14, atol=0)
+
+
+# ------------------------------------------------------------------- ERIs
+def test_eri_water7_vs_reference_stream():
+    g = load("water7.npz")
+    b, _ = configs.water7()
+    ob = O.OracleBasis(b)
+    E = sph_tensor(ob, b)
+    lhs, rhs, val = g["stream_lhs"], g["stream_rhs"], g["stream_val"]
+    M = ob.nsph
+    assert len(val) == (M * (M + 1) // 2) * (M * (M + 1) // 2 + 1) // 2 == 406
+    ours = np.array([E[pair_of(l) + pair_of(r)] for l, r in zip(lhs, rhs)])
+    assert np.all(np.abs(ours - val) <= 1e-14 + 1e-12 * np.abs(val))
+    assert float(np.sum(ours)) == pytest.approx(float(g["checksum"]), rel=1e-12)
+
+
+def test_eri_blocks_organic5_vs_reference():
+    g = load("organic5.npz")
+    b, _ = configs.organic30(natoms=5)
+    ob = O.OracleBasis(b)
+    Q = ob.schwarz(False)
+
+    def qi(i, j):
+        i, j = max(i, j), min(i, j)
+        return Q[i * (i + 1) // 2 + j]
+
+    for k, q in enumerate(g["quads"]):
+        ref = g["blocks"][g["block_off"][k]:g["block_off"][k + 1]]
+        ours = ob.cart_block(*[int(x) for x in q]).ravel()
+        floor = 1e-13 * qi(q[0], q[1]) * qi(q[2], q[3])
+        assert np.all(np.abs(ours - ref) <= 1e-14 + 1e-12 * np.abs(ref) + floor), q
+
+
+def test_eri_ss_closed_form():
+    # s-only three-centre basis: (ab|cd) = 2 pi^{5/2}/(pq sqrt(p+q)) K_ab K_cd F0(T) per primitive quartet
+    from tlno_amd.basis import Basis
+    rng = np.random.default_rng(9)
+    shells = [(rng.uniform(-2, 2, 3), 0, rng.uniform(0.2, 20, 2), rng.uniform(0.2, 1, 2)) for _ in range(3)]
+    b = Basis.from_shells(shells)
+    ob = O.OracleBasis(b)
+    mp.mp.dps = 30
+    off = np.concatenate([[0], np.cumsum(b.nprim)])
+
+    def prims(i):
+        return [(mp.mpf(b.exps[k]), mp.mpf(b.coefs[k])) for k in range(off[i], off[i + 1])], [mp.mpf(x) for x in b.centers[i]]
+
+    for (i, j, k, l) in [(0, 0, 0, 0), (0, 1, 2, 1), (2, 1, 0, 0), (1, 2, 2, 0)]:
+        (pa, A), (pb, B), (pc, Cc), (pd, Dd) = prims(i), prims(j), prims(k), prims(l)
+        tot = mp.mpf(0)
+        for a, ca in pa:
+            for bb, cb in pb:
+                for c, cc in pc:
+                    for d, cd in pd:
+                        p, q = a + bb, c + d
+                        P = [(a * A[x] + bb * B[x]) / p for x in range(3)]
+                        Qc = [(c * Cc[x] + d * Dd[x]) / q for x in range(3)]
+                        Kab = mp.e ** (-a * bb / p * sum((A[x] - B[x]) ** 2 for x in range(3)))
+                        Kcd = mp.e ** (-c * d / q * sum((Cc[x] - Dd[x]) ** 2 for x in range(3)))
+                        T = p * q / (p + q) * sum((P[x] - Qc[x]) ** 2 for x in range(3))
+                        F0 = mp.sqrt(mp.pi / (4 * T)) * mp.erf(mp.sqrt(T)) if T > 0 else mp.mpf(1)
+                        tot += ca * cb * cc * cd * 2 * mp.pi ** 2.5 / (p * q * mp.sqrt(p + q)) * Kab * Kcd * F0
+        assert ob.cart_block(i, j, k, l)[0, 0, 0, 0] == pytest.approx(float(tot), rel=1e-13)
+
+
+def test_eri_permutational_symmetry_and_translation():
+    b, _ = configs.organic30(natoms=3)
+    ob = O.OracleBasis(b)
+    ng = ob.ngroups
+    rng = np.random.default_rng(4)
+    for _ in range(25):
+        a, bb, c, d = (int(x) for x in rng.integers(0, ng, 4))
+        X = ob.cart_block(a, bb, c, d)
+        for perm, axes in (((bb, a, c, d), (1, 0, 2, 3)), ((a, bb, d, c), (0, 1, 3, 2)),
+                           ((c, d, a, bb), (2, 3, 0, 1)), ((d, c, bb, a), (3, 2, 1, 0))):
+            Y = ob.cart_block(*perm)
+            assert np.allclose(np.transpose(X, axes), Y, rtol=1e-12, atol=1e-14 * np.abs(X).max())
+    # translation invariance
+    b2, _ = configs.organic30(natoms=3)
+    b2.centers = b2.centers + np.array([3.1, -1.7, 0.9])
+    ob2 = O.OracleBasis(b2)
+    for _ in range(15):
+        q = tuple(int(x) for x in rng.integers(0, ng, 4))
+        X, Y = ob.cart_block(*q), ob2.cart_block(*q)
+        assert np.allclose(X, Y, rtol=1e-10, atol=1e-12 * max(1.0, np.abs(X).max()))
+
+
+# -------------------------------------------------------------------- J/K
+def test_jk_water7_vs_reference():
+    g = load("water7.npz")
+    b, D = configs.water7()
+    ob = O.OracleBasis(b)
+    for thr, tag in ((0.0, "0"), (1e-12, "12")):
+        J, K = ob.jk(D, spherical=True, schwarz=thr)
+        assert np.abs(J - g[f"J_{tag}"]).max() < 1e-10
+        assert np.abs(K - g[f"K_{tag}"]).max() < 1e-10
+        assert np.abs(J - 0.5 * K - g[f"G_{tag}"]).max() < 1e-10
+
+
+def test_jk_organic5_spherical_vs_reference():
+    g = load("organic5.npz")
+    b, _ = configs.organic30(natoms=5)
+    ob = O.OracleBasis(b)
+    J, K = ob.jk(g["Dsph"], spherical=True)
+    assert np.abs(J - g["J_0"]).max() < 1e-10
+    assert np.abs(K - g["K_0"]).max() < 1e-10
+    assert np.abs(J - 0.5 * K - g["G_0"]).max() < 1e-10
+
+
+def test_jk_identities():
+    b, D = configs.water7()
+    ob = O.OracleBasis(b)
+    E = sph_tensor(ob, b)
+    J, K = ob.jk(D, spherical=True)
+    assert np.allclose(J, np.einsum("abcd,cd->ab", E, D), atol=1e-13)
+    assert np.allclose(K, np.einsum("abcd,bd->ac", E, D), atol=1e-13)
+    assert np.allclose(J, J.T, atol=1e-13) and np.allclose(K, K.T, atol=1e-13)
+    # K is positive semidefinite for a PSD density (ERI tensor is PSD)
+    assert np.linalg.eigvalsh(K).min() > -1e-12
+
+
+# ------------------------------------------------------------------ S/T/V
+def test_onee_water7_vs_reference():
+    g = load("water7.npz")
+    b, _ = configs.water7()
+    ob = O.OracleBasis(b)
+    for sph in (0, 1):
+        S, T, V = ob.onee(b.nuclear_charges, b.nuclear_centers, spherical=bool(sph))
+        for x, r in ((S, g[f"S{sph}"]), (T, g[f"T{sph}"]), (V, g[f"V{sph}"])):
+            assert np.all(np.abs(x - r) <= 1e-14 + 1e-12 * np.abs(r) + 1e-14 * np.abs(r).max())
+        assert np.allclose(np.diag(S), 1.0, atol=1e-14)  # unit-diagonal convention
+
+
+def test_onee_organic5_overlap_nuclear_vs_reference():
+    g = load("organic5.npz")
+    b, _ = configs.organic30(natoms=5)
+    ob = O.OracleBasis(b)
+    for sph in (0, 1):
+        S, T, V = ob.onee(b.nuclear_charges, b.nuclear_centers, spherical=bool(sph))
+        for x, r in ((S, g[f"S{sph}"]), (V, g[f"V{sph}"])):
+            d = np.sqrt(np.abs(np.diag(r)))
+            assert np.all(np.abs(x - r) <= 1e-14 + 1e-12 * np.abs(r) + 1e-13 * np.outer(d, d))
+
+
+def _exact_kinetic(b, ia, ib):
+    mp.mp.dps = 40
+    off = np.concatenate([[0], np.cumsum(b.nprim)])
+
+    def S1(i, j, a, bb, A, B):
+        p = a + bb
+        P = (a * A + bb * B) / p
+        K = mp.e ** (-a * bb / p * (A - B) ** 2)
+        tot = mp.mpf(0)
+        for k in range(i + 1):
+            for l in range(j + 1):
+                n = k + l
+                if n % 2:
+                    continue
+                tot += mp.binomial(i, k) * mp.binomial(j, l) * (P - A) ** (i - k) * (P - B) ** (j - l) * \
+                    mp.gamma(mp.mpf(n + 1) / 2) / p ** (mp.mpf(n + 1) / 2)
+        return K * tot
+
+    A, B = b.centers[ia], b.centers[ib]
+    tot = mp.mpf(0)
+    for x in range(off[ia], off[ia + 1]):
+        for y in range(off[ib], off[ib + 1]):
+            al, be = mp.mpf(b.exps[x]), mp.mpf(b.exps[y])
+            s, t = [], []
+            for k in range(3):
+                i, j = int(b.ang[ia][k]), int(b.ang[ib][k])
+                Ak, Bk = mp.mpf(A[k]), mp.mpf(B[k])
+                tv = 4 * al * be * S1(i + 1, j + 1, al, be, Ak, Bk)
+                if i > 0:
+                    tv -= 2 * i * be * S1(i - 1, j + 1, al, be, Ak, Bk)
+                if j > 0:
+                    tv -= 2 * j * al * S1(i + 1, j - 1, al, be, Ak, Bk)
+                if i > 0 and j > 0:
+                    tv += i * j * S1(i - 1, j - 1, al, be, Ak, Bk)
+                s.append(S1(i, j, al, be, Ak, Bk))
+                t.append(tv / 2)
+            tot += mp.mpf(b.coefs[x]) * mp.mpf(b.coefs[y]) * (t[0] * s[1] * s[2] + s[0] * t[1] * s[2] + s[0] * s[1] * t[2])
+    return float(tot)
+
+
+def test_kinetic_exact_values_and_reference_cancellation():
+    """The oracle's kinetic matrix matches a 40-digit evaluation; the
+    reference's one-sided form loses digits for tight/diffuse pairs
+    (exponents 5000 vs 0.08), documented here rather than matched."""
+    g = load("organic5.npz")
+    b, _ = configs.organic30(natoms=5)
+    ob = O.OracleBasis(b)
+    S, T, V = ob.onee(b.nuclear_charges, b.nuclear_centers)
+    Tr = g["T0"]
+    worst = np.unravel_index(np.argmax(np.abs(T - Tr)), T.shape)
+    for (i, j) in (worst, (0, 0), (3, 7)):
+        exact = _exact_kinetic(b, i, j)
+        assert abs(T[i, j] - exact) <= 1e-14 + 1e-12 * abs(exact)
+    i, j = worst
+    assert abs(Tr[i, j] - _exact_kinetic(b, i, j)) > 1e-9  # reference error at that entry
+
+
+def test_sph_transform_pins_reference_numbering():
+    """spherical S of the oracle equals the reference's (numbering, phases, norms)"""
+    g = load("organic5.npz")
+    b, _ = configs.organic30(natoms=5)
+    ob = O.OracleBasis(b)
+    S, _, _ = ob.onee(b.nuclear_charges, b.nuclear_centers, spherical=True)
+    assert np.allclose(S, g["S1"], atol=1e-13)
+    assert np.allclose(np.diag(S), 1.0, atol=1e-13)
