@@ -88,6 +88,40 @@ def gpu_run(b, D, steps, warmup, thr):
     return plan, wall, classes, J, K
 
 
+def mixed_run(steps=3, gate=1e-6):
+    """water40 (1000 functions): FP64 GPU build vs the mixed-precision build
+    (FP32 recurrences for Q_ab Q_cd max|D| < gate), both from HBM-resident D."""
+    lib = _lib.load()
+    b, D = configs.water40()
+    thr = 1e-12
+    plan = DirectPlan(b, ScreeningOptions(integral_threshold=thr), cartesian=True)
+    M = plan.M
+    dD = hipmem.DeviceArray.from_host(D)
+    dJ = hipmem.DeviceArray((M, M))
+    dK = hipmem.DeviceArray((M, M))
+    out = {}
+    for name, g in (("fp64", 0.0), ("mixed", gate)):
+        _lib.check(lib.tlno_gpu_set_mixed_precision(plan.handle, g))
+        _lib.check(lib.tlno_gpu_build_jk_device(plan.handle, dD.ptr, dJ.ptr, dK.ptr, thr))
+        hipmem.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            _lib.check(lib.tlno_gpu_build_jk_device(plan.handle, dD.ptr, dJ.ptr, dK.ptr, thr))
+        hipmem.synchronize()
+        out[name] = ((time.perf_counter() - t0) / steps, dJ.to_host(), dK.to_host())
+    st = plan.last_mixed_stats()
+    sq, pq = plan.last_build_stats()
+    _lib.check(lib.tlno_gpu_set_mixed_precision(plan.handle, 0.0))
+    (t64, J0, K0), (tm, J1, K1) = out["fp64"], out["mixed"]
+    return {"workload": "water40 (40 waters, 1000 Cartesian functions, Schwarz 1e-12)",
+            "fp32_gate": gate, "fp64_ms": t64 * 1e3, "mixed_ms": tm * 1e3, "speedup": t64 / tm,
+            "fp32_share_shell_quartets": st["fp32_shell_quartets"] / sq,
+            "fp32_share_prim_quartets": st["fp32_prim_quartets"] / pq,
+            "max_abs_err_J": float(np.abs(J1 - J0).max()), "max_abs_err_K": float(np.abs(K1 - K0).max()),
+            "rms_err_J": float(np.sqrt(np.mean((J1 - J0) ** 2))),
+            "rms_err_K": float(np.sqrt(np.mean((K1 - K0) ** 2)))}
+
+
 def cpu_baseline(b, seconds_cap=60.0):
     try:
         from oracle import oracle as O
@@ -120,6 +154,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="organic30", choices=["organic30", "water40"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the J/K build shards by "
@@ -162,6 +197,8 @@ def main():
                      "whole_build_tflops": total_gflop / (ms_per * 1e-3) / 1e3},
         "classes": [{k: (round(v, 6) if isinstance(v, float) else v) for k, v in c.items()} for c in classes],
     }
+    if not args.no_mixed:
+        result["mixed_precision"] = mixed_run()
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(b)
         if result["cpu_baseline"]["value"]:

@@ -61,6 +61,21 @@ int tlno_gpu_plan_info(const tlno_gpu_plan* plan, int* M, int* n_groups,
  * (rys_engine.ipp:1966-2062). */
 int tlno_gpu_plan_groups(const tlno_gpu_plan* plan, int max_groups, int* L, int* comps);
 
+/* Shell pairs of the plan in the plan's order (class ss, ps, pp, ds, dp, dd,
+ * then descending Schwarz bound): bra/ket-side group of each pair (higher L
+ * first), Schwarz bound Q (sqrt max|(ab|ab)| in the plan's basis), and the
+ * first pair index of each class, class_begin[0..6].  Any pointer may be
+ * NULL.  Counterpart of the reference's pair-group list and pair_group_max
+ * (rys_engine.ipp:2436-2460, 16164-16258). */
+int tlno_gpu_plan_pairs(const tlno_gpu_plan* plan, int* group_a, int* group_b, double* Q,
+                        int* class_begin);
+
+/* max |(ab|cd)| over the unit-diagonal Cartesian block of each shell quartet
+ * (pair_p[i] | pair_q[i]) given as plan pair indices (training labels for the
+ * learned screen; no reference counterpart). */
+int tlno_gpu_quartet_maxabs(tlno_gpu_plan* plan, int n, const int* pair_p, const int* pair_q,
+                            double* out);
+
 /* Replaces build_coulomb_j / build_exchange_k / build_rhf_g
  * (jk_direct.hpp:80-96) and build_jk_direct_for_G_rys (eri_direct_jk.hpp:36-39):
  * J[a,b] = sum_cd P[c,d](ab|cd), K[a,c] = sum_bd P[b,d](ab|cd), G = J - K/2.
@@ -76,6 +91,26 @@ int tlno_gpu_build_jk(tlno_gpu_plan* plan, const double* density, double* J,
  * validation is done on this path. */
 int tlno_gpu_build_jk_device(tlno_gpu_plan* plan, const double* d_density,
                              double* d_J, double* d_K, double schwarz);
+
+/* Density-coupled screening, EriPackedBuildOptions::density_threshold /
+ * ScreeningOptions::density_threshold (twoe_integrals.hpp:58-64,
+ * jk_direct.hpp:18-24): following builds also skip shell quartets with
+ * Q_ab Q_cd * 4 max(g(ab), g_max(class of cd)) < density_threshold, where g(x) is
+ * the largest |D| in the rows of the shells of pair x.  That factor bounds the
+ * reference's CVHF factor max(4 max(rho_ab, rho_cd), cross blocks) from above,
+ * so no quartet the reference keeps is dropped.  0 turns it off (default). */
+int tlno_gpu_set_density_threshold(tlno_gpu_plan* plan, double density_threshold);
+
+/* Mixed precision (no reference counterpart): with fp32_threshold > 0,
+ * following builds of this plan evaluate the 2D recurrences and the Cartesian
+ * contraction of every shell quartet with Q_ab Q_cd max|D| < fp32_threshold in
+ * FP32 (Rys roots and weights, T and the prefactor stay FP64; every J/K
+ * accumulation is FP64).  0 restores FP64 everywhere (the default).
+ * tlno_gpu_last_mixed_stats reports the FP32 share of the last build and the
+ * max|D| it gated with. */
+int tlno_gpu_set_mixed_precision(tlno_gpu_plan* plan, double fp32_threshold);
+int tlno_gpu_last_mixed_stats(const tlno_gpu_plan* plan, long long* fp32_shell_quartets,
+                              long long* fp32_prim_quartets, double* max_abs_density);
 
 /* Work statistics of the last build: shell quartets and primitive quartets
  * evaluated (after Schwarz screening). */

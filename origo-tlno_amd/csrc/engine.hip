@@ -226,6 +226,10 @@ struct TaskSet {
     long long prim_quartets = 0;
     long long cls_sq[6][6] = {};  // shell quartets per class pair
     long long cls_pq[6][6] = {};  // primitive quartets per class pair
+    // per class pair, in task order: max bound of each task (descending) and
+    // running shell/primitive quartet counts (for the precision split)
+    std::vector<double> bmax[6][6];
+    std::vector<long long> cum_sq[6][6], cum_pq[6][6];
 };
 
 }  // namespace tlno_amd
@@ -260,6 +264,23 @@ struct tlno_gpu_plan {
     std::map<double, tlno_amd::TaskSet> tasks;
     long long last_shell_quartets = 0, last_prim_quartets = 0;
     double last_thr = -1.0;
+    // mixed precision: shell quartets with Q_ab Q_cd max|D| < fp32_gate run
+    // the FP32 recurrences (0 = FP64 everywhere)
+    double fp32_gate = 0.0;
+    double* d_scalar = nullptr;  // device scratch for max|D|
+    // density screen (the reference's density_threshold): 0 = off
+    double density_thr = 0.0;
+    int* d_row_group = nullptr;              // [Mout] shell of each row of the caller's basis
+    int *d_pgA = nullptr, *d_pgB = nullptr, *d_pcls = nullptr, *d_clsb = nullptr;
+    double* d_Q = nullptr;                   // [npair] bounds in plan order
+    unsigned long long* d_gmax = nullptr;    // [ngroups]
+    unsigned long long* d_clsmax = nullptr;  // [6]
+    double* d_pairfac = nullptr;             // [npair]
+    int* d_cut = nullptr;                    // [6 npair]
+    unsigned long long* d_counter = nullptr; // shell quartets evaluated by the last build
+    long long last_evaluated = 0, last_evaluated_prim = 0;
+    double last_maxD = 0.0;
+    long long last_fp32_sq = 0, last_fp32_pq = 0;
     // optional per-class kernel timing with HIP events on the plan stream
     int timing = 0;
     hipEvent_t ev[6][6][2] = {};
@@ -461,10 +482,66 @@ __global__ void k_rys(DevTables tb, const double* T, double* r, double* w, int n
 inline unsigned grid_for(long n, int block) { return static_cast<unsigned>((n + block - 1) / block); }
 
 // grid: x covers the tasks (one wave each, 4 waves per block), y the chunks
-void launch_jk(const ClassKernels& k, JKArgs& args, long long ntask, hipStream_t st) {
+void launch_jk(JKKernel kern, int nchunk, JKArgs& args, long long ntask, hipStream_t st) {
     void* params[] = {&args};
     const unsigned grid = grid_for(ntask * 64, 256);
-    HIP_OK(hipLaunchKernel(reinterpret_cast<const void*>(k.jk), dim3(grid, k.nchunk), dim3(256), params, 0, st));
+    HIP_OK(hipLaunchKernel(reinterpret_cast<const void*>(kern), dim3(grid, nchunk), dim3(256), params, 0, st));
+}
+
+// ---- density screen (per build) ----
+// gmax[g] = max |D[i][j]| over the rows i of shell g (caller's basis)
+__global__ void k_group_rowmax(const double* __restrict__ D, const int* row_group, int M,
+                               unsigned long long* gmax) {
+    const int i = blockIdx.x;
+    double m = 0.0;
+    for (int j = threadIdx.x; j < M; j += blockDim.x) m = fmax(m, fabs(D[(long)i * M + j]));
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) m = fmax(m, __shfl_xor(m, s, 64));
+    if ((threadIdx.x & 63) == 0)
+        atomicMax(gmax + row_group[i], static_cast<unsigned long long>(__double_as_longlong(m)));
+}
+
+// pairfac[P] = max(gmax[A], gmax[B]); clsmax[class] = max over the class
+__global__ void k_pair_factor(const unsigned long long* gmax, const int* gA, const int* gB, const int* pcls,
+                              int npair, double* pairfac, unsigned long long* clsmax) {
+    const int P = blockIdx.x * blockDim.x + threadIdx.x;
+    if (P >= npair) return;
+    const double g = fmax(__longlong_as_double(gmax[gA[P]]), __longlong_as_double(gmax[gB[P]]));
+    pairfac[P] = g;
+    atomicMax(clsmax + pcls[P], static_cast<unsigned long long>(__double_as_longlong(g)));
+}
+
+// cut[6 P + Y]: end of the ket prefix of class Y kept for bra pair P, from
+// Q_P Q_q 4 max(g(P), max_Y g) >= tau — an upper bound of the reference's
+// CVHF factor max(4 max(rho_ab, rho_cd), cross blocks), since every density
+// block touching the quartet lies in the rows of one of its four shells.
+__global__ void k_density_cut(const double* Q, const int* pcls, const int* cls_begin, const double* pairfac,
+                              const unsigned long long* clsmax, int npair, double tau, int* cut) {
+    const int P = blockIdx.x * blockDim.x + threadIdx.x;
+    if (P >= npair) return;
+    const int X = pcls[P];
+    for (int Y = 0; Y <= X; ++Y) {
+        const double f = 4.0 * fmax(pairfac[P], __longlong_as_double(clsmax[Y]));
+        const double qp = Q[P] * f;
+        int lo = cls_begin[Y], hi = cls_begin[Y + 1];
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (qp * Q[mid] >= tau) lo = mid + 1;
+            else hi = mid;
+        }
+        cut[6 * P + Y] = lo;
+    }
+}
+
+// max |x| over n entries into *out (zeroed by the caller); non-negative
+// doubles order like their bit patterns, so an integer max suffices
+__global__ void k_absmax(const double* __restrict__ x, long n, unsigned long long* out) {
+    double m = 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        m = fmax(m, fabs(x[i]));
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) m = fmax(m, __shfl_xor(m, s, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, static_cast<unsigned long long>(__double_as_longlong(m)));
 }
 
 void launch_store(const ClassKernels& k, StoreArgs& args, hipStream_t st) {
@@ -723,6 +800,32 @@ void build_plan(tlno_gpu_plan* pl, int n, const double* centers, const int* ang,
     HIP_OK(hipMalloc(&pl->dKacc, mc2 * sizeof(double)));
     HIP_OK(hipMalloc(&pl->dJout, mo2 * sizeof(double)));
     HIP_OK(hipMalloc(&pl->dKout, mo2 * sizeof(double)));
+    HIP_OK(hipMalloc(&pl->d_scalar, 8 * sizeof(double)));
+    // density-screen metadata
+    {
+        std::vector<int> row_group(pl->Mout), pcls(np), clsb(7);
+        for (int g = 0; g < ng; ++g) {
+            const GroupH& G = pl->groups[g];
+            if (pl->spherical)
+                for (int s = 0; s < 2 * G.L + 1; ++s) row_group[G.sph_base + s] = g;
+            else
+                for (int k = 0; k < ncart(G.L); ++k) row_group[G.comp[k]] = g;
+        }
+        for (int c = 0; c < 6; ++c)
+            for (int P = pl->cls_begin[c]; P < pl->cls_begin[c + 1]; ++P) pcls[P] = c;
+        for (int c = 0; c <= 6; ++c) clsb[c] = pl->cls_begin[c];
+        dev_upload(pl->d_row_group, row_group);
+        dev_upload(pl->d_pgA, pl->pgA);
+        dev_upload(pl->d_pgB, pl->pgB);
+        dev_upload(pl->d_pcls, pcls);
+        dev_upload(pl->d_clsb, clsb);
+        dev_upload(pl->d_Q, pl->Q);
+        HIP_OK(hipMalloc(&pl->d_gmax, std::max(1, ng) * sizeof(unsigned long long)));
+        HIP_OK(hipMalloc(&pl->d_clsmax, 8 * sizeof(unsigned long long)));
+        HIP_OK(hipMalloc(&pl->d_pairfac, std::max(1, np) * sizeof(double)));
+        HIP_OK(hipMalloc(&pl->d_cut, 6 * static_cast<std::size_t>(std::max(1, np)) * sizeof(int)));
+        HIP_OK(hipMalloc(&pl->d_counter, 2 * sizeof(unsigned long long)));
+    }
 }
 
 const TaskSet& get_tasks(tlno_gpu_plan* pl, double thr) {
@@ -769,14 +872,36 @@ const TaskSet& get_tasks(tlno_gpu_plan* pl, double thr) {
                 int split = 1;
                 if (waves > 0 && waves < kTargetWaves)
                     split = static_cast<int>(std::min<long long>(256, (kTargetWaves + waves - 1) / waves));
+                std::vector<int4> cls;
                 for (const int4& t : base) {
                     const int pbn = pl->pn_host[t.x];
                     const int s = std::min(split, pbn);
                     const int step = (pbn + s - 1) / s;
                     for (int i0 = 0; i0 < pbn; i0 += step) {
                         const int i1 = std::min(pbn, i0 + step);
-                        all.push_back(make_int4(t.x, t.y, t.z, i0 | (i1 << 16)));
+                        cls.push_back(make_int4(t.x, t.y, t.z, i0 | (i1 << 16)));
                     }
+                }
+                // Order the class's tasks by their largest bound Q_P Q_q_begin
+                // (kets are bound-sorted, so q_begin carries the task's max):
+                // heavy tasks start first, and the tasks below any gate on
+                // bound x max|D| form a suffix (the FP32 share of the
+                // mixed-precision route).
+                std::stable_sort(cls.begin(), cls.end(), [&](const int4& u, const int4& v) {
+                    return pl->Q[u.x] * pl->Q[u.y] > pl->Q[v.x] * pl->Q[v.y];
+                });
+                auto& bm = ts.bmax[X][Y];
+                auto& csq = ts.cum_sq[X][Y];
+                auto& cpq = ts.cum_pq[X][Y];
+                csq.push_back(0);
+                cpq.push_back(0);
+                for (const int4& t : cls) {
+                    bm.push_back(pl->Q[t.x] * pl->Q[t.y]);
+                    const int qe = std::min(t.z, t.y + 64);
+                    const long long ib0 = t.w & 0xffff, ib1 = (t.w >> 16) & 0xffff;
+                    csq.push_back(csq.back() + (ib0 == 0 ? qe - t.y : 0));
+                    cpq.push_back(cpq.back() + (pref[qe] - pref[t.y]) * (ib1 - ib0));
+                    all.push_back(t);
                 }
             }
             ts.cnt[X][Y] = static_cast<long long>(all.size()) - ts.off[X][Y];
@@ -816,6 +941,32 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
     a.M = pl->Mcart;
     a.do_j = dJ != nullptr;
     a.do_k = dK != nullptr;
+    double maxD = 0.0;
+    if (pl->fp32_gate > 0.0 && (a.do_j || a.do_k)) {
+        // max |D| in the caller's basis, the same basis as the Schwarz bounds
+        k_fill<<<1, 64, 0, st>>>(pl->d_scalar, 1, 0.0);
+        k_absmax<<<static_cast<unsigned>(std::min<long>(1024, grid_for(mo2, 256))), 256, 0, st>>>(
+            dD, mo2, reinterpret_cast<unsigned long long*>(pl->d_scalar));
+        HIP_OK(hipMemcpyAsync(&maxD, pl->d_scalar, sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+    }
+    pl->last_maxD = maxD;
+    a.cut = nullptr;
+    a.qlist = nullptr;
+    a.counter = pl->d_counter;
+    k_fill<<<1, 64, 0, st>>>(reinterpret_cast<double*>(pl->d_counter), 2, 0.0);
+    if (pl->density_thr > 0.0 && (a.do_j || a.do_k)) {
+        const int ng = static_cast<int>(pl->groups.size());
+        k_fill<<<grid_for(ng, 256), 256, 0, st>>>(reinterpret_cast<double*>(pl->d_gmax), ng, 0.0);
+        k_fill<<<1, 64, 0, st>>>(reinterpret_cast<double*>(pl->d_clsmax), 8, 0.0);
+        k_group_rowmax<<<pl->Mout, 256, 0, st>>>(dD, pl->d_row_group, pl->Mout, pl->d_gmax);
+        k_pair_factor<<<grid_for(pl->npair, 256), 256, 0, st>>>(pl->d_gmax, pl->d_pgA, pl->d_pgB, pl->d_pcls,
+                                                                pl->npair, pl->d_pairfac, pl->d_clsmax);
+        k_density_cut<<<grid_for(pl->npair, 256), 256, 0, st>>>(pl->d_Q, pl->d_pcls, pl->d_clsb, pl->d_pairfac,
+                                                                pl->d_clsmax, pl->npair, pl->density_thr,
+                                                                pl->d_cut);
+        HIP_OK(hipGetLastError());
+    }
     if (a.do_j || a.do_k) {
         // Class-pair kernels are independent (their J/K contributions are
         // atomic adds), so they run concurrently on the plan's side streams,
@@ -827,16 +978,35 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
             for (int s = 0; s < kSideStreams; ++s) HIP_OK(hipStreamWaitEvent(pl->side[s], pl->ev_fork, 0));
         }
         int rr = 0;
+        pl->last_fp32_sq = pl->last_fp32_pq = 0;
         for (int X = 5; X >= 0; --X)
             for (int Y = X; Y >= 0; --Y) {
                 const long long nt = ts.cnt[X][Y];
                 if (nt == 0) continue;
-                a.tasks = ts.d_tasks + ts.off[X][Y];
-                a.ntasks = static_cast<int>(nt);
+                // FP64 prefix / FP32 suffix of the bound-ordered task list
+                long long n64 = nt;
+                if (pl->fp32_gate > 0.0) {
+                    const auto& bm = ts.bmax[X][Y];
+                    const double cut = pl->fp32_gate / std::max(maxD, 1e-300);
+                    n64 = std::lower_bound(bm.begin(), bm.end(), cut, [](double v, double c) { return v >= c; }) -
+                          bm.begin();
+                    pl->last_fp32_sq += ts.cum_sq[X][Y][nt] - ts.cum_sq[X][Y][n64];
+                    pl->last_fp32_pq += ts.cum_pq[X][Y][nt] - ts.cum_pq[X][Y][n64];
+                }
                 const ClassKernels& k = g_kernels[X][Y];
                 const hipStream_t cs = concurrent ? pl->side[rr++ % kSideStreams] : st;
                 if (pl->timing) HIP_OK(hipEventRecord(pl->ev[X][Y][0], cs));
-                launch_jk(k, a, nt, cs);
+                a.cut = pl->density_thr > 0.0 ? pl->d_cut + Y : nullptr;
+                if (n64 > 0) {
+                    a.tasks = ts.d_tasks + ts.off[X][Y];
+                    a.ntasks = static_cast<int>(n64);
+                    launch_jk(k.jk, k.nchunk, a, n64, cs);
+                }
+                if (nt > n64) {
+                    a.tasks = ts.d_tasks + ts.off[X][Y] + n64;
+                    a.ntasks = static_cast<int>(nt - n64);
+                    launch_jk(k.jk32, k.nchunk, a, nt - n64, cs);
+                }
                 if (pl->timing) HIP_OK(hipEventRecord(pl->ev[X][Y][1], cs));
             }
         if (concurrent)
@@ -861,6 +1031,12 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
     }
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(st));
+    {
+        unsigned long long evaluated[2] = {0, 0};
+        HIP_OK(hipMemcpy(evaluated, pl->d_counter, sizeof(evaluated), hipMemcpyDeviceToHost));
+        pl->last_evaluated = static_cast<long long>(evaluated[0]);
+        pl->last_evaluated_prim = static_cast<long long>(evaluated[1]);
+    }
     pl->last_shell_quartets = ts.shell_quartets;
     pl->last_prim_quartets = ts.prim_quartets;
     pl->last_thr = thr;
@@ -910,7 +1086,9 @@ tlno_gpu_plan::~tlno_gpu_plan() {
     };
     f(dA); f(dAB); f(dpw); f(dao); f(dpoff); f(dpn); f(ddiag); f(dpP); f(dpexp); f(dpL);
     f(d_int2basis); f(d_intL); f(d_intloc); f(d_intsb); f(d_sphL); f(d_sphloc); f(d_sphib);
-    f(dDin); f(dDint); f(dJacc); f(dKacc); f(dJout); f(dKout);
+    f(dDin); f(dDint); f(dJacc); f(dKacc); f(dJout); f(dKout); f(d_scalar);
+    f(d_row_group); f(d_pgA); f(d_pgB); f(d_pcls); f(d_clsb); f(d_Q); f(d_gmax); f(d_clsmax); f(d_pairfac);
+    f(d_cut); f(d_counter);
     for (auto& kv : tasks) f(kv.second.d_tasks);
     if (timing)
         for (int X = 0; X < 6; ++X)
@@ -998,6 +1176,98 @@ int tlno_gpu_class_stats(const tlno_gpu_plan* pl, long long* shellq, long long* 
     });
 }
 
+int tlno_gpu_plan_pairs(const tlno_gpu_plan* pl, int* group_a, int* group_b, double* Q, int* class_begin) {
+    return guarded([&] {
+        if (!pl) throw std::runtime_error("jk_direct: null J/K plan backend.");
+        for (int k = 0; k < pl->npair; ++k) {
+            if (group_a) group_a[k] = pl->pgA[k];
+            if (group_b) group_b[k] = pl->pgB[k];
+            if (Q) Q[k] = pl->Q[k];
+        }
+        if (class_begin)
+            for (int c = 0; c <= 6; ++c) class_begin[c] = pl->cls_begin[c];
+    });
+}
+
+int tlno_gpu_quartet_maxabs(tlno_gpu_plan* pl, int n, const int* pp, const int* pq, double* out) {
+    return guarded([&] {
+        if (!pl) throw std::runtime_error("jk_direct: null J/K plan backend.");
+        auto cls_of_pair = [&](int P) {
+            int c = 0;
+            while (!(P >= pl->cls_begin[c] && P < pl->cls_begin[c + 1])) ++c;
+            return c;
+        };
+        auto nval = [&](int P, int Q) {
+            return static_cast<long long>(ncart(pl->groups[pl->pgA[P]].L)) * ncart(pl->groups[pl->pgB[P]].L) *
+                   ncart(pl->groups[pl->pgA[Q]].L) * ncart(pl->groups[pl->pgB[Q]].L);
+        };
+        std::map<std::pair<int, int>, std::vector<int>> by_class;
+        for (int i = 0; i < n; ++i) {
+            if (pp[i] < 0 || pp[i] >= pl->npair || pq[i] < 0 || pq[i] >= pl->npair)
+                throw std::runtime_error("tlno_gpu: pair index out of range");
+            int X = cls_of_pair(pp[i]), Y = cls_of_pair(pq[i]);
+            by_class[{std::max(X, Y), std::min(X, Y)}].push_back(i);
+        }
+        for (auto& kv : by_class) {
+            const auto& idx = kv.second;
+            for (std::size_t b0 = 0; b0 < idx.size(); b0 += 262144) {
+                const std::size_t b1 = std::min(idx.size(), b0 + 262144);
+                std::vector<int2> qs;
+                std::vector<long long> off;
+                long long total = 0;
+                for (std::size_t j = b0; j < b1; ++j) {
+                    int P = pp[idx[j]], Q = pq[idx[j]];
+                    if (cls_of_pair(P) < cls_of_pair(Q)) std::swap(P, Q);
+                    qs.push_back(make_int2(P, Q));
+                    off.push_back(total);
+                    total += nval(P, Q);
+                }
+                off.push_back(total);
+                double* dout = nullptr;
+                HIP_OK(hipMalloc(&dout, static_cast<std::size_t>(total) * sizeof(double)));
+                std::vector<long long> offq(off.begin(), off.end() - 1);
+                run_store(pl, kv.first.first, kv.first.second, qs, offq, dout);
+                std::vector<double> h(static_cast<std::size_t>(total));
+                HIP_OK(hipMemcpy(h.data(), dout, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+                HIP_OK(hipFree(dout));
+                for (std::size_t j = b0; j < b1; ++j) {
+                    double m = 0.0;
+                    for (long long t = off[j - b0]; t < off[j - b0 + 1]; ++t) m = std::max(m, std::abs(h[t]));
+                    out[idx[j]] = m;
+                }
+            }
+        }
+    });
+}
+
+int tlno_gpu_set_density_threshold(tlno_gpu_plan* pl, double density_threshold) {
+    return guarded([&] {
+        if (!pl) throw std::runtime_error("jk_direct: null J/K plan backend.");
+        if (!std::isfinite(density_threshold) || density_threshold < 0.0)
+            throw std::runtime_error("eri_ao_tensor_hgp: density_threshold must be finite and >= 0.");
+        pl->density_thr = density_threshold;
+    });
+}
+
+int tlno_gpu_set_mixed_precision(tlno_gpu_plan* pl, double fp32_threshold) {
+    return guarded([&] {
+        if (!pl) throw std::runtime_error("jk_direct: null J/K plan backend.");
+        if (!std::isfinite(fp32_threshold) || fp32_threshold < 0.0)
+            throw std::runtime_error("tlno_gpu: fp32 threshold must be finite and >= 0");
+        pl->fp32_gate = fp32_threshold;
+    });
+}
+
+int tlno_gpu_last_mixed_stats(const tlno_gpu_plan* pl, long long* fp32_shell_quartets,
+                              long long* fp32_prim_quartets, double* max_abs_density) {
+    return guarded([&] {
+        if (!pl) throw std::runtime_error("jk_direct: null J/K plan backend.");
+        if (fp32_shell_quartets) *fp32_shell_quartets = pl->last_fp32_sq;
+        if (fp32_prim_quartets) *fp32_prim_quartets = pl->last_fp32_pq;
+        if (max_abs_density) *max_abs_density = pl->last_maxD;
+    });
+}
+
 int tlno_gpu_plan_groups(const tlno_gpu_plan* pl, int max_groups, int* L, int* comps) {
     return guarded([&] {
         if (!pl) throw std::runtime_error("jk_direct: null J/K plan backend.");
@@ -1037,8 +1307,8 @@ int tlno_gpu_build_jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, do
 int tlno_gpu_last_build_stats(const tlno_gpu_plan* pl, long long* sq, long long* pq) {
     return guarded([&] {
         if (!pl) throw std::runtime_error("jk_direct: null J/K plan backend.");
-        if (sq) *sq = pl->last_shell_quartets;
-        if (pq) *pq = pl->last_prim_quartets;
+        if (sq) *sq = pl->last_evaluated;
+        if (pq) *pq = pl->last_evaluated_prim;
     });
 }
 

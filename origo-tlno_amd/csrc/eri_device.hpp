@@ -161,28 +161,30 @@ struct PairData {
 };
 
 // ----------------------------------------------------- one shell quartet ---
-template <int LA, int LB, int LC, int LD>
-__device__ __forceinline__ void axis_table(double C00, double D00, double B00, double B10, double B01,
-                                           double g00, double AB, double CD,
-                                           double (&I)[LA + 1][LB + 1][LC + 1][LD + 1]) {
+// R is the arithmetic type of the 2D recurrences and the contraction: double,
+// or float for the mixed-precision route (roots, weights, T and the
+// prefactor are always formed in double and rounded once).
+template <typename R, int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void axis_table(R C00, R D00, R B00, R B10, R B01, R g00, R AB, R CD,
+                                           R (&I)[LA + 1][LB + 1][LC + 1][LD + 1]) {
     constexpr int LAB = LA + LB, LCD = LC + LD;
-    double G[LAB + 1][LCD + 1];
+    R G[LAB + 1][LCD + 1];
     G[0][0] = g00;
     if constexpr (LAB > 0) G[1][0] = C00 * g00;
 #pragma unroll
-    for (int e = 1; e < LAB; ++e) G[e + 1][0] = C00 * G[e][0] + (e * B10) * G[e - 1][0];
+    for (int e = 1; e < LAB; ++e) G[e + 1][0] = C00 * G[e][0] + (R(e) * B10) * G[e - 1][0];
 #pragma unroll
     for (int f = 0; f < LCD; ++f) {
 #pragma unroll
         for (int e = 0; e <= LAB; ++e) {
-            double v = D00 * G[e][f];
-            if (e > 0) v = fma(e * B00, G[e - 1][f], v);
-            if (f > 0) v = fma(f * B01, G[e][f - 1], v);
+            R v = D00 * G[e][f];
+            if (e > 0) v = fma(R(e) * B00, G[e - 1][f], v);
+            if (f > 0) v = fma(R(f) * B01, G[e][f - 1], v);
             G[e][f + 1] = v;
         }
     }
     // ket HRR: H[e][c][d], (c, d+1) = (c+1, d) + CD (c, d)
-    double H[LAB + 1][LCD + 1][LD + 1];
+    R H[LAB + 1][LCD + 1][LD + 1];
 #pragma unroll
     for (int e = 0; e <= LAB; ++e) {
 #pragma unroll
@@ -197,7 +199,7 @@ __device__ __forceinline__ void axis_table(double C00, double D00, double B00, d
     for (int c = 0; c <= LC; ++c)
 #pragma unroll
         for (int d = 0; d <= LD; ++d) {
-            double Jb[LAB + 1][LB + 1];
+            R Jb[LAB + 1][LB + 1];
 #pragma unroll
             for (int e = 0; e <= LAB; ++e) Jb[e][0] = H[e][c][d];
 #pragma unroll
@@ -225,22 +227,22 @@ struct Shape {
 
 constexpr double kTwoPi52 = 34.986836655249725693;  // 2 pi^{5/2}
 
-// Accumulates the lane's (unnormalized-component) block slice into acc.
-template <int LA, int LB, int LC, int LD, int AC, int CC>
+// Accumulates the lane's block slice (unit-diagonal components) into acc.
+template <typename R, int LA, int LB, int LC, int LD, int AC, int CC>
 __device__ __forceinline__ void quartet(const DevTables& tb, const PairData& pd, int P, int Q, int ib0, int ib1,
-                                        double (&acc)[Shape<LA, LB, LC, LD, AC, CC>::NOUT]) {
+                                        R (&acc)[Shape<LA, LB, LC, LD, AC, CC>::NOUT]) {
     using S = Shape<LA, LB, LC, LD, AC, CC>;
     constexpr CartT<LA> ca = make_cart<LA>();
     constexpr CartT<LB> cb = make_cart<LB>();
     constexpr CartT<LC> cc = make_cart<LC>();
     constexpr CartT<LD> cd = make_cart<LD>();
 #pragma unroll
-    for (int o = 0; o < S::NOUT; ++o) acc[o] = 0.0;
+    for (int o = 0; o < S::NOUT; ++o) acc[o] = R(0);
 
     const double Ax = pd.A[3 * P], Ay = pd.A[3 * P + 1], Az = pd.A[3 * P + 2];
-    const double ABx = pd.AB[3 * P], ABy = pd.AB[3 * P + 1], ABz = pd.AB[3 * P + 2];
+    const R ABx = R(pd.AB[3 * P]), ABy = R(pd.AB[3 * P + 1]), ABz = R(pd.AB[3 * P + 2]);
     const double Cx = pd.A[3 * Q], Cy = pd.A[3 * Q + 1], Cz = pd.A[3 * Q + 2];
-    const double CDx = pd.AB[3 * Q], CDy = pd.AB[3 * Q + 1], CDz = pd.AB[3 * Q + 2];
+    const R CDx = R(pd.AB[3 * Q]), CDy = R(pd.AB[3 * Q + 1]), CDz = R(pd.AB[3 * Q + 2]);
     // bra primitive pairs [ib0, ib1) of P (a task may own a slice of them;
     // the block is linear in the primitives, so slices digest independently)
     const int pb0 = pd.poff[P];
@@ -264,24 +266,25 @@ __device__ __forceinline__ void quartet(const DevTables& tb, const PairData& pd,
             const double pref = kTwoPi52 * wab * wcd * sqrt(ipq);
             double rt[S::NR], wt[S::NR];
             rys_roots<S::NR>(tb, T, rt, wt);
-            const double qo = q * ipq, po = p * ipq;
-            const double i2p = 0.5 / p, i2q = 0.5 / q;
-            const double PAx = Pp.x - Ax, PAy = Pp.y - Ay, PAz = Pp.z - Az;
-            const double QCx = Qp.x - Cx, QCy = Qp.y - Cy, QCz = Qp.z - Cz;
+            const R qo = R(q * ipq), po = R(p * ipq);
+            const R hpq = R(0.5 * ipq), i2p = R(0.5 / p), i2q = R(0.5 / q);
+            const R PAx = R(Pp.x - Ax), PAy = R(Pp.y - Ay), PAz = R(Pp.z - Az);
+            const R QCx = R(Qp.x - Cx), QCy = R(Qp.y - Cy), QCz = R(Qp.z - Cz);
+            const R pqx = R(PQx), pqy = R(PQy), pqz = R(PQz);
 #pragma unroll
             for (int r = 0; r < S::NR; ++r) {
-                const double t2 = rt[r];
-                const double B00 = 0.5 * ipq * t2;
-                const double B10 = i2p * (1.0 - qo * t2);
-                const double B01 = i2q * (1.0 - po * t2);
-                const double qt = qo * t2, pt = po * t2;
-                double Ix[LA + 1][LB + 1][LC + 1][LD + 1];
-                double Iy[LA + 1][LB + 1][LC + 1][LD + 1];
-                double Iz[LA + 1][LB + 1][LC + 1][LD + 1];
-                axis_table<LA, LB, LC, LD>(PAx - qt * PQx, QCx + pt * PQx, B00, B10, B01, 1.0, ABx, CDx, Ix);
-                axis_table<LA, LB, LC, LD>(PAy - qt * PQy, QCy + pt * PQy, B00, B10, B01, 1.0, ABy, CDy, Iy);
-                axis_table<LA, LB, LC, LD>(PAz - qt * PQz, QCz + pt * PQz, B00, B10, B01, pref * wt[r], ABz,
-                                           CDz, Iz);
+                const R t2 = R(rt[r]);
+                const R B00 = hpq * t2;
+                const R B10 = i2p * (R(1) - qo * t2);
+                const R B01 = i2q * (R(1) - po * t2);
+                const R qt = qo * t2, pt = po * t2;
+                R Ix[LA + 1][LB + 1][LC + 1][LD + 1];
+                R Iy[LA + 1][LB + 1][LC + 1][LD + 1];
+                R Iz[LA + 1][LB + 1][LC + 1][LD + 1];
+                axis_table<R, LA, LB, LC, LD>(PAx - qt * pqx, QCx + pt * pqx, B00, B10, B01, R(1), ABx, CDx, Ix);
+                axis_table<R, LA, LB, LC, LD>(PAy - qt * pqy, QCy + pt * pqy, B00, B10, B01, R(1), ABy, CDy, Iy);
+                axis_table<R, LA, LB, LC, LD>(PAz - qt * pqz, QCz + pt * pqz, B00, B10, B01, R(pref * wt[r]), ABz,
+                                              CDz, Iz);
                 int o = 0;
 #pragma unroll
                 for (int ai = 0; ai < S::NAL; ++ai)
@@ -291,11 +294,9 @@ __device__ __forceinline__ void quartet(const DevTables& tb, const PairData& pd,
                         for (int ci = 0; ci < S::NCL; ++ci)
 #pragma unroll
                             for (int d = 0; d < S::ND; ++d, ++o) {
-                                constexpr int dummy = 0;
-                                (void)dummy;
                                 const int a = S::acomp(ai), c = S::ccomp(ci);
-                                const double xy = Ix[ca.e[0][a]][cb.e[0][b]][cc.e[0][c]][cd.e[0][d]] *
-                                                  Iy[ca.e[1][a]][cb.e[1][b]][cc.e[1][c]][cd.e[1][d]];
+                                const R xy = Ix[ca.e[0][a]][cb.e[0][b]][cc.e[0][c]][cd.e[0][d]] *
+                                             Iy[ca.e[1][a]][cb.e[1][b]][cc.e[1][c]][cd.e[1][d]];
                                 acc[o] = fma(xy, Iz[ca.e[2][a]][cb.e[2][b]][cc.e[2][c]][cd.e[2][d]], acc[o]);
                             }
             }
@@ -312,7 +313,7 @@ __device__ __forceinline__ void quartet(const DevTables& tb, const PairData& pd,
 #pragma unroll
                 for (int d = 0; d < S::ND; ++d, ++o) {
                     const int a = S::acomp(ai), c = S::ccomp(ci);
-                    acc[o] *= ca.nang[a] * cb.nang[b] * cc.nang[c] * cd.nang[d];
+                    acc[o] *= R(ca.nang[a] * cb.nang[b] * cc.nang[c] * cd.nang[d]);
                 }
 }
 
@@ -320,13 +321,16 @@ __device__ __forceinline__ void quartet(const DevTables& tb, const PairData& pd,
 struct JKArgs {
     PairData pd;
     DevTables tb;
-    const int4* tasks;  // {P, q_begin, q_end, 0}
+    const int4* tasks;  // range mode {P, q_begin, q_end, slice}; list mode {P, offset, count, slice}
     int ntasks;
     const double* D;    // internal Cartesian density, M x M
     double* J;          // Jacc
     double* K;          // Kacc
     int M;
     int do_j, do_k;
+    const int* cut;     // density screen: ket range of P ends at cut[6 P] (pointer offset by ket class; nullptr: off)
+    const int* qlist;   // list mode: kets of a task are qlist[offset .. offset + count)
+    unsigned long long* counter;  // shell quartets evaluated (nullptr: not counted)
 };
 
 struct StoreArgs {
@@ -345,7 +349,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // Task = {P, q_begin, q_end, bra primitive slice ib0 | ib1 << 16}.
-template <int LA, int LB, int LC, int LD, int AC, int CC>
+template <typename R, int LA, int LB, int LC, int LD, int AC, int CC>
 __device__ __forceinline__ void eri_jk_body(const JKArgs& a) {
     using S = Shape<LA, LB, LC, LD, AC, CC>;
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -355,15 +359,34 @@ __device__ __forceinline__ void eri_jk_body(const JKArgs& a) {
     // the task is wave-uniform: keep it in scalar registers
     const int P = __builtin_amdgcn_readfirstlane(t.x);
     const int qb = __builtin_amdgcn_readfirstlane(t.y);
-    const int qe = __builtin_amdgcn_readfirstlane(t.z);
+    int qe = __builtin_amdgcn_readfirstlane(t.z);
     const int tw = __builtin_amdgcn_readfirstlane(t.w);
-    const int qraw = qb + lane;
-    const bool active = qraw < qe;
-    const int Q = active ? qraw : qe - 1;
+    bool active;
+    int Q;
+    if (a.qlist) {  // list mode: qb = offset, qe = count
+        if (qe <= 0) return;
+        active = lane < qe;
+        Q = a.qlist[qb + (active ? lane : qe - 1)];
+    } else {
+        if (a.cut) qe = min(qe, a.cut[6 * P]);  // density screen (per bra pair, this ket class)
+        if (qe <= qb) return;                          // whole wave leaves together
+        const int qraw = qb + lane;
+        active = qraw < qe;
+        Q = active ? qraw : qe - 1;
+    }
     const int ib0 = tw & 0xffff, ib1 = (tw >> 16) & 0xffff;
+    if (a.counter && blockIdx.y == 0) {  // [0] shell quartets, [1] primitive quartets evaluated (chunk 0 counts)
+        int npq = active ? a.pd.pn[Q] * (ib1 - ib0) : 0;
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) npq += __shfl_xor(npq, m, 64);
+        if (lane == 0) {
+            if (ib0 == 0) atomicAdd(a.counter, static_cast<unsigned long long>(a.qlist ? qe : min(qe - qb, 64)));
+            atomicAdd(a.counter + 1, static_cast<unsigned long long>(npq));
+        }
+    }
 
-    double acc[S::NOUT];
-    quartet<LA, LB, LC, LD, AC, CC>(a.tb, a.pd, P, Q, ib0, ib1, acc);
+    R acc[S::NOUT];
+    quartet<R, LA, LB, LC, LD, AC, CC>(a.tb, a.pd, P, Q, ib0, ib1, acc);
 
     // degeneracy scale: 1/2 per diagonal pair, 1/2 if P == Q
     double s = 1.0;
@@ -401,7 +424,7 @@ __device__ __forceinline__ void eri_jk_body(const JKArgs& a) {
                 for (int ai = 0; ai < S::NAL; ++ai)
 #pragma unroll
                     for (int b = 0; b < S::NB; ++b) {
-                        const double v = acc[((ai * S::NB + b) * S::NCL + ci) * S::ND + d];
+                        const double v = double(acc[((ai * S::NB + b) * S::NCL + ci) * S::ND + d]);
                         jcd = fma(v, dab[ai][b], jcd);
                         jab[ai][b] = fma(v, dcd, jab[ai][b]);
                     }
@@ -429,7 +452,7 @@ __device__ __forceinline__ void eri_jk_body(const JKArgs& a) {
                 for (int b = 0; b < S::NB; ++b)
 #pragma unroll
                     for (int d = 0; d < S::ND; ++d)
-                        kac = fma(acc[((ai * S::NB + b) * S::NCL + ci) * S::ND + d], Dm[(B0 + b) * M + D0 + d], kac);
+                        kac = fma(double(acc[((ai * S::NB + b) * S::NCL + ci) * S::ND + d]), Dm[(B0 + b) * M + D0 + d], kac);
                 unsafeAtomicAdd(a.K + ga * M + gc, s * kac);
             }
 #pragma unroll
@@ -440,7 +463,7 @@ __device__ __forceinline__ void eri_jk_body(const JKArgs& a) {
                 for (int b = 0; b < S::NB; ++b)
 #pragma unroll
                     for (int ci = 0; ci < S::NCL; ++ci)
-                        kad = fma(acc[((ai * S::NB + b) * S::NCL + ci) * S::ND + d],
+                        kad = fma(double(acc[((ai * S::NB + b) * S::NCL + ci) * S::ND + d]),
                                   Dm[(B0 + b) * M + C0 + S::ccomp(ci)], kad);
                 unsafeAtomicAdd(a.K + ga * M + gd, s * kad);
             }
@@ -456,7 +479,7 @@ __device__ __forceinline__ void eri_jk_body(const JKArgs& a) {
                 for (int ai = 0; ai < S::NAL; ++ai)
 #pragma unroll
                     for (int d = 0; d < S::ND; ++d)
-                        kbc = fma(acc[((ai * S::NB + b) * S::NCL + ci) * S::ND + d],
+                        kbc = fma(double(acc[((ai * S::NB + b) * S::NCL + ci) * S::ND + d]),
                                   Dm[(A0 + S::acomp(ai)) * M + D0 + d], kbc);
                 unsafeAtomicAdd(a.K + gb * M + gc, s * kbc);
             }
@@ -468,7 +491,7 @@ __device__ __forceinline__ void eri_jk_body(const JKArgs& a) {
                 for (int ai = 0; ai < S::NAL; ++ai)
 #pragma unroll
                     for (int ci = 0; ci < S::NCL; ++ci)
-                        kbd = fma(acc[((ai * S::NB + b) * S::NCL + ci) * S::ND + d],
+                        kbd = fma(double(acc[((ai * S::NB + b) * S::NCL + ci) * S::ND + d]),
                                   Dm[(A0 + S::acomp(ai)) * M + C0 + S::ccomp(ci)], kbd);
                 unsafeAtomicAdd(a.K + gb * M + gd, s * kbd);
             }
@@ -485,7 +508,7 @@ __device__ __forceinline__ void eri_store_body(const StoreArgs& a) {
     if (i >= a.n) return;
     const int2 pq = a.quartets[i];
     double acc[S::NOUT];
-    quartet<LA, LB, LC, LD, AC, CC>(a.tb, a.pd, pq.x, pq.y, 0, a.pd.pn[pq.x], acc);
+    quartet<double, LA, LB, LC, LD, AC, CC>(a.tb, a.pd, pq.x, pq.y, 0, a.pd.pn[pq.x], acc);
     double* out = a.out + a.off[i];
     int o = 0;
 #pragma unroll
@@ -506,7 +529,8 @@ using StoreKernel = void (*)(StoreArgs);
 // One kernel per class pair; the output chunk is blockIdx.y.
 struct ClassKernels {
     int nchunk = 0;
-    JKKernel jk = nullptr;
+    JKKernel jk = nullptr;    // FP64 recurrences
+    JKKernel jk32 = nullptr;  // FP32 recurrences (mixed-precision route)
     StoreKernel store = nullptr;
 };
 

@@ -24,11 +24,11 @@ struct ChunkPolicy {
     static constexpr int cc(int i) { return mode == 2 ? i % NC : -1; }
 };
 
-template <int LA, int LB, int LC, int LD, int... I>
+template <typename R, int LA, int LB, int LC, int LD, int... I>
 __global__ __launch_bounds__(256) void k_eri_jk(JKArgs a) {
     using CP = ChunkPolicy<LA, LB, LC, LD>;
     const int ch = blockIdx.y;
-    ((ch == I ? (eri_jk_body<LA, LB, LC, LD, CP::ac(I), CP::cc(I)>(a), 0) : 0), ...);
+    ((ch == I ? (eri_jk_body<R, LA, LB, LC, LD, CP::ac(I), CP::cc(I)>(a), 0) : 0), ...);
 }
 
 template <int LA, int LB, int LC, int LD, int... I>
@@ -40,7 +40,8 @@ __global__ __launch_bounds__(256) void k_eri_store(StoreArgs a) {
 
 template <int LA, int LB, int LC, int LD, int... I>
 void reg_kernels(ClassKernels& k, std::integer_sequence<int, I...>) {
-    k.jk = k_eri_jk<LA, LB, LC, LD, I...>;
+    k.jk = k_eri_jk<double, LA, LB, LC, LD, I...>;
+    k.jk32 = k_eri_jk<float, LA, LB, LC, LD, I...>;
     k.store = k_eri_store<LA, LB, LC, LD, I...>;
 }
 

@@ -9,7 +9,9 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libtlno_amd.so")
+# TLNO_AMD_LIB selects another build of the same library (e.g. a variant
+# under test); by default the in-tree build is loaded.
+LIB_PATH = os.environ.get("TLNO_AMD_LIB") or os.path.join(_HERE, "libtlno_amd.so")
 
 D = C.POINTER(C.c_double)
 I = C.POINTER(C.c_int)
@@ -23,10 +25,15 @@ SIGNATURES = {
     "tlno_gpu_plan_destroy": (None, [C.c_void_p]),
     "tlno_gpu_plan_info": (C.c_int, [C.c_void_p, I, I, I, LL]),
     "tlno_gpu_plan_groups": (C.c_int, [C.c_void_p, C.c_int, I, I]),
+    "tlno_gpu_plan_pairs": (C.c_int, [C.c_void_p, I, I, D, I]),
+    "tlno_gpu_quartet_maxabs": (C.c_int, [C.c_void_p, C.c_int, I, I, D]),
     "tlno_gpu_build_jk": (C.c_int, [C.c_void_p, D, D, D, C.c_double]),
     "tlno_gpu_build_jk_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double]),
     "tlno_gpu_last_build_stats": (C.c_int, [C.c_void_p, LL, LL]),
     "tlno_gpu_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "tlno_gpu_set_mixed_precision": (C.c_int, [C.c_void_p, C.c_double]),
+    "tlno_gpu_set_density_threshold": (C.c_int, [C.c_void_p, C.c_double]),
+    "tlno_gpu_last_mixed_stats": (C.c_int, [C.c_void_p, LL, LL, D]),
     "tlno_gpu_class_stats": (C.c_int, [C.c_void_p, LL, LL, D, LL]),
     "tlno_gpu_eri_cart_blocks": (C.c_int, [C.c_void_p, C.c_int, I, LL, D]),
     "tlno_gpu_boys": (C.c_int, [C.c_int, D, C.c_int, D]),

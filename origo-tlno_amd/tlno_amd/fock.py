@@ -12,9 +12,10 @@ Same names, argument meaning and error behaviour:
     ``RuntimeError("eri_ao_tensor_hgp: density ...")``);
   * thresholds must be finite and >= 0.
 
-Options the reference exposes whose screens are not built on the GPU yet
-(primitive FASTSCHWARZ, decay, density/CVHF) raise NotImplementedError when
-nonzero instead of being silently ignored.
+``density_threshold`` (the reference's density-coupled screen) is supported.
+The primitive FASTSCHWARZ and decay screens the reference exposes are not
+built on the GPU; they raise NotImplementedError when nonzero instead of being
+silently ignored.
 """
 from __future__ import annotations
 
@@ -66,7 +67,10 @@ def _validate_options(o: ScreeningOptions):
     _check_threshold("primitive_fastschwarz_threshold", o.primitive_fastschwarz_threshold)
     _check_threshold("decay_threshold", o.decay_threshold)
     _check_threshold("density_threshold", o.density_threshold)
-    for name in ("primitive_fastschwarz_threshold", "decay_threshold", "density_threshold"):
+    if o.density_threshold > 0.0 and o.decay_threshold > 0.0:
+        raise RuntimeError("eri_ao_tensor_hgp: density screening requires Schwarz bounds and cannot be combined "
+                           "with the heuristic decay metric.")
+    for name in ("primitive_fastschwarz_threshold", "decay_threshold"):
         if getattr(o, name) != 0.0:
             raise NotImplementedError(f"{name} screening is not available in the GPU engine")
 
@@ -128,10 +132,13 @@ class DirectPlan:
         except Exception:
             pass
 
-    def jk(self, density, want_j=True, want_k=True, schwarz=None):
+    def jk(self, density, want_j=True, want_k=True, schwarz=None, density_threshold=None):
         """Raw J and K (either may be skipped) on host arrays."""
         thr = self.options.integral_threshold if schwarz is None else schwarz
         _check_threshold("schwarz_threshold", thr)
+        dthr = self.options.density_threshold if density_threshold is None else density_threshold
+        _check_threshold("density_threshold", dthr)
+        _lib.check(_lib.load().tlno_gpu_set_density_threshold(self._h, float(dthr)))
         M = self.M
         D = np.ascontiguousarray(density, dtype=np.float64)
         if D.shape != (M, M):
@@ -142,6 +149,18 @@ class DirectPlan:
                                                  J.ctypes.data_as(_lib.D) if want_j else None,
                                                  K.ctypes.data_as(_lib.D) if want_k else None, float(thr)))
         return J, K
+
+    def set_mixed_precision(self, fp32_threshold):
+        """Route shell quartets with Q_ab Q_cd max|D| < fp32_threshold to the
+        FP32 recurrences (0 disables); J/K accumulation stays FP64."""
+        _lib.check(_lib.load().tlno_gpu_set_mixed_precision(self._h, float(fp32_threshold)))
+
+    def last_mixed_stats(self):
+        sq = C.c_longlong()
+        pq = C.c_longlong()
+        md = C.c_double()
+        _lib.check(_lib.load().tlno_gpu_last_mixed_stats(self._h, C.byref(sq), C.byref(pq), C.byref(md)))
+        return {"fp32_shell_quartets": sq.value, "fp32_prim_quartets": pq.value, "max_abs_density": md.value}
 
     def last_build_stats(self):
         sq = C.c_longlong()
@@ -165,6 +184,26 @@ class DirectPlan:
         for q, o, s in zip(quads, offs, sizes):
             res.append(out[o:o + s].reshape([ncart(Ls[g]) for g in q]))
         return res
+
+    def pairs(self):
+        """(group_a, group_b, Schwarz bound, class_begin[7]) in plan pair order."""
+        n = self.profile.pair_groups
+        ga = np.zeros(n, np.int32)
+        gb = np.zeros(n, np.int32)
+        Q = np.zeros(n)
+        cb = np.zeros(7, np.int32)
+        _lib.check(_lib.load().tlno_gpu_plan_pairs(self._h, ga.ctypes.data_as(_lib.I), gb.ctypes.data_as(_lib.I),
+                                                   Q.ctypes.data_as(_lib.D), cb.ctypes.data_as(_lib.I)))
+        return ga, gb, Q, cb
+
+    def quartet_maxabs(self, P, Q):
+        """max |(ab|cd)| of the unit-diagonal block of each (P[i] | Q[i]) (plan pair indices)."""
+        P = np.ascontiguousarray(P, dtype=np.int32)
+        Q = np.ascontiguousarray(Q, dtype=np.int32)
+        out = np.zeros(len(P))
+        _lib.check(_lib.load().tlno_gpu_quartet_maxabs(self._h, len(P), P.ctypes.data_as(_lib.I),
+                                                       Q.ctypes.data_as(_lib.I), out.ctypes.data_as(_lib.D)))
+        return out
 
     def group_L(self):
         """L of each shell group of the plan, in first-appearance order."""
@@ -218,21 +257,21 @@ def _opts(plan, options):
 
 def build_coulomb_j(plan: DirectPlan, density, options: ScreeningOptions | None = None) -> CoulombJResult:
     o = _opts(plan, options)
-    J, _ = plan.jk(density, True, False, o.integral_threshold)
+    J, _ = plan.jk(density, True, False, o.integral_threshold, o.density_threshold)
     return CoulombJResult(J, plan.M)
 
 
 def build_exchange_k(plan: DirectPlan, density, options: ScreeningOptions | None = None) -> ExchangeKResult:
     """K[a,c] = sum_(b,d) P[b,d] (ab|cd), for a real symmetric density."""
     o = _opts(plan, options)
-    _, K = plan.jk(density, False, True, o.integral_threshold)
+    _, K = plan.jk(density, False, True, o.integral_threshold, o.density_threshold)
     return ExchangeKResult(K, plan.M)
 
 
 def build_rhf_g(plan: DirectPlan, density, options: ScreeningOptions | None = None) -> DirectResult:
     """G = J - K/2 from one traversal."""
     o = _opts(plan, options)
-    J, K = plan.jk(density, True, True, o.integral_threshold)
+    J, K = plan.jk(density, True, True, o.integral_threshold, o.density_threshold)
     return DirectResult(J - 0.5 * K, plan.M)
 
 

@@ -1,0 +1,51 @@
+"""GPU: mixed-precision J/K (FP32 recurrences for quartets with
+Q_ab Q_cd max|D| < 1e-6, FP64 roots and FP64 accumulation) against the FP64
+GPU build, which is itself checked against the oracle and the reference.
+
+Tolerance: max-abs J/K error <= 1e-7 (the target for the mixed route); the
+RMS error is reported.
+"""
+import numpy as np
+import pytest
+
+from tlno_amd import configs
+from tlno_amd.fock import DirectPlan, ScreeningOptions
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("nwat", [10, 40])
+def test_mixed_precision_error_bound(nwat):
+    b, D = configs.water_cluster(nwat)
+    plan = DirectPlan(b, ScreeningOptions(integral_threshold=1e-12), cartesian=True)
+    J0, K0 = plan.jk(D)
+    plan.set_mixed_precision(1e-6)
+    J1, K1 = plan.jk(D)
+    st = plan.last_mixed_stats()
+    sq, pq = plan.last_build_stats()
+    plan.set_mixed_precision(0.0)
+    ej, ek = np.abs(J1 - J0).max(), np.abs(K1 - K0).max()
+    rj = np.sqrt(np.mean((J1 - J0) ** 2))
+    rk = np.sqrt(np.mean((K1 - K0) ** 2))
+    print(f"water{nwat}: FP32 share {st['fp32_shell_quartets'] / sq:.3f} of shell quartets, "
+          f"{st['fp32_prim_quartets'] / pq:.3f} of primitive quartets; max|dJ| {ej:.2e} max|dK| {ek:.2e} "
+          f"rms {rj:.2e} {rk:.2e}")
+    assert st["fp32_shell_quartets"] > 0
+    assert ej <= 1e-7 and ek <= 1e-7
+    # the gate is off again: bit-identical to the first FP64 build
+    J2, K2 = plan.jk(D)
+    assert np.array_equal(J2, J0) or np.abs(J2 - J0).max() < 1e-13
+
+
+def test_mixed_gate_zero_and_huge():
+    b, D = configs.water_cluster(4)
+    plan = DirectPlan(b, ScreeningOptions(integral_threshold=1e-12), cartesian=True)
+    J0, K0 = plan.jk(D)
+    plan.set_mixed_precision(1e30)  # everything FP32
+    J1, K1 = plan.jk(D)
+    sq, _ = plan.last_build_stats()
+    assert plan.last_mixed_stats()["fp32_shell_quartets"] == sq
+    # all-FP32 is still close in relative terms
+    assert np.abs(J1 - J0).max() < 1e-4 * np.abs(J0).max()
+    with pytest.raises(Exception):
+        plan.set_mixed_precision(-1.0)
