@@ -88,9 +88,19 @@ def gpu_run(b, D, steps, warmup, thr):
     return plan, wall, classes, J, K
 
 
-def mixed_run(steps=3, gate=1e-6):
-    """water40 (1000 functions): FP64 GPU build vs the mixed-precision build
-    (FP32 recurrences for Q_ab Q_cd max|D| < gate), both from HBM-resident D."""
+SCREEN_TAU = 2e-9  # density threshold of the screened water40 builds (J/K error budget 1e-7)
+
+
+def mixed_run(steps=3, gate=1e-6, tau=SCREEN_TAU):
+    """water40 (1000 functions), densities resident in HBM, Schwarz 1e-12:
+      fp64           the plain FP64 GPU build;
+      mixed          FP32 recurrences for Q_ab Q_cd max|D| < gate;
+      density+mixed  plus the reference's density-coupled screen at tau (the
+                     method without the network);
+      learned+mixed  plus the learned screen (shipped weights): the network's
+                     estimate replaces Q_ab Q_cd in the screen and in the gate.
+    Errors are max-abs / RMS against the FP64 build."""
+    from tlno_amd import learned
     lib = _lib.load()
     b, D = configs.water40()
     thr = 1e-12
@@ -99,27 +109,43 @@ def mixed_run(steps=3, gate=1e-6):
     dD = hipmem.DeviceArray.from_host(D)
     dJ = hipmem.DeviceArray((M, M))
     dK = hipmem.DeviceArray((M, M))
-    out = {}
-    for name, g in (("fp64", 0.0), ("mixed", gate)):
+    params, margin = learned.load_weights()
+    runs = {}
+    for name, g, t, use_net in (("fp64", 0.0, 0.0, False), ("mixed", gate, 0.0, False),
+                                ("density+mixed", gate, tau, False), ("learned+mixed", gate, tau, True)):
         _lib.check(lib.tlno_gpu_set_mixed_precision(plan.handle, g))
+        _lib.check(lib.tlno_gpu_set_density_threshold(plan.handle, t))
+        plan.set_learned_screen(params if use_net else None, margin)
         _lib.check(lib.tlno_gpu_build_jk_device(plan.handle, dD.ptr, dJ.ptr, dK.ptr, thr))
         hipmem.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
             _lib.check(lib.tlno_gpu_build_jk_device(plan.handle, dD.ptr, dJ.ptr, dK.ptr, thr))
         hipmem.synchronize()
-        out[name] = ((time.perf_counter() - t0) / steps, dJ.to_host(), dK.to_host())
-    st = plan.last_mixed_stats()
-    sq, pq = plan.last_build_stats()
+        sec = (time.perf_counter() - t0) / steps
+        sq, pq = plan.last_build_stats()
+        runs[name] = {"ms": sec * 1e3, "J": dJ.to_host(), "K": dK.to_host(), "shell_quartets": sq,
+                      "prim_quartets": pq, "fp32": plan.last_mixed_stats(), "route": plan.last_route_stats()}
+    plan.set_learned_screen(None)
+    _lib.check(lib.tlno_gpu_set_density_threshold(plan.handle, 0.0))
     _lib.check(lib.tlno_gpu_set_mixed_precision(plan.handle, 0.0))
-    (t64, J0, K0), (tm, J1, K1) = out["fp64"], out["mixed"]
-    return {"workload": "water40 (40 waters, 1000 Cartesian functions, Schwarz 1e-12)",
-            "fp32_gate": gate, "fp64_ms": t64 * 1e3, "mixed_ms": tm * 1e3, "speedup": t64 / tm,
-            "fp32_share_shell_quartets": st["fp32_shell_quartets"] / sq,
-            "fp32_share_prim_quartets": st["fp32_prim_quartets"] / pq,
-            "max_abs_err_J": float(np.abs(J1 - J0).max()), "max_abs_err_K": float(np.abs(K1 - K0).max()),
-            "rms_err_J": float(np.sqrt(np.mean((J1 - J0) ** 2))),
-            "rms_err_K": float(np.sqrt(np.mean((K1 - K0) ** 2)))}
+    base = runs["fp64"]
+    out = {"workload": "water40 (40 waters, 1000 Cartesian functions, Schwarz 1e-12)", "fp32_gate": gate,
+           "density_threshold": tau, "learned_margin_decades": margin, "fp64_ms": base["ms"],
+           "fp64_shell_quartets": base["shell_quartets"]}
+    for name in ("mixed", "density+mixed", "learned+mixed"):
+        r = runs[name]
+        out[name] = {"ms": r["ms"], "speedup_vs_fp64": base["ms"] / r["ms"],
+                     "shell_quartets": r["shell_quartets"],
+                     "fp32_share_shell_quartets": r["fp32"]["fp32_shell_quartets"] / max(1, r["shell_quartets"]),
+                     "max_abs_err_J": float(np.abs(r["J"] - base["J"]).max()),
+                     "max_abs_err_K": float(np.abs(r["K"] - base["K"]).max()),
+                     "rms_err_J": float(np.sqrt(np.mean((r["J"] - base["J"]) ** 2))),
+                     "rms_err_K": float(np.sqrt(np.mean((r["K"] - base["K"]) ** 2)))}
+        if r["route"]["candidates"]:
+            out[name]["route"] = r["route"]
+    out["learned_gain_over_density_only"] = runs["density+mixed"]["ms"] / runs["learned+mixed"]["ms"]
+    return out
 
 
 def cpu_baseline(b, seconds_cap=60.0):

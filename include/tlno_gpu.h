@@ -101,6 +101,26 @@ int tlno_gpu_build_jk_device(tlno_gpu_plan* plan, const double* d_density,
  * so no quartet the reference keeps is dropped.  0 turns it off (default). */
 int tlno_gpu_set_density_threshold(tlno_gpu_plan* plan, double density_threshold);
 
+/* Learned screen (no reference counterpart).  params: TLNO_GPU_MLP_NALL
+ * floats of a network trained with tlno_gpu_mlp_* (NULL turns the screen
+ * off).  Following builds evaluate the network on the matrix cores for every
+ * candidate shell quartet and replace Q_ab Q_cd by the estimate
+ * est = Q_ab Q_cd min(1, 10^(y + margin)) in the density test, in the Schwarz
+ * test and in the FP32 gate (y: predicted log10(max|(ab|cd)| / (Q_ab Q_cd)),
+ * margin: safety offset in decades). */
+int tlno_gpu_set_learned_screen(tlno_gpu_plan* plan, const float* params, double margin);
+
+/* The 32 learned-screen features of n quartets (pair_p[i] | pair_q[i]) (plan
+ * pair indices), out[32 i + k]. */
+int tlno_gpu_quartet_features(tlno_gpu_plan* plan, int n, const int* pair_p, const int* pair_q,
+                              float* out);
+
+/* Per-quartet routing statistics of the last build (zeros if it did not use
+ * the density test or the learned screen): stats[0] candidate quartets,
+ * [1] kept by the density test, [2] kept after the learned screen,
+ * [3] of those sent to FP32. */
+int tlno_gpu_last_route_stats(const tlno_gpu_plan* plan, long long* stats);
+
 /* Mixed precision (no reference counterpart): with fp32_threshold > 0,
  * following builds of this plan evaluate the 2D recurrences and the Cartesian
  * contraction of every shell quartet with Q_ab Q_cd max|D| < fp32_threshold in
@@ -152,6 +172,37 @@ int tlno_gpu_rys(int N, int n, const double* T, double* roots, double* weights);
  * nuclear charges Z (n_nuc) and centres R (3*n_nuc, bohr). */
 int tlno_gpu_onee(tlno_gpu_plan* plan, int n_nuc, const double* Z, const double* R,
                   double* S, double* T, double* V);
+
+/* ---- learned-screen network: 32 -> 64 -> 64 -> 1, ReLU, bf16 weights,
+ * FP32 sums, trained on the GPU (no reference counterpart).  Parameters are
+ * TLNO_GPU_MLP_NALL floats: W1[64][32], b1[64], W2[64][64], b2[64], w3[64],
+ * b3, feature mean[32], feature 1/std[32]. */
+#define TLNO_GPU_MLP_NFEAT 32
+#define TLNO_GPU_MLP_NPARAM 6337
+#define TLNO_GPU_MLP_NALL 6401
+typedef struct tlno_gpu_mlp tlno_gpu_mlp;
+/* He-uniform initialisation from seed, identity feature normalization. */
+int tlno_gpu_mlp_create(int device, unsigned seed, tlno_gpu_mlp** out);
+void tlno_gpu_mlp_destroy(tlno_gpu_mlp* mlp);
+int tlno_gpu_mlp_get_params(tlno_gpu_mlp* mlp, float* params);
+/* Replaces all parameters and resets the optimizer state. */
+int tlno_gpu_mlp_set_params(tlno_gpu_mlp* mlp, const float* params);
+/* Sets the feature normalization to the mean and 1/std of n samples X[32 i + k]. */
+int tlno_gpu_mlp_set_normalization(tlno_gpu_mlp* mlp, long long n, const float* X);
+/* Loss mean_i c(r_i) r_i^2 (r = t - y, c = w_under if r > 0 else 1) over n
+ * samples and its gradient with respect to the TLNO_GPU_MLP_NPARAM parameters. */
+int tlno_gpu_mlp_loss_grad(tlno_gpu_mlp* mlp, long long n, const float* X, const float* t,
+                           float w_under, float* grad, double* loss);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on the full batch. */
+int tlno_gpu_mlp_adam_step(tlno_gpu_mlp* mlp, long long n, const float* X, const float* t,
+                           float lr, float w_under);
+/* Mini-batch Adam over shuffled epochs, cosine decay from lr to lr/20;
+ * epoch_loss[e] = mean batch loss of epoch e (may be NULL). */
+int tlno_gpu_mlp_train(tlno_gpu_mlp* mlp, long long n, const float* X, const float* t, int epochs,
+                       int batch, float lr, float w_under, unsigned seed, float* epoch_loss);
+/* Network output for n samples; matrix_cores = 1 uses the bf16 MFMA evaluator
+ * of the screening kernel, 0 the per-lane FP32 evaluator. */
+int tlno_gpu_mlp_predict(tlno_gpu_mlp* mlp, long long n, const float* X, float* y, int matrix_cores);
 
 #ifdef __cplusplus
 }

@@ -26,24 +26,15 @@
 #include <string>
 #include <vector>
 
+#include "common.hpp"
 #include "eri_device.hpp"
+#include "mlp.hpp"
 #include "onee.hpp"
 #include "tlno_gpu.h"
 
 namespace tlno_amd {
 
 thread_local std::string g_err;
-
-struct HipError : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-
-#define HIP_OK(x)                                                                          \
-    do {                                                                                   \
-        hipError_t e_ = (x);                                                               \
-        if (e_ != hipSuccess)                                                              \
-            throw HipError(std::string("hip: ") + hipGetErrorString(e_) + " at " #x);      \
-    } while (0)
 
 // ------------------------------------------------------------- device ctx --
 struct DeviceCtx {
@@ -230,6 +221,15 @@ struct TaskSet {
     // running shell/primitive quartet counts (for the precision split)
     std::vector<double> bmax[6][6];
     std::vector<long long> cum_sq[6][6], cum_pq[6][6];
+    // per-quartet routing (density test, learned screen): rows of at most
+    // kRouteSeg candidate kets {P, k_begin, k_end, 6 X + Y}, the qlist region
+    // of each row, and the task capacity of each class pair; built on first use
+    int4* d_rows = nullptr;
+    long long* d_rbase = nullptr;
+    long long nrows = 0;
+    long long qtotal = 0;
+    long long toff[36] = {}, tcap[36] = {};
+    long long tcap_total = 0;
 };
 
 }  // namespace tlno_amd
@@ -281,6 +281,24 @@ struct tlno_gpu_plan {
     long long last_evaluated = 0, last_evaluated_prim = 0;
     double last_maxD = 0.0;
     long long last_fp32_sq = 0, last_fp32_pq = 0;
+    // per-quartet routing: pair features, shell rows, shell-block max |D|,
+    // learned-screen operands, and the work lists the routing kernel fills
+    float4 *d_pf0 = nullptr, *d_pf1 = nullptr, *d_pcen = nullptr;
+    int* d_grows = nullptr;                  // [ngroups][6] rows of each shell in the caller's basis (-1 pad)
+    double* d_blk = nullptr;                 // [ngroups^2] max |D| over each shell block
+    double* d_bsum = nullptr;                // [ngroups^2] sum |D| over each shell block
+    int learned = 0;
+    double learned_margin = 0.0;
+    uint4* d_frag = nullptr;
+    float *d_mvec = nullptr, *d_mnorm = nullptr;
+    float mb3 = 0.0f;
+    int* d_qlist = nullptr;
+    long long qlist_cap = 0;
+    int4 *d_t64 = nullptr, *d_t32 = nullptr;
+    long long tlist_cap = 0;
+    unsigned* d_tcount = nullptr;            // [2][36] tasks emitted per route and class pair
+    unsigned long long* d_rstats = nullptr;  // [4]
+    long long last_route[4] = {0, 0, 0, 0};  // candidates, density-kept, kept, FP32 (last routed build)
     // optional per-class kernel timing with HIP events on the plan stream
     int timing = 0;
     hipEvent_t ev[6][6][2] = {};
@@ -479,7 +497,6 @@ __global__ void k_rys(DevTables tb, const double* T, double* r, double* w, int n
     }
 }
 
-inline unsigned grid_for(long n, int block) { return static_cast<unsigned>((n + block - 1) / block); }
 
 // grid: x covers the tasks (one wave each, 4 waves per block), y the chunks
 void launch_jk(JKKernel kern, int nchunk, JKArgs& args, long long ntask, hipStream_t st) {
@@ -522,7 +539,9 @@ __global__ void k_density_cut(const double* Q, const int* pcls, const int* cls_b
     const int X = pcls[P];
     for (int Y = 0; Y <= X; ++Y) {
         const double f = 4.0 * fmax(pairfac[P], __longlong_as_double(clsmax[Y]));
-        const double qp = Q[P] * f;
+        // (1 + 1e-12): rounding never moves the prefix below a quartet the
+        // exact per-quartet test keeps
+        const double qp = Q[P] * f * (1.0 + 1e-12);
         int lo = cls_begin[Y], hi = cls_begin[Y + 1];
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
@@ -542,6 +561,188 @@ __global__ void k_absmax(const double* __restrict__ x, long n, unsigned long lon
 #pragma unroll
     for (int s = 32; s >= 1; s >>= 1) m = fmax(m, __shfl_xor(m, s, 64));
     if ((threadIdx.x & 63) == 0) atomicMax(out, static_cast<unsigned long long>(__double_as_longlong(m)));
+}
+
+// blk[gi][gj] = max |D| over the rows of shell gi and columns of shell gj and
+// the transposed block (the reference's group_pair_density_max,
+// integrals/two_electron/fock/detail/direct_build.ipp:126-178);
+// bsum[gi][gj] = the sum over the block of the larger of |D_ab|, |D_ba|
+// (bounds what one shell quartet adds to a J or K element per unit of its
+// largest integral; used by the learned screen)
+__global__ void k_block_max(const double* __restrict__ D, const int* __restrict__ grows, int ng, int M,
+                            double* __restrict__ blk, double* __restrict__ bsum) {
+    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= static_cast<long long>(ng) * ng) return;
+    const int gi = static_cast<int>(t / ng), gj = static_cast<int>(t - static_cast<long long>(gi) * ng);
+    double m = 0.0, sum = 0.0;
+    for (int a = 0; a < 6; ++a) {
+        const int ra = grows[6 * gi + a];
+        if (ra < 0) break;
+        for (int b = 0; b < 6; ++b) {
+            const int rb = grows[6 * gj + b];
+            if (rb < 0) break;
+            const double v = fmax(fabs(D[static_cast<long long>(ra) * M + rb]), fabs(D[static_cast<long long>(rb) * M + ra]));
+            m = fmax(m, v);
+            sum += v;
+        }
+    }
+    blk[t] = m;
+    bsum[t] = sum;
+}
+
+// ---- per-quartet routing ----
+// One wave per row {P, k_begin, k_end, 6 X + Y} of candidate kets (Schwarz
+// survivors, cut further by the density prefix bound when that is on).  Each
+// candidate is dropped, sent to the FP32 route or to the FP64 route:
+//   * density test (the reference's rys_density_shell_quartet_live,
+//     integrals/two_electron/rys/rys_engine.ipp:1124-1167): keep iff
+//     est * max(4 max(rho_ab, rho_cd), max(D_ca, D_da, D_cb, D_db)) >= tau
+//     (J only: the Coulomb term, K only: the exchange term);
+//   * learned screen: est = Q_ab Q_cd min(1, 10^(y + margin)) with y the
+//     network's prediction of log10(max|(ab|cd)| / (Q_ab Q_cd)); est must
+//     reach the Schwarz threshold and, with the density test on, pass it
+//     again with block sums of |D| in place of block maxima (est estimates
+//     one integral, a block of them adds up); without it est = Q_ab Q_cd;
+//   * precision: FP32 iff est * max|D| < gate.
+// Survivors are compacted into the row's qlist region (FP64 from the front,
+// FP32 from the back) and emitted as list-mode tasks of up to 64 kets.
+constexpr int kRouteSeg = 512;
+struct RouteArgs {
+    const int4* rows;
+    const long long* rbase;
+    long long nrows;
+    const int* cut;     // [6 npair] density prefix bound (nullptr: off)
+    const double* Q;
+    const int* pgA;
+    const int* pgB;
+    int ng;
+    const double* blk;  // shell-block max |D| (nullptr: no density test)
+    const double* bsum; // shell-block sum |D|
+    int mode;           // 0 J and K, 1 J only, 2 K only
+    double tau;
+    double thr;
+    MlpPacked mlp;
+    FeatArgs fa;
+    double margin;
+    double gate;
+    const double* maxD;
+    int* qlist;
+    int4* t64;
+    int4* t32;
+    long long toff[36];
+    unsigned* tcount;             // [2][36]
+    unsigned long long* stats;    // candidates, density-kept, kept, FP32
+};
+
+__device__ inline int4 list_task(int P, long long off, int cnt) {
+    return make_int4(P, static_cast<int>(static_cast<unsigned>(off & 0xffffffffll)),
+                     cnt | static_cast<int>((off >> 32) << 8), 0);
+}
+
+// LEARNED = false leaves the network out of the kernel (fewer registers,
+// more waves in flight for the density test's gathers)
+template <bool LEARNED>
+__global__ __launch_bounds__(256) void k_route(RouteArgs a) {
+    const long long wave = (static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (wave >= a.nrows) return;  // whole wave leaves together
+    const int4 r = a.rows[wave];
+    const int P = __builtin_amdgcn_readfirstlane(r.x);
+    const int k0 = __builtin_amdgcn_readfirstlane(r.y);
+    const int kend = __builtin_amdgcn_readfirstlane(r.z);
+    const int xy = __builtin_amdgcn_readfirstlane(r.w);
+    const int X = xy / 6, Y = xy - 6 * (xy / 6);
+    int qe = kend;
+    if (a.cut) qe = min(qe, a.cut[6 * P + Y]);
+    const long long base = a.rbase[wave];
+    const long long len = kend - k0;
+    const double QP = a.Q[P];
+    const int ga = a.pgA[P], gb = a.pgB[P];
+    const long long ng = a.ng;
+    const double rhoP = a.blk ? a.blk[ga * ng + gb] : 0.0;
+    const double sigP = a.blk ? a.bsum[ga * ng + gb] : 0.0;
+    const double maxD = a.gate > 0.0 ? *a.maxD : 0.0;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int n64 = 0, n32 = 0;
+    unsigned long long cand = 0, dkept = 0;
+    for (int q0 = k0; q0 < qe; q0 += 64) {
+        const int Qr = q0 + lane;
+        const bool valid = Qr < qe;
+        const int Qi = valid ? Qr : qe - 1;
+        const double qq = QP * a.Q[Qi];
+        bool keep = valid;
+        const long long gc = a.pgA[Qi], gd = a.pgB[Qi];
+        if (a.blk) {
+            const double coul = 4.0 * fmax(rhoP, a.blk[gc * ng + gd]);
+            const double exch = fmax(fmax(a.blk[gc * ng + ga], a.blk[gd * ng + ga]),
+                                     fmax(a.blk[gc * ng + gb], a.blk[gd * ng + gb]));
+            const double fac = a.mode == 1 ? coul : (a.mode == 2 ? exch : fmax(coul, exch));
+            keep = keep && qq * fac >= a.tau;
+        }
+        cand += valid ? 1 : 0;
+        dkept += keep ? 1 : 0;
+        double est = qq;
+        if (LEARNED && __ballot(keep) != 0ull) {  // wave-uniform: every lane runs the network
+            float x[MLP_IN];
+            quartet_features(a.fa, P, Qi, X, Y, x);
+            const float y = mlp_forward_wave(a.mlp, x);
+            est = qq * fmin(1.0, exp(2.302585092994046 * (static_cast<double>(y) + a.margin)));
+            if (a.blk) {
+                const double coul = 4.0 * fmax(sigP, a.bsum[gc * ng + gd]);
+                const double exch = fmax(fmax(a.bsum[gc * ng + ga], a.bsum[gd * ng + ga]),
+                                         fmax(a.bsum[gc * ng + gb], a.bsum[gd * ng + gb]));
+                const double fac = a.mode == 1 ? coul : (a.mode == 2 ? exch : fmax(coul, exch));
+                keep = keep && est * fac >= a.tau;
+            }
+            keep = keep && est >= a.thr;
+        }
+        const bool to32 = keep && a.gate > 0.0 && est * maxD < a.gate;
+        const bool to64 = keep && !to32;
+        const unsigned long long m64 = __ballot(to64), m32 = __ballot(to32);
+        if (to64) a.qlist[base + n64 + __popcll(m64 & below)] = Qr;
+        if (to32) a.qlist[base + len - 1 - (n32 + __popcll(m32 & below))] = Qr;
+        n64 += __popcll(m64);
+        n32 += __popcll(m32);
+    }
+    if (n64 > 0) {
+        const int nt = (n64 + 63) >> 6;
+        unsigned b = 0;
+        if (lane == 0) b = atomicAdd(a.tcount + xy, static_cast<unsigned>(nt));
+        b = __shfl(b, 0, 64);
+        for (int i = lane; i < nt; i += 64)
+            a.t64[a.toff[xy] + b + i] = list_task(P, base + 64ll * i, min(64, n64 - 64 * i));
+    }
+    if (n32 > 0) {
+        const int nt = (n32 + 63) >> 6;
+        unsigned b = 0;
+        if (lane == 0) b = atomicAdd(a.tcount + 36 + xy, static_cast<unsigned>(nt));
+        b = __shfl(b, 0, 64);
+        for (int i = lane; i < nt; i += 64) {
+            const int c = min(64, n32 - 64 * i);
+            a.t32[a.toff[xy] + b + i] = list_task(P, base + len - 64ll * i - c, c);
+        }
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        cand += __shfl_xor(cand, m, 64);
+        dkept += __shfl_xor(dkept, m, 64);
+    }
+    if (lane == 0) {
+        atomicAdd(a.stats + 0, cand);
+        atomicAdd(a.stats + 1, dkept);
+        atomicAdd(a.stats + 2, static_cast<unsigned long long>(n64 + n32));
+        atomicAdd(a.stats + 3, static_cast<unsigned long long>(n32));
+    }
+}
+
+// features of n quartets (pair_p[i] | pair_q[i]); out[32 i ..]
+__global__ void k_features(FeatArgs fa, const int* __restrict__ pcls, const int* __restrict__ pp,
+                           const int* __restrict__ pq, int n, float* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float x[MLP_IN];
+    quartet_features(fa, pp[i], pq[i], pcls[pp[i]], pcls[pq[i]], x);
+    for (int k = 0; k < MLP_IN; ++k) out[static_cast<long long>(i) * MLP_IN + k] = x[k];
 }
 
 void launch_store(const ClassKernels& k, StoreArgs& args, hipStream_t st) {
@@ -619,6 +820,52 @@ void upload_pairs(tlno_gpu_plan* pl, const std::vector<int>& order,
     dev_upload(pl->dpw, pw);
     dev_upload(pl->dpexp, pab);
     dev_upload(pl->dpL, pL);
+}
+
+// Features of each shell pair for the learned screen (plan order):
+//   f0 = {log10 Q, log10 p_min, log10 p_max, log10 mu}, mu and P taken from the
+//   primitive pair with the smallest p (the most diffuse one);
+//   f1 = {|AB|, mu |AB|^2, log10 sum_prim |K c_a c_b / p|, log10 n_prim_pairs};
+//   cen = {P, p_min}.
+void upload_pair_features(tlno_gpu_plan* pl, const std::vector<int>& order,
+                          const std::vector<std::vector<double4>>& prims_P,
+                          const std::vector<std::vector<double>>& prims_w,
+                          const std::vector<std::vector<double>>& prims_ab) {
+    const int np = static_cast<int>(order.size());
+    std::vector<float4> f0(np), f1(np), cen(np);
+    auto lg = [](double v) { return static_cast<float>(std::log10(std::max(v, 1e-300))); };
+    for (int k = 0; k < np; ++k) {
+        const int src = order[k];
+        const auto& P = prims_P[src];
+        const GroupH& Ga = pl->groups[pl->pgA[k]];
+        const GroupH& Gb = pl->groups[pl->pgB[k]];
+        double r2 = 0.0;
+        for (int x = 0; x < 3; ++x) r2 += (Ga.C[x] - Gb.C[x]) * (Ga.C[x] - Gb.C[x]);
+        if (P.empty()) {
+            f0[k] = make_float4(lg(pl->Q[k]), 0.0f, 0.0f, 0.0f);
+            f1[k] = make_float4(static_cast<float>(std::sqrt(r2)), 0.0f, -300.0f, 0.0f);
+            cen[k] = make_float4(static_cast<float>(Ga.C[0]), static_cast<float>(Ga.C[1]),
+                                 static_cast<float>(Ga.C[2]), 1.0f);
+            continue;
+        }
+        std::size_t imin = 0;
+        double pmax = 0.0, wsum = 0.0;
+        for (std::size_t i = 0; i < P.size(); ++i) {
+            if (P[i].w < P[imin].w) imin = i;
+            pmax = std::max(pmax, P[i].w);
+            wsum += std::abs(prims_w[src][i]);
+        }
+        const double al = prims_ab[src][2 * imin], be = prims_ab[src][2 * imin + 1];
+        const double pmin = P[imin].w, mu = al * be / pmin;
+        f0[k] = make_float4(lg(pl->Q[k]), lg(pmin), lg(pmax), lg(mu));
+        f1[k] = make_float4(static_cast<float>(std::sqrt(r2)), static_cast<float>(mu * r2), lg(wsum),
+                            lg(static_cast<double>(P.size())));
+        cen[k] = make_float4(static_cast<float>(P[imin].x), static_cast<float>(P[imin].y),
+                             static_cast<float>(P[imin].z), static_cast<float>(pmin));
+    }
+    dev_upload(pl->d_pf0, f0);
+    dev_upload(pl->d_pf1, f1);
+    dev_upload(pl->d_pcen, cen);
 }
 
 void build_plan(tlno_gpu_plan* pl, int n, const double* centers, const int* ang, const int* nprim,
@@ -801,6 +1048,7 @@ void build_plan(tlno_gpu_plan* pl, int n, const double* centers, const int* ang,
     HIP_OK(hipMalloc(&pl->dJout, mo2 * sizeof(double)));
     HIP_OK(hipMalloc(&pl->dKout, mo2 * sizeof(double)));
     HIP_OK(hipMalloc(&pl->d_scalar, 8 * sizeof(double)));
+    upload_pair_features(pl, final_order, prims_P, prims_w, prims_ab);
     // density-screen metadata
     {
         std::vector<int> row_group(pl->Mout), pcls(np), clsb(7);
@@ -815,6 +1063,19 @@ void build_plan(tlno_gpu_plan* pl, int n, const double* centers, const int* ang,
             for (int P = pl->cls_begin[c]; P < pl->cls_begin[c + 1]; ++P) pcls[P] = c;
         for (int c = 0; c <= 6; ++c) clsb[c] = pl->cls_begin[c];
         dev_upload(pl->d_row_group, row_group);
+        std::vector<int> grows(6 * static_cast<std::size_t>(std::max(1, ng)), -1);
+        for (int g = 0; g < ng; ++g) {
+            const GroupH& G = pl->groups[g];
+            if (pl->spherical)
+                for (int s = 0; s < 2 * G.L + 1; ++s) grows[6 * g + s] = G.sph_base + s;
+            else
+                for (int k = 0; k < ncart(G.L); ++k) grows[6 * g + k] = G.comp[k];
+        }
+        dev_upload(pl->d_grows, grows);
+        HIP_OK(hipMalloc(&pl->d_blk, std::max<std::size_t>(1, static_cast<std::size_t>(ng) * ng) * sizeof(double)));
+        HIP_OK(hipMalloc(&pl->d_bsum, std::max<std::size_t>(1, static_cast<std::size_t>(ng) * ng) * sizeof(double)));
+        HIP_OK(hipMalloc(&pl->d_tcount, 72 * sizeof(unsigned)));
+        HIP_OK(hipMalloc(&pl->d_rstats, 4 * sizeof(unsigned long long)));
         dev_upload(pl->d_pgA, pl->pgA);
         dev_upload(pl->d_pgB, pl->pgB);
         dev_upload(pl->d_pcls, pcls);
@@ -824,11 +1085,11 @@ void build_plan(tlno_gpu_plan* pl, int n, const double* centers, const int* ang,
         HIP_OK(hipMalloc(&pl->d_clsmax, 8 * sizeof(unsigned long long)));
         HIP_OK(hipMalloc(&pl->d_pairfac, std::max(1, np) * sizeof(double)));
         HIP_OK(hipMalloc(&pl->d_cut, 6 * static_cast<std::size_t>(std::max(1, np)) * sizeof(int)));
-        HIP_OK(hipMalloc(&pl->d_counter, 2 * sizeof(unsigned long long)));
+        HIP_OK(hipMalloc(&pl->d_counter, 4 * sizeof(unsigned long long)));
     }
 }
 
-const TaskSet& get_tasks(tlno_gpu_plan* pl, double thr) {
+TaskSet& get_tasks(tlno_gpu_plan* pl, double thr) {
     auto it = pl->tasks.find(thr);
     if (it != pl->tasks.end()) return it->second;
     TaskSet ts;
@@ -913,12 +1174,81 @@ const TaskSet& get_tasks(tlno_gpu_plan* pl, double thr) {
     return pl->tasks.emplace(thr, ts).first->second;
 }
 
+// Candidate rows for the per-quartet routing kernel (once per threshold):
+// every Schwarz-surviving (P | ket range) split into rows of <= kRouteSeg
+// kets, each owning a qlist region of its length; per class pair the task
+// capacity is the number of 64-ket chunks of its rows (either route).
+void ensure_route(tlno_gpu_plan* pl, TaskSet& ts, double thr) {
+    if (ts.d_rbase) return;
+    std::vector<int4> rows;
+    std::vector<long long> rbase;
+    long long q = 0, cap = 0;
+    for (int X = 5; X >= 0; --X)
+        for (int Y = X; Y >= 0; --Y) {
+            const int xy = 6 * X + Y;
+            ts.toff[xy] = cap;
+            ts.tcap[xy] = 0;
+            const int k0 = pl->cls_begin[Y], k1 = pl->cls_begin[Y + 1];
+            for (int P = pl->cls_begin[X]; P < pl->cls_begin[X + 1] && k1 > k0; ++P) {
+                int end = k1;
+                if (thr > 0.0) {
+                    int lo = k0, hi = k1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) / 2;
+                        if (pl->Q[P] * pl->Q[mid] >= thr) lo = mid + 1;
+                        else hi = mid;
+                    }
+                    end = lo;
+                }
+                if (X == Y) end = std::min(end, P + 1);
+                for (int s0 = k0; s0 < end; s0 += kRouteSeg) {
+                    const int s1 = std::min(end, s0 + kRouteSeg);
+                    rows.push_back(make_int4(P, s0, s1, xy));
+                    rbase.push_back(q);
+                    q += s1 - s0;
+                    ts.tcap[xy] += (s1 - s0 + 63) / 64;
+                }
+            }
+            cap += ts.tcap[xy];
+        }
+    if (q >= (1ll << 55)) throw std::runtime_error("tlno_gpu: quartet list too large for the task encoding");
+    ts.nrows = static_cast<long long>(rows.size());
+    ts.qtotal = q;
+    ts.tcap_total = cap;
+    dev_upload(ts.d_rows, rows);
+    rbase.push_back(q);  // never empty: marks the list as built
+    dev_upload(ts.d_rbase, rbase);
+    if (q > pl->qlist_cap) {
+        if (pl->d_qlist) HIP_OK(hipFree(pl->d_qlist));
+        pl->d_qlist = nullptr;
+        HIP_OK(hipMalloc(&pl->d_qlist, q * sizeof(int)));
+        pl->qlist_cap = q;
+    }
+    if (cap > pl->tlist_cap) {
+        if (pl->d_t64) HIP_OK(hipFree(pl->d_t64));
+        if (pl->d_t32) HIP_OK(hipFree(pl->d_t32));
+        pl->d_t64 = pl->d_t32 = nullptr;
+        HIP_OK(hipMalloc(&pl->d_t64, cap * sizeof(int4)));
+        HIP_OK(hipMalloc(&pl->d_t32, cap * sizeof(int4)));
+        pl->tlist_cap = cap;
+    }
+}
+
+// bra primitive slices per list-mode task so that a small launch still
+// spreads over the machine (same policy as the range-mode task split)
+int list_split(long long ntask, int nchunk) {
+    const long long waves = ntask * nchunk;
+    constexpr long long kTargetWaves = 16384;
+    if (waves <= 0 || waves >= kTargetWaves) return 1;
+    return static_cast<int>(std::min<long long>(256, (kTargetWaves + waves - 1) / waves));
+}
+
 void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, double thr) {
     DeviceCtx& ctx = device_ctx(pl->device);
     HIP_OK(hipSetDevice(pl->device));
     if (!std::isfinite(thr) || thr < 0.0)
         throw std::runtime_error("eri_ao_tensor_hgp: schwarz_threshold must be finite and >= 0.");
-    const TaskSet& ts = get_tasks(pl, thr);
+    TaskSet& ts = get_tasks(pl, thr);
     const hipStream_t st = pl->stream;
     const long mc2 = static_cast<long>(pl->Mcart) * pl->Mcart;
     const long mo2 = static_cast<long>(pl->Mout) * pl->Mout;
@@ -954,7 +1284,7 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
     a.cut = nullptr;
     a.qlist = nullptr;
     a.counter = pl->d_counter;
-    k_fill<<<1, 64, 0, st>>>(reinterpret_cast<double*>(pl->d_counter), 2, 0.0);
+    k_fill<<<1, 64, 0, st>>>(reinterpret_cast<double*>(pl->d_counter), 4, 0.0);
     if (pl->density_thr > 0.0 && (a.do_j || a.do_k)) {
         const int ng = static_cast<int>(pl->groups.size());
         k_fill<<<grid_for(ng, 256), 256, 0, st>>>(reinterpret_cast<double*>(pl->d_gmax), ng, 0.0);
@@ -965,7 +1295,50 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
         k_density_cut<<<grid_for(pl->npair, 256), 256, 0, st>>>(pl->d_Q, pl->d_pcls, pl->d_clsb, pl->d_pairfac,
                                                                 pl->d_clsmax, pl->npair, pl->density_thr,
                                                                 pl->d_cut);
+        const long long ng2 = static_cast<long long>(ng) * ng;
+        if (ng2 > 0)
+            k_block_max<<<grid_for(ng2, 256), 256, 0, st>>>(dD, pl->d_grows, ng, pl->Mout, pl->d_blk, pl->d_bsum);
         HIP_OK(hipGetLastError());
+    }
+    a.split = 1;
+    const bool route = (pl->density_thr > 0.0 || pl->learned) && (a.do_j || a.do_k);
+    unsigned tcount[72] = {};
+    if (route) {
+        ensure_route(pl, ts, thr);
+        k_fill<<<1, 64, 0, st>>>(reinterpret_cast<double*>(pl->d_tcount), 36, 0.0);
+        k_fill<<<1, 64, 0, st>>>(reinterpret_cast<double*>(pl->d_rstats), 4, 0.0);
+        RouteArgs ra;
+        ra.rows = ts.d_rows;
+        ra.rbase = ts.d_rbase;
+        ra.nrows = ts.nrows;
+        ra.cut = pl->density_thr > 0.0 ? pl->d_cut : nullptr;
+        ra.Q = pl->d_Q;
+        ra.pgA = pl->d_pgA;
+        ra.pgB = pl->d_pgB;
+        ra.ng = static_cast<int>(pl->groups.size());
+        ra.blk = pl->density_thr > 0.0 ? pl->d_blk : nullptr;
+        ra.bsum = pl->d_bsum;
+        ra.mode = (a.do_j && a.do_k) ? 0 : (a.do_j ? 1 : 2);
+        ra.tau = pl->density_thr;
+        ra.thr = thr;
+        ra.mlp = MlpPacked{pl->d_frag, pl->d_mvec, pl->d_mnorm, pl->mb3};
+        ra.fa = FeatArgs{pl->d_pf0, pl->d_pf1, pl->d_pcen};
+        ra.margin = pl->learned_margin;
+        ra.gate = pl->fp32_gate;
+        ra.maxD = pl->d_scalar;
+        ra.qlist = pl->d_qlist;
+        ra.t64 = pl->d_t64;
+        ra.t32 = pl->d_t32;
+        for (int i = 0; i < 36; ++i) ra.toff[i] = ts.toff[i];
+        ra.tcount = pl->d_tcount;
+        ra.stats = pl->d_rstats;
+        if (ts.nrows > 0) {
+            if (pl->learned) k_route<true><<<grid_for(ts.nrows * 64, 256), 256, 0, st>>>(ra);
+            else k_route<false><<<grid_for(ts.nrows * 64, 256), 256, 0, st>>>(ra);
+        }
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(tcount, pl->d_tcount, sizeof(tcount), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
     }
     if (a.do_j || a.do_k) {
         // Class-pair kernels are independent (their J/K contributions are
@@ -978,8 +1351,36 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
             for (int s = 0; s < kSideStreams; ++s) HIP_OK(hipStreamWaitEvent(pl->side[s], pl->ev_fork, 0));
         }
         int rr = 0;
-        pl->last_fp32_sq = pl->last_fp32_pq = 0;
-        for (int X = 5; X >= 0; --X)
+        for (int X = 5; X >= 0 && route; --X)
+            for (int Y = X; Y >= 0; --Y) {
+                const int xy = 6 * X + Y;
+                const long long n64 = tcount[xy], n32 = tcount[36 + xy];
+                if (n64 + n32 == 0) continue;
+                const ClassKernels& k = g_kernels[X][Y];
+                const hipStream_t cs = concurrent ? pl->side[rr++ % kSideStreams] : st;
+                if (pl->timing) HIP_OK(hipEventRecord(pl->ev[X][Y][0], cs));
+                a.cut = nullptr;
+                a.qlist = pl->d_qlist;
+                if (n64 > 0) {
+                    a.tasks = pl->d_t64 + ts.toff[xy];
+                    a.ntasks = static_cast<int>(n64);
+                    a.split = list_split(n64, k.nchunk);
+                    launch_jk(k.jk, k.nchunk, a, n64 * a.split, cs);
+                }
+                if (n32 > 0) {
+                    // the FP32 share goes to a stream of its own so that it
+                    // overlaps the class's FP64 share
+                    const hipStream_t cs32 = (concurrent && n64 > 0) ? pl->side[rr++ % kSideStreams] : cs;
+                    a.tasks = pl->d_t32 + ts.toff[xy];
+                    a.ntasks = static_cast<int>(n32);
+                    a.split = list_split(n32, k.nchunk);
+                    launch_jk(k.jk32, k.nchunk, a, n32 * a.split, cs32);
+                }
+                if (pl->timing) HIP_OK(hipEventRecord(pl->ev[X][Y][1], cs));
+            }
+        a.qlist = nullptr;
+        a.split = 1;
+        for (int X = 5; X >= 0 && !route; --X)
             for (int Y = X; Y >= 0; --Y) {
                 const long long nt = ts.cnt[X][Y];
                 if (nt == 0) continue;
@@ -990,13 +1391,11 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
                     const double cut = pl->fp32_gate / std::max(maxD, 1e-300);
                     n64 = std::lower_bound(bm.begin(), bm.end(), cut, [](double v, double c) { return v >= c; }) -
                           bm.begin();
-                    pl->last_fp32_sq += ts.cum_sq[X][Y][nt] - ts.cum_sq[X][Y][n64];
-                    pl->last_fp32_pq += ts.cum_pq[X][Y][nt] - ts.cum_pq[X][Y][n64];
                 }
                 const ClassKernels& k = g_kernels[X][Y];
                 const hipStream_t cs = concurrent ? pl->side[rr++ % kSideStreams] : st;
                 if (pl->timing) HIP_OK(hipEventRecord(pl->ev[X][Y][0], cs));
-                a.cut = pl->density_thr > 0.0 ? pl->d_cut + Y : nullptr;
+                a.cut = nullptr;  // density screening always takes the routed path
                 if (n64 > 0) {
                     a.tasks = ts.d_tasks + ts.off[X][Y];
                     a.ntasks = static_cast<int>(n64);
@@ -1032,10 +1431,19 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(st));
     {
-        unsigned long long evaluated[2] = {0, 0};
+        unsigned long long evaluated[4] = {0, 0, 0, 0};
         HIP_OK(hipMemcpy(evaluated, pl->d_counter, sizeof(evaluated), hipMemcpyDeviceToHost));
         pl->last_evaluated = static_cast<long long>(evaluated[0]);
         pl->last_evaluated_prim = static_cast<long long>(evaluated[1]);
+        pl->last_fp32_sq = static_cast<long long>(evaluated[2]);
+        pl->last_fp32_pq = static_cast<long long>(evaluated[3]);
+        if (route) {
+            unsigned long long rs[4] = {0, 0, 0, 0};
+            HIP_OK(hipMemcpy(rs, pl->d_rstats, sizeof(rs), hipMemcpyDeviceToHost));
+            for (int i = 0; i < 4; ++i) pl->last_route[i] = static_cast<long long>(rs[i]);
+        } else {
+            for (int i = 0; i < 4; ++i) pl->last_route[i] = 0;
+        }
     }
     pl->last_shell_quartets = ts.shell_quartets;
     pl->last_prim_quartets = ts.prim_quartets;
@@ -1064,19 +1472,6 @@ void validate_density(const double* D, int M) {
         }
 }
 
-template <class F>
-int guarded(F&& f) {
-    try {
-        f();
-        return 0;
-    } catch (const std::exception& e) {
-        g_err = e.what();
-        return 1;
-    } catch (...) {
-        g_err = "tlno_gpu: unknown error";
-        return 1;
-    }
-}
 
 }  // namespace tlno_amd
 
@@ -1089,7 +1484,13 @@ tlno_gpu_plan::~tlno_gpu_plan() {
     f(dDin); f(dDint); f(dJacc); f(dKacc); f(dJout); f(dKout); f(d_scalar);
     f(d_row_group); f(d_pgA); f(d_pgB); f(d_pcls); f(d_clsb); f(d_Q); f(d_gmax); f(d_clsmax); f(d_pairfac);
     f(d_cut); f(d_counter);
-    for (auto& kv : tasks) f(kv.second.d_tasks);
+    f(d_pf0); f(d_pf1); f(d_pcen); f(d_grows); f(d_blk); f(d_bsum); f(d_frag); f(d_mvec); f(d_mnorm);
+    f(d_qlist); f(d_t64); f(d_t32); f(d_tcount); f(d_rstats);
+    for (auto& kv : tasks) {
+        f(kv.second.d_tasks);
+        f(kv.second.d_rows);
+        f(kv.second.d_rbase);
+    }
     if (timing)
         for (int X = 0; X < 6; ++X)
             for (int Y = 0; Y < 6; ++Y)
@@ -1255,6 +1656,66 @@ int tlno_gpu_set_mixed_precision(tlno_gpu_plan* pl, double fp32_threshold) {
         if (!std::isfinite(fp32_threshold) || fp32_threshold < 0.0)
             throw std::runtime_error("tlno_gpu: fp32 threshold must be finite and >= 0");
         pl->fp32_gate = fp32_threshold;
+    });
+}
+
+int tlno_gpu_quartet_features(tlno_gpu_plan* pl, int n, const int* pair_p, const int* pair_q, float* out) {
+    return guarded([&] {
+        if (!pl) throw std::runtime_error("jk_direct: null J/K plan backend.");
+        if (n < 0 || (n > 0 && (!pair_p || !pair_q || !out))) throw std::runtime_error("tlno_gpu: bad feature arguments");
+        for (int i = 0; i < n; ++i)
+            if (pair_p[i] < 0 || pair_p[i] >= pl->npair || pair_q[i] < 0 || pair_q[i] >= pl->npair)
+                throw std::runtime_error("tlno_gpu: pair index out of range");
+        if (n == 0) return;
+        HIP_OK(hipSetDevice(pl->device));
+        int *dp = nullptr, *dq = nullptr;
+        float* dout = nullptr;
+        HIP_OK(hipMalloc(&dp, n * sizeof(int)));
+        HIP_OK(hipMalloc(&dq, n * sizeof(int)));
+        HIP_OK(hipMalloc(&dout, static_cast<std::size_t>(n) * MLP_IN * sizeof(float)));
+        HIP_OK(hipMemcpy(dp, pair_p, n * sizeof(int), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(dq, pair_q, n * sizeof(int), hipMemcpyHostToDevice));
+        k_features<<<grid_for(n, 256), 256, 0, pl->stream>>>(FeatArgs{pl->d_pf0, pl->d_pf1, pl->d_pcen}, pl->d_pcls,
+                                                             dp, dq, n, dout);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipStreamSynchronize(pl->stream));
+        HIP_OK(hipMemcpy(out, dout, static_cast<std::size_t>(n) * MLP_IN * sizeof(float), hipMemcpyDeviceToHost));
+        HIP_OK(hipFree(dp));
+        HIP_OK(hipFree(dq));
+        HIP_OK(hipFree(dout));
+    });
+}
+
+int tlno_gpu_set_learned_screen(tlno_gpu_plan* pl, const float* params, double margin) {
+    return guarded([&] {
+        if (!pl) throw std::runtime_error("jk_direct: null J/K plan backend.");
+        if (!params) {
+            pl->learned = 0;
+            return;
+        }
+        if (!std::isfinite(margin)) throw std::runtime_error("tlno_gpu: learned-screen margin must be finite");
+        for (int i = 0; i < MLP_NALL; ++i)
+            if (!std::isfinite(params[i])) throw std::runtime_error("tlno_gpu: non-finite network parameter");
+        HIP_OK(hipSetDevice(pl->device));
+        std::vector<uint4> frag(MLP_NFRAG * 64);
+        std::vector<float> vec(192), norm(64);
+        float b3 = 0.0f;
+        mlp_pack(params, frag.data(), vec.data(), norm.data(), &b3);
+        HIP_OK(hipStreamSynchronize(pl->stream));
+        dev_upload(pl->d_frag, frag);
+        dev_upload(pl->d_mvec, vec);
+        dev_upload(pl->d_mnorm, norm);
+        pl->mb3 = b3;
+        pl->learned_margin = margin;
+        pl->learned = 1;
+    });
+}
+
+int tlno_gpu_last_route_stats(const tlno_gpu_plan* pl, long long* stats) {
+    return guarded([&] {
+        if (!pl) throw std::runtime_error("jk_direct: null J/K plan backend.");
+        if (!stats) throw std::runtime_error("tlno_gpu: null output");
+        for (int i = 0; i < 4; ++i) stats[i] = pl->last_route[i];
     });
 }
 

@@ -29,6 +29,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 #include "tables.hpp"
 
@@ -321,8 +322,10 @@ __device__ __forceinline__ void quartet(const DevTables& tb, const PairData& pd,
 struct JKArgs {
     PairData pd;
     DevTables tb;
-    const int4* tasks;  // range mode {P, q_begin, q_end, slice}; list mode {P, offset, count, slice}
+    const int4* tasks;  // range mode {P, q_begin, q_end, slice};
+                        // list mode {P, offset bits 0-31, count | offset bits 32-55 << 8, unused}
     int ntasks;
+    int split;          // list mode: waves per task (bra primitive slices)
     const double* D;    // internal Cartesian density, M x M
     double* J;          // Jacc
     double* K;          // Kacc
@@ -330,7 +333,7 @@ struct JKArgs {
     int do_j, do_k;
     const int* cut;     // density screen: ket range of P ends at cut[6 P] (pointer offset by ket class; nullptr: off)
     const int* qlist;   // list mode: kets of a task are qlist[offset .. offset + count)
-    unsigned long long* counter;  // shell quartets evaluated (nullptr: not counted)
+    unsigned long long* counter;  // [0] shell, [1] primitive quartets evaluated, [2], [3] the FP32 share
 };
 
 struct StoreArgs {
@@ -354,34 +357,54 @@ __device__ __forceinline__ void eri_jk_body(const JKArgs& a) {
     using S = Shape<LA, LB, LC, LD, AC, CC>;
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
-    if (wave >= a.ntasks) return;  // whole wave leaves together
-    const int4 t = a.tasks[wave];
+    const int task = a.qlist ? wave / a.split : wave;
+    if (task >= a.ntasks) return;  // whole wave leaves together
+    const int4 t = a.tasks[task];
     // the task is wave-uniform: keep it in scalar registers
     const int P = __builtin_amdgcn_readfirstlane(t.x);
-    const int qb = __builtin_amdgcn_readfirstlane(t.y);
-    int qe = __builtin_amdgcn_readfirstlane(t.z);
-    const int tw = __builtin_amdgcn_readfirstlane(t.w);
     bool active;
-    int Q;
-    if (a.qlist) {  // list mode: qb = offset, qe = count
-        if (qe <= 0) return;
-        active = lane < qe;
-        Q = a.qlist[qb + (active ? lane : qe - 1)];
+    int Q, ib0, ib1, nq;
+    if (a.qlist) {
+        // list mode: up to 64 kets at qlist[offset ..], bra primitive slice
+        // (wave % split) of split near-equal slices
+        const long long off = static_cast<long long>(static_cast<unsigned>(__builtin_amdgcn_readfirstlane(t.y))) |
+                              (static_cast<long long>(__builtin_amdgcn_readfirstlane(t.z) >> 8) << 32);
+        nq = __builtin_amdgcn_readfirstlane(t.z) & 255;
+        const int pbn = a.pd.pn[P];
+        const int ns = min(a.split, pbn);
+        const int slice = wave - task * a.split;
+        if (nq <= 0 || slice >= ns) return;
+        const int step = (pbn + ns - 1) / ns;
+        ib0 = slice * step;
+        ib1 = min(pbn, ib0 + step);
+        if (ib0 >= ib1) return;
+        active = lane < nq;
+        Q = a.qlist[off + (active ? lane : nq - 1)];
     } else {
+        const int qb = __builtin_amdgcn_readfirstlane(t.y);
+        int qe = __builtin_amdgcn_readfirstlane(t.z);
+        const int tw = __builtin_amdgcn_readfirstlane(t.w);
         if (a.cut) qe = min(qe, a.cut[6 * P]);  // density screen (per bra pair, this ket class)
-        if (qe <= qb) return;                          // whole wave leaves together
+        if (qe <= qb) return;                     // whole wave leaves together
         const int qraw = qb + lane;
         active = qraw < qe;
         Q = active ? qraw : qe - 1;
+        nq = min(qe - qb, 64);
+        ib0 = tw & 0xffff;
+        ib1 = (tw >> 16) & 0xffff;
     }
-    const int ib0 = tw & 0xffff, ib1 = (tw >> 16) & 0xffff;
-    if (a.counter && blockIdx.y == 0) {  // [0] shell quartets, [1] primitive quartets evaluated (chunk 0 counts)
+    if (a.counter && blockIdx.y == 0) {  // chunk 0 counts
         int npq = active ? a.pd.pn[Q] * (ib1 - ib0) : 0;
 #pragma unroll
         for (int m = 32; m >= 1; m >>= 1) npq += __shfl_xor(npq, m, 64);
         if (lane == 0) {
-            if (ib0 == 0) atomicAdd(a.counter, static_cast<unsigned long long>(a.qlist ? qe : min(qe - qb, 64)));
+            constexpr int base = std::is_same<R, float>::value ? 2 : 0;
+            if (ib0 == 0) {
+                atomicAdd(a.counter, static_cast<unsigned long long>(nq));
+                if (base) atomicAdd(a.counter + base, static_cast<unsigned long long>(nq));
+            }
             atomicAdd(a.counter + 1, static_cast<unsigned long long>(npq));
+            if (base) atomicAdd(a.counter + base + 1, static_cast<unsigned long long>(npq));
         }
     }
 

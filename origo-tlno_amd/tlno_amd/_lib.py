@@ -16,6 +16,7 @@ LIB_PATH = os.environ.get("TLNO_AMD_LIB") or os.path.join(_HERE, "libtlno_amd.so
 D = C.POINTER(C.c_double)
 I = C.POINTER(C.c_int)
 LL = C.POINTER(C.c_longlong)
+F = C.POINTER(C.c_float)
 
 # every symbol the header declares, with (restype, argtypes)
 SIGNATURES = {
@@ -39,6 +40,19 @@ SIGNATURES = {
     "tlno_gpu_boys": (C.c_int, [C.c_int, D, C.c_int, D]),
     "tlno_gpu_rys": (C.c_int, [C.c_int, C.c_int, D, D, D]),
     "tlno_gpu_onee": (C.c_int, [C.c_void_p, C.c_int, D, D, D, D, D]),
+    "tlno_gpu_set_learned_screen": (C.c_int, [C.c_void_p, F, C.c_double]),
+    "tlno_gpu_quartet_features": (C.c_int, [C.c_void_p, C.c_int, I, I, F]),
+    "tlno_gpu_last_route_stats": (C.c_int, [C.c_void_p, LL]),
+    "tlno_gpu_mlp_create": (C.c_int, [C.c_int, C.c_uint, C.POINTER(C.c_void_p)]),
+    "tlno_gpu_mlp_destroy": (None, [C.c_void_p]),
+    "tlno_gpu_mlp_get_params": (C.c_int, [C.c_void_p, F]),
+    "tlno_gpu_mlp_set_params": (C.c_int, [C.c_void_p, F]),
+    "tlno_gpu_mlp_set_normalization": (C.c_int, [C.c_void_p, C.c_longlong, F]),
+    "tlno_gpu_mlp_loss_grad": (C.c_int, [C.c_void_p, C.c_longlong, F, F, C.c_float, F, D]),
+    "tlno_gpu_mlp_adam_step": (C.c_int, [C.c_void_p, C.c_longlong, F, F, C.c_float, C.c_float]),
+    "tlno_gpu_mlp_train": (C.c_int, [C.c_void_p, C.c_longlong, F, F, C.c_int, C.c_int, C.c_float, C.c_float,
+                                      C.c_uint, F]),
+    "tlno_gpu_mlp_predict": (C.c_int, [C.c_void_p, C.c_longlong, F, F, C.c_int]),
 }
 
 _lib = None

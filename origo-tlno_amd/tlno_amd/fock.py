@@ -155,6 +155,25 @@ class DirectPlan:
         FP32 recurrences (0 disables); J/K accumulation stays FP64."""
         _lib.check(_lib.load().tlno_gpu_set_mixed_precision(self._h, float(fp32_threshold)))
 
+    def set_learned_screen(self, params=None, margin=0.0):
+        """Turn the learned shell-quartet screen on (params: 6401 floats of a
+        trained network, margin in decades) or off (params None)."""
+        lib = _lib.load()
+        if params is None:
+            _lib.check(lib.tlno_gpu_set_learned_screen(self._h, None, 0.0))
+            return
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.shape != (6401,):
+            raise ValueError("learned screen: expected 6401 network parameters")
+        _lib.check(lib.tlno_gpu_set_learned_screen(self._h, p.ctypes.data_as(_lib.F), float(margin)))
+
+    def last_route_stats(self):
+        """Per-quartet routing counts of the last build: candidates, kept by
+        the density test, kept after the learned screen, sent to FP32."""
+        out = np.zeros(4, np.int64)
+        _lib.check(_lib.load().tlno_gpu_last_route_stats(self._h, out.ctypes.data_as(_lib.LL)))
+        return {"candidates": int(out[0]), "density_kept": int(out[1]), "kept": int(out[2]), "fp32": int(out[3])}
+
     def last_mixed_stats(self):
         sq = C.c_longlong()
         pq = C.c_longlong()

@@ -23,6 +23,11 @@ tests) and the reference's outputs:
                    configs.spherical_density(321, 5, 60): G = J - K/2 at
                    Schwarz 1e-12 in full, and 16 bilinear probes u_i^T X v_i
                    of J and K (python gen_golden.py organic30 rewrites only it)
+  density.npz      4-water cluster (96 spherical functions), density
+                   configs.spherical_density(96, 9, 20): the reference's
+                   density-screened J, K and G (density_threshold 1e-9, 1e-7,
+                   1e-5; Schwarz 1e-12) and its unscreened G
+                   (python gen_golden.py density rewrites only it)
 """
 import hashlib
 import os
@@ -53,6 +58,8 @@ def main(only=None):
     ref = O.Reference()
     if only == "organic30":
         return organic30(ref)
+    if only == "density":
+        return density(ref)
     t0 = time.time()
     # ---------------------------------------------------------- Boys / Rys
     xs = np.concatenate([[0.0, 1e-10, 1e-3, 0.5, 5.0, 11.99, 12.0, 12.5, 25.0, 39.99, 40.0, 45.0, 80.0, 200.0],
@@ -114,7 +121,20 @@ def main(only=None):
     np.savez_compressed(os.path.join(HERE, "organic5.npz"), **out)
 
     organic30(ref)
+    density(ref)
     print(f"golden fixtures written in {time.time() - t0:.1f} s")
+
+
+def density(ref):
+    # ---------------------------------------------------------- density screen
+    b, _ = configs.water_cluster(4)
+    M = ref.plan_M(b)
+    Ds = sph_density(M, 9, 20)
+    out = {"basis_sha": basis_hash(b), "G_0": ref.direct(b, Ds, "G", schwarz=1e-12, threads=1)[0]}
+    for tau, tag in ((1e-9, "9"), (1e-7, "7"), (1e-5, "5")):
+        for w in "JKG":
+            out[f"{w}_{tag}"] = ref.direct(b, Ds, w, schwarz=1e-12, density_threshold=tau, threads=1)[0]
+    np.savez_compressed(os.path.join(HERE, "density.npz"), **out)
 
 
 def organic30(ref):

@@ -6,6 +6,8 @@ reference compiled from its sources), through the C ABI.
   organic5    spherical J/K/G with Cartesian-generated d shells
   organic30   the benchmark molecule in the reference's spherical basis:
               G in full and bilinear probes of J and K, Schwarz 1e-12
+  density     density-screened J/K/G of a 4-water cluster at three
+              density thresholds
 Tolerance for J/K/G: max-abs 1e-10.
 """
 import os
@@ -97,3 +99,23 @@ def test_organic30_benchmark_molecule_vs_reference():
     assert np.allclose(kp, g["K_probe"], rtol=0, atol=1e-9)
     assert np.linalg.norm(J) == pytest.approx(float(g["J_fro"]), rel=1e-12)
     assert np.linalg.norm(K) == pytest.approx(float(g["K_fro"]), rel=1e-12)
+
+
+@pytest.mark.parametrize("tau,tag", [(1e-9, "9"), (1e-7, "7"), (1e-5, "5")])
+def test_density_screen_vs_reference(tau, tag):
+    """The reference's density-screened J (J-only test), K (K-only test) and
+    G (combined test): the per-quartet test keeps the same quartets, so the
+    results agree to 1e-10 even where the screen changes G by 1e-5."""
+    g = load("density.npz")
+    b, _ = configs.water_cluster(4)
+    plan = DirectPlan(b, ScreeningOptions(integral_threshold=1e-12))
+    Ds = configs.spherical_density(plan.M, 9, 20)
+    J, _ = plan.jk(Ds, True, False, density_threshold=tau)
+    _, K = plan.jk(Ds, False, True, density_threshold=tau)
+    G = build_rhf_g(plan, Ds, ScreeningOptions(integral_threshold=1e-12, density_threshold=tau)).g
+    print(f"tau {tau:g}: screen moves G by {np.abs(g[f'G_{tag}'] - g['G_0']).max():.2e}; "
+          f"|dJ| {np.abs(J - g[f'J_{tag}']).max():.2e} |dK| {np.abs(K - g[f'K_{tag}']).max():.2e} "
+          f"|dG| {np.abs(G - g[f'G_{tag}']).max():.2e}")
+    assert np.abs(J - g[f"J_{tag}"]).max() < 1e-10
+    assert np.abs(K - g[f"K_{tag}"]).max() < 1e-10
+    assert np.abs(G - g[f"G_{tag}"]).max() < 1e-10
