@@ -499,10 +499,15 @@ __global__ void k_rys(DevTables tb, const double* T, double* r, double* w, int n
 
 
 // grid: x covers the tasks (one wave each, 4 waves per block), y the chunks
-void launch_jk(JKKernel kern, int nchunk, JKArgs& args, long long ntask, hipStream_t st) {
+// nwaves waves per chunk, every chunk group of the class (FP32 or FP64 kernels)
+void launch_jk(const ClassKernels& k, bool fp32, JKArgs& args, long long nwaves, hipStream_t st) {
     void* params[] = {&args};
-    const unsigned grid = grid_for(ntask * 64, 256);
-    HIP_OK(hipLaunchKernel(reinterpret_cast<const void*>(kern), dim3(grid, nchunk), dim3(256), params, 0, st));
+    const unsigned grid = grid_for(nwaves * 64, 256);
+    for (int g = 0; g < k.ngroup; ++g) {
+        const JKKernel kern = fp32 ? k.grp[g].jk32 : k.grp[g].jk;
+        HIP_OK(hipLaunchKernel(reinterpret_cast<const void*>(kern), dim3(grid, k.grp[g].n), dim3(256), params, 0,
+                               st));
+    }
 }
 
 // ---- density screen (per build) ----
@@ -1365,7 +1370,7 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
                     a.tasks = pl->d_t64 + ts.toff[xy];
                     a.ntasks = static_cast<int>(n64);
                     a.split = list_split(n64, k.nchunk);
-                    launch_jk(k.jk, k.nchunk, a, n64 * a.split, cs);
+                    launch_jk(k, false, a, n64 * a.split, cs);
                 }
                 if (n32 > 0) {
                     // the FP32 share goes to a stream of its own so that it
@@ -1374,7 +1379,7 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
                     a.tasks = pl->d_t32 + ts.toff[xy];
                     a.ntasks = static_cast<int>(n32);
                     a.split = list_split(n32, k.nchunk);
-                    launch_jk(k.jk32, k.nchunk, a, n32 * a.split, cs32);
+                    launch_jk(k, true, a, n32 * a.split, cs32);
                 }
                 if (pl->timing) HIP_OK(hipEventRecord(pl->ev[X][Y][1], cs));
             }
@@ -1399,12 +1404,12 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
                 if (n64 > 0) {
                     a.tasks = ts.d_tasks + ts.off[X][Y];
                     a.ntasks = static_cast<int>(n64);
-                    launch_jk(k.jk, k.nchunk, a, n64, cs);
+                    launch_jk(k, false, a, n64, cs);
                 }
                 if (nt > n64) {
                     a.tasks = ts.d_tasks + ts.off[X][Y] + n64;
                     a.ntasks = static_cast<int>(nt - n64);
-                    launch_jk(k.jk32, k.nchunk, a, nt - n64, cs);
+                    launch_jk(k, true, a, nt - n64, cs);
                 }
                 if (pl->timing) HIP_OK(hipEventRecord(pl->ev[X][Y][1], cs));
             }

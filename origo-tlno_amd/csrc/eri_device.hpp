@@ -214,25 +214,27 @@ __device__ __forceinline__ void axis_table(R C00, R D00, R B00, R B10, R B01, R 
         }
 }
 
-template <int LA, int LB, int LC, int LD, int AC, int CC>
+// Output chunk of a class: all bra components a (AC < 0) or just a = AC; all
+// ket components c (CC < 0) or c in [CC, CC + CN).
+template <int LA, int LB, int LC, int LD, int AC, int CC, int CN>
 struct Shape {
     static constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
     static constexpr int LAB = LA + LB, LCD = LC + LD;
     static constexpr int NR = (LAB + LCD) / 2 + 1;
     static constexpr int NAL = AC < 0 ? NA : 1;
-    static constexpr int NCL = CC < 0 ? NC : 1;
+    static constexpr int NCL = CC < 0 ? NC : CN;
     static constexpr int NOUT = NAL * NB * NCL * ND;
     __device__ static constexpr int acomp(int ai) { return AC < 0 ? ai : AC; }
-    __device__ static constexpr int ccomp(int ci) { return CC < 0 ? ci : CC; }
+    __device__ static constexpr int ccomp(int ci) { return CC < 0 ? ci : CC + ci; }
 };
 
 constexpr double kTwoPi52 = 34.986836655249725693;  // 2 pi^{5/2}
 
 // Accumulates the lane's block slice (unit-diagonal components) into acc.
-template <typename R, int LA, int LB, int LC, int LD, int AC, int CC>
+template <typename R, int LA, int LB, int LC, int LD, int AC, int CC, int CN>
 __device__ __forceinline__ void quartet(const DevTables& tb, const PairData& pd, int P, int Q, int ib0, int ib1,
-                                        R (&acc)[Shape<LA, LB, LC, LD, AC, CC>::NOUT]) {
-    using S = Shape<LA, LB, LC, LD, AC, CC>;
+                                        R (&acc)[Shape<LA, LB, LC, LD, AC, CC, CN>::NOUT]) {
+    using S = Shape<LA, LB, LC, LD, AC, CC, CN>;
     constexpr CartT<LA> ca = make_cart<LA>();
     constexpr CartT<LB> cb = make_cart<LB>();
     constexpr CartT<LC> cc = make_cart<LC>();
@@ -352,9 +354,9 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // Task = {P, q_begin, q_end, bra primitive slice ib0 | ib1 << 16}.
-template <typename R, int LA, int LB, int LC, int LD, int AC, int CC>
+template <typename R, int LA, int LB, int LC, int LD, int AC, int CC, int CN>
 __device__ __forceinline__ void eri_jk_body(const JKArgs& a) {
-    using S = Shape<LA, LB, LC, LD, AC, CC>;
+    using S = Shape<LA, LB, LC, LD, AC, CC, CN>;
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     const int task = a.qlist ? wave / a.split : wave;
@@ -393,7 +395,7 @@ __device__ __forceinline__ void eri_jk_body(const JKArgs& a) {
         ib0 = tw & 0xffff;
         ib1 = (tw >> 16) & 0xffff;
     }
-    if (a.counter && blockIdx.y == 0) {  // chunk 0 counts
+    if (a.counter && AC <= 0 && CC <= 0) {  // the class's first output chunk counts
         int npq = active ? a.pd.pn[Q] * (ib1 - ib0) : 0;
 #pragma unroll
         for (int m = 32; m >= 1; m >>= 1) npq += __shfl_xor(npq, m, 64);
@@ -409,7 +411,7 @@ __device__ __forceinline__ void eri_jk_body(const JKArgs& a) {
     }
 
     R acc[S::NOUT];
-    quartet<R, LA, LB, LC, LD, AC, CC>(a.tb, a.pd, P, Q, ib0, ib1, acc);
+    quartet<R, LA, LB, LC, LD, AC, CC, CN>(a.tb, a.pd, P, Q, ib0, ib1, acc);
 
     // degeneracy scale: 1/2 per diagonal pair, 1/2 if P == Q
     double s = 1.0;
@@ -524,14 +526,14 @@ __device__ __forceinline__ void eri_jk_body(const JKArgs& a) {
 
 // Writes the lane's slice of the unit-diagonal Cartesian block of (P|Q) into
 // out[off + ((a*NB + b)*NC + c)*ND + d] (bra/ket shells in pair orientation).
-template <int LA, int LB, int LC, int LD, int AC, int CC>
+template <int LA, int LB, int LC, int LD, int AC, int CC, int CN>
 __device__ __forceinline__ void eri_store_body(const StoreArgs& a) {
-    using S = Shape<LA, LB, LC, LD, AC, CC>;
+    using S = Shape<LA, LB, LC, LD, AC, CC, CN>;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     const int2 pq = a.quartets[i];
     double acc[S::NOUT];
-    quartet<double, LA, LB, LC, LD, AC, CC>(a.tb, a.pd, pq.x, pq.y, 0, a.pd.pn[pq.x], acc);
+    quartet<double, LA, LB, LC, LD, AC, CC, CN>(a.tb, a.pd, pq.x, pq.y, 0, a.pd.pn[pq.x], acc);
     double* out = a.out + a.off[i];
     int o = 0;
 #pragma unroll
@@ -549,11 +551,19 @@ __device__ __forceinline__ void eri_store_body(const StoreArgs& a) {
 using JKKernel = void (*)(JKArgs);
 using StoreKernel = void (*)(StoreArgs);
 
-// One kernel per class pair; the output chunk is blockIdx.y.
-struct ClassKernels {
-    int nchunk = 0;
+// The output chunks of a class are compiled into groups of a few chunks per
+// kernel (blockIdx.y selects the chunk inside the group): one kernel holding
+// every chunk would be register-allocated for all of them at once.
+constexpr int kMaxChunkGroups = 36;
+struct JKGroup {
     JKKernel jk = nullptr;    // FP64 recurrences
     JKKernel jk32 = nullptr;  // FP32 recurrences (mixed-precision route)
+    int c0 = 0, n = 0;        // chunks [c0, c0 + n)
+};
+struct ClassKernels {
+    int nchunk = 0;
+    int ngroup = 0;
+    JKGroup grp[kMaxChunkGroups];
     StoreKernel store = nullptr;
 };
 

@@ -1,14 +1,26 @@
 // Compile-time registration of the per-class ERI kernels.  Each class pair
-// (bra X >= ket Y) gets one J/K kernel and one store kernel whose output
-// chunk is blockIdx.y:
-//   full block in registers when it holds at most 81 values (one chunk),
-//   otherwise one chunk per bra component (AC), and when that is still above
-//   81 values, one per (bra component, ket component) pair (AC, CC).
+// (bra X >= ket Y) is split into output chunks of at most TLNO_CHUNK_MAXOUT
+// values:
+//   mode 0: the full block (one chunk);
+//   mode 1: one chunk per bra component a;
+//   mode 2: one chunk per bra component a and group of CN ket components c
+//           (the largest CN dividing ncart(LC) that fits the limit).
+// A chunk holds its outputs in registers for the whole primitive loop, so
+// fewer, larger chunks repeat less root and recurrence work at the price of
+// registers.  J/K kernels hold TLNO_CHUNK_GROUP chunks each
+// (chunk = c0 + blockIdx.y); the store kernel holds all chunks (blockIdx.y).
 #pragma once
 
 #include <utility>
 
 #include "eri_device.hpp"
+
+#ifndef TLNO_CHUNK_MAXOUT
+#define TLNO_CHUNK_MAXOUT 108
+#endif
+#ifndef TLNO_CHUNK_GROUP
+#define TLNO_CHUNK_GROUP 2
+#endif
 
 namespace tlno_amd {
 
@@ -18,40 +30,68 @@ template <int LA, int LB, int LC, int LD>
 struct ChunkPolicy {
     static constexpr int NA = ncart(LA), NC = ncart(LC);
     static constexpr int nc = ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD);
-    static constexpr int mode = nc <= 81 ? 0 : (nc / NA <= 81 ? 1 : 2);
-    static constexpr int nchunk = mode == 0 ? 1 : (mode == 1 ? NA : NA * NC);
-    static constexpr int ac(int i) { return mode == 0 ? -1 : (mode == 1 ? i : i / NC); }
-    static constexpr int cc(int i) { return mode == 2 ? i % NC : -1; }
+    static constexpr int per_c = nc / NA / NC;  // outputs per (a, c)
+    static constexpr int mode = nc <= TLNO_CHUNK_MAXOUT ? 0 : (nc / NA <= TLNO_CHUNK_MAXOUT ? 1 : 2);
+    static constexpr int pick_cn() {
+        for (int g = NC; g > 1; --g)
+            if (NC % g == 0 && per_c * g <= TLNO_CHUNK_MAXOUT) return g;
+        return 1;
+    }
+    static constexpr int cn = mode == 2 ? pick_cn() : 1;
+    static constexpr int ncg = NC / cn;  // ket component groups (mode 2)
+    static constexpr int nchunk = mode == 0 ? 1 : (mode == 1 ? NA : NA * ncg);
+    static constexpr int ac(int i) { return mode == 0 ? -1 : (mode == 1 ? i : i / ncg); }
+    static constexpr int cc(int i) { return mode == 2 ? (i % ncg) * cn : -1; }
 };
 
-template <typename R, int LA, int LB, int LC, int LD, int... I>
+template <typename R, int LA, int LB, int LC, int LD, int C0, int... J>
 __global__ __launch_bounds__(256) void k_eri_jk(JKArgs a) {
     using CP = ChunkPolicy<LA, LB, LC, LD>;
-    const int ch = blockIdx.y;
-    ((ch == I ? (eri_jk_body<R, LA, LB, LC, LD, CP::ac(I), CP::cc(I)>(a), 0) : 0), ...);
+    const int ch = C0 + static_cast<int>(blockIdx.y);
+    ((ch == C0 + J ? (eri_jk_body<R, LA, LB, LC, LD, CP::ac(C0 + J), CP::cc(C0 + J), CP::cn>(a), 0) : 0), ...);
 }
 
 template <int LA, int LB, int LC, int LD, int... I>
 __global__ __launch_bounds__(256) void k_eri_store(StoreArgs a) {
     using CP = ChunkPolicy<LA, LB, LC, LD>;
     const int ch = blockIdx.y;
-    ((ch == I ? (eri_store_body<LA, LB, LC, LD, CP::ac(I), CP::cc(I)>(a), 0) : 0), ...);
+    ((ch == I ? (eri_store_body<LA, LB, LC, LD, CP::ac(I), CP::cc(I), CP::cn>(a), 0) : 0), ...);
 }
 
 template <int LA, int LB, int LC, int LD, int... I>
-void reg_kernels(ClassKernels& k, std::integer_sequence<int, I...>) {
-    k.jk = k_eri_jk<double, LA, LB, LC, LD, I...>;
-    k.jk32 = k_eri_jk<float, LA, LB, LC, LD, I...>;
+void reg_store(ClassKernels& k, std::integer_sequence<int, I...>) {
     k.store = k_eri_store<LA, LB, LC, LD, I...>;
+}
+
+template <int LA, int LB, int LC, int LD, int C0, int... J>
+void reg_group(ClassKernels& k, std::integer_sequence<int, J...>) {
+    JKGroup& g = k.grp[k.ngroup++];
+    g.jk = k_eri_jk<double, LA, LB, LC, LD, C0, J...>;
+    g.jk32 = k_eri_jk<float, LA, LB, LC, LD, C0, J...>;
+    g.c0 = C0;
+    g.n = static_cast<int>(sizeof...(J));
+}
+
+template <int LA, int LB, int LC, int LD, int... G>
+void reg_groups(ClassKernels& k, std::integer_sequence<int, G...>) {
+    using CP = ChunkPolicy<LA, LB, LC, LD>;
+    constexpr int GS = TLNO_CHUNK_GROUP;
+    (reg_group<LA, LB, LC, LD, G * GS>(
+         k, std::make_integer_sequence<int, (CP::nchunk - G * GS < GS ? CP::nchunk - G * GS : GS)>{}),
+     ...);
 }
 
 template <int X, int Y>
 void reg_class(ClassKernels (&tab)[6][6]) {
     constexpr int LA = kClassL[X][0], LB = kClassL[X][1], LC = kClassL[Y][0], LD = kClassL[Y][1];
     using CP = ChunkPolicy<LA, LB, LC, LD>;
+    constexpr int ngroup = (CP::nchunk + TLNO_CHUNK_GROUP - 1) / TLNO_CHUNK_GROUP;
+    static_assert(ngroup <= kMaxChunkGroups, "too many chunk groups");
     ClassKernels& k = tab[X][Y];
     k.nchunk = CP::nchunk;
-    reg_kernels<LA, LB, LC, LD>(k, std::make_integer_sequence<int, CP::nchunk>{});
+    k.ngroup = 0;
+    reg_groups<LA, LB, LC, LD>(k, std::make_integer_sequence<int, ngroup>{});
+    reg_store<LA, LB, LC, LD>(k, std::make_integer_sequence<int, CP::nchunk>{});
 }
 
 }  // namespace tlno_amd
