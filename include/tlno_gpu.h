@@ -101,6 +101,13 @@ int tlno_gpu_build_jk_device(tlno_gpu_plan* plan, const double* d_density,
  * so no quartet the reference keeps is dropped.  0 turns it off (default). */
 int tlno_gpu_set_density_threshold(tlno_gpu_plan* plan, double density_threshold);
 
+/* Canonical ERI stream checksum: the sum of every unique pure-spherical
+ * (ab|cd) (a >= b, c >= d, pair(ab) >= pair(cd)) over shell quartets with
+ * Q_ab Q_cd >= schwarz, the one-element result of the reference's
+ * build_eri_packed_for_G_rys (twoe_integrals.hpp:108-113,
+ * rys_engine.ipp:17096-17103).  Spherical plans only. */
+int tlno_gpu_eri_checksum(tlno_gpu_plan* plan, double schwarz, double* checksum);
+
 /* Learned screen (no reference counterpart).  params: TLNO_GPU_MLP_NALL
  * floats of a network trained with tlno_gpu_mlp_* (NULL turns the screen
  * off).  Following builds evaluate the network on the matrix cores for every

@@ -1267,7 +1267,7 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
     HIP_OK(hipGetLastError());
     if (dJ) k_fill<<<grid_for(mc2, 256), 256, 0, st>>>(pl->dJacc, mc2, 0.0);
     if (dK) k_fill<<<grid_for(mc2, 256), 256, 0, st>>>(pl->dKacc, mc2, 0.0);
-    JKArgs a;
+    JKArgs a{};
     a.pd = pl->pair_data();
     a.tb = ctx.tb;
     a.D = pl->dDint;
@@ -1463,6 +1463,139 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
                 }
         ++pl->timed_builds;
     }
+}
+
+// ---- canonical ERI stream checksum ----
+// Sum of every unique spherical (ab|cd), a >= b, c >= d, (ab) >= (cd) (pair
+// index), over Schwarz-surviving shell quartets, the value the reference's
+// packed endpoint returns (rys_engine.ipp:270-300, 17096-17103).  Shell
+// quartets with P != Q contribute sum_abcd wP[ab] wQ[cd] (ab|cd)_cart with
+//   wP[a][b] = u[a] u[b], u[a] = sum_s C[s][a]          (A != B),
+//   wP[a][b] = sum_{s >= t} C[s][a] C[t][b]             (A == B),
+// evaluated inside the J/K kernels (checksum mode); the npair quartets (P|P)
+// are projected and summed on the host with the (ab) >= (cd) rule.
+double eri_checksum(tlno_gpu_plan* pl, double thr) {
+    if (!pl->spherical)
+        throw std::runtime_error("eri_ao_tensor_hgp: the ERI stream is pure spherical; build a spherical plan.");
+    if (!std::isfinite(thr) || thr < 0.0)
+        throw std::runtime_error("eri_ao_tensor_hgp: schwarz_threshold must be finite and >= 0.");
+    DeviceCtx& ctx = device_ctx(pl->device);
+    HIP_OK(hipSetDevice(pl->device));
+    const C2S& cs = c2s_table();
+    // weights per pair kind (2 * class + diag)
+    std::vector<double> cw(12 * 36, 0.0);
+    for (int c = 0; c < 6; ++c) {
+        const int LA = (c == 0 ? 0 : (c <= 2 ? 1 : 2)), LB = (c == 0 || c == 1 || c == 3) ? 0 : (c == 2 || c == 4 ? 1 : 2);
+        const int na = ncart(LA), nb = ncart(LB);
+        for (int a = 0; a < na; ++a)
+            for (int b = 0; b < nb; ++b) {
+                double ua = 0.0, ub = 0.0;
+                for (int t = 0; t < 2 * LA + 1; ++t) ua += cs.c[LA][t][a];
+                for (int t = 0; t < 2 * LB + 1; ++t) ub += cs.c[LB][t][b];
+                cw[(2 * c) * 36 + a * nb + b] = ua * ub;
+                if (LA == LB) {
+                    double w = 0.0;
+                    for (int s1 = 0; s1 < 2 * LA + 1; ++s1)
+                        for (int t1 = 0; t1 <= s1; ++t1) w += cs.c[LA][s1][a] * cs.c[LB][t1][b];
+                    cw[(2 * c + 1) * 36 + a * nb + b] = w;
+                }
+            }
+    }
+    double* dcw = nullptr;
+    double* dchk = nullptr;
+    HIP_OK(hipMalloc(&dcw, cw.size() * sizeof(double)));
+    HIP_OK(hipMalloc(&dchk, sizeof(double)));
+    HIP_OK(hipMemcpy(dcw, cw.data(), cw.size() * sizeof(double), hipMemcpyHostToDevice));
+    const hipStream_t st = pl->stream;
+    k_fill<<<1, 64, 0, st>>>(dchk, 1, 0.0);
+    const TaskSet& ts = get_tasks(pl, thr);
+    JKArgs a{};
+    a.pd = pl->pair_data();
+    a.tb = ctx.tb;
+    a.M = pl->Mcart;
+    a.split = 1;
+    a.cw = dcw;
+    a.chk = dchk;
+    for (int X = 5; X >= 0; --X)
+        for (int Y = X; Y >= 0; --Y) {
+            const long long nt = ts.cnt[X][Y];
+            if (nt == 0) continue;
+            a.tasks = ts.d_tasks + ts.off[X][Y];
+            a.ntasks = static_cast<int>(nt);
+            launch_jk(g_kernels[X][Y], false, a, nt, st);
+        }
+    HIP_OK(hipGetLastError());
+    double total = 0.0;
+    HIP_OK(hipMemcpyAsync(&total, dchk, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    HIP_OK(hipFree(dcw));
+    HIP_OK(hipFree(dchk));
+    // (P|P) quartets: full spherical blocks, (ab) >= (cd) and a >= b, c >= d
+    auto pair_index = [](long long i, long long j) {
+        if (i < j) std::swap(i, j);
+        return i * (i + 1) / 2 + j;
+    };
+    for (int X = 0; X < 6; ++X) {
+        const int b0 = pl->cls_begin[X], b1 = pl->cls_begin[X + 1];
+        std::vector<int2> qs;
+        std::vector<long long> off;
+        const int LA = (X == 0 ? 0 : (X <= 2 ? 1 : 2)), LB = (X == 0 || X == 1 || X == 3) ? 0 : (X == 2 || X == 4 ? 1 : 2);
+        const int na = ncart(LA), nb = ncart(LB), nab = na * nb;
+        for (int P = b0; P < b1; ++P)
+            if (pl->Q[P] * pl->Q[P] >= thr) {
+                qs.push_back(make_int2(P, P));
+                off.push_back(static_cast<long long>(qs.size() - 1) * nab * nab);
+            }
+        if (qs.empty()) continue;
+        double* dout = nullptr;
+        HIP_OK(hipMalloc(&dout, qs.size() * nab * nab * sizeof(double)));
+        run_store(pl, X, X, qs, off, dout);
+        std::vector<double> h(qs.size() * nab * nab);
+        HIP_OK(hipMemcpy(h.data(), dout, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+        HIP_OK(hipFree(dout));
+        const int sa = 2 * LA + 1, sb = 2 * LB + 1;
+        std::vector<double> half(static_cast<std::size_t>(sa) * sb * nab), sph(static_cast<std::size_t>(sa) * sb * sa * sb);
+        for (std::size_t k = 0; k < qs.size(); ++k) {
+            const int P = qs[k].x;
+            const GroupH& Ga = pl->groups[pl->pgA[P]];
+            const GroupH& Gb = pl->groups[pl->pgB[P]];
+            const double* blk = h.data() + k * nab * nab;
+            // bra projection, then ket projection
+            for (int s1 = 0; s1 < sa; ++s1)
+                for (int t1 = 0; t1 < sb; ++t1)
+                    for (int cd = 0; cd < nab; ++cd) {
+                        double v = 0.0;
+                        for (int a1 = 0; a1 < na; ++a1)
+                            for (int b1i = 0; b1i < nb; ++b1i)
+                                v += cs.c[LA][s1][a1] * cs.c[LB][t1][b1i] * blk[(a1 * nb + b1i) * nab + cd];
+                        half[(static_cast<std::size_t>(s1) * sb + t1) * nab + cd] = v;
+                    }
+            for (int st1 = 0; st1 < sa * sb; ++st1)
+                for (int s2 = 0; s2 < sa; ++s2)
+                    for (int t2 = 0; t2 < sb; ++t2) {
+                        double v = 0.0;
+                        for (int c1 = 0; c1 < na; ++c1)
+                            for (int d1 = 0; d1 < nb; ++d1)
+                                v += cs.c[LA][s2][c1] * cs.c[LB][t2][d1] * half[static_cast<std::size_t>(st1) * nab + c1 * nb + d1];
+                        sph[static_cast<std::size_t>(st1) * sa * sb + s2 * sb + t2] = v;
+                    }
+            const bool diag = pl->pgA[P] == pl->pgB[P];
+            for (int s1 = 0; s1 < sa; ++s1)
+                for (int t1 = 0; t1 < sb; ++t1) {
+                    const long long i1 = Ga.sph_base + s1, j1 = Gb.sph_base + t1;
+                    if (diag && j1 > i1) continue;
+                    const long long p1 = pair_index(i1, j1);
+                    for (int s2 = 0; s2 < sa; ++s2)
+                        for (int t2 = 0; t2 < sb; ++t2) {
+                            const long long i2 = Ga.sph_base + s2, j2 = Gb.sph_base + t2;
+                            if (diag && j2 > i2) continue;
+                            if (pair_index(i2, j2) > p1) continue;
+                            total += sph[(static_cast<std::size_t>(s1) * sb + t1) * sa * sb + s2 * sb + t2];
+                        }
+                }
+        }
+    }
+    return total;
 }
 
 void validate_density(const double* D, int M) {
@@ -1688,6 +1821,14 @@ int tlno_gpu_quartet_features(tlno_gpu_plan* pl, int n, const int* pair_p, const
         HIP_OK(hipFree(dp));
         HIP_OK(hipFree(dq));
         HIP_OK(hipFree(dout));
+    });
+}
+
+int tlno_gpu_eri_checksum(tlno_gpu_plan* pl, double schwarz, double* checksum) {
+    return guarded([&] {
+        if (!pl) throw std::runtime_error("eri_ao_tensor_hgp: null packed ERI plan.");
+        if (!checksum) throw std::runtime_error("tlno_gpu: null output");
+        *checksum = eri_checksum(pl, schwarz);
     });
 }
 

@@ -336,6 +336,11 @@ struct JKArgs {
     const int* cut;     // density screen: ket range of P ends at cut[6 P] (pointer offset by ket class; nullptr: off)
     const int* qlist;   // list mode: kets of a task are qlist[offset .. offset + count)
     unsigned long long* counter;  // [0] shell, [1] primitive quartets evaluated, [2], [3] the FP32 share
+    // checksum mode (cw != nullptr, no J/K): chk += sum over the block of
+    // cw[pair kind of P][a][b] * cw[pair kind of Q][c][d] * (ab|cd), for
+    // P != Q; pair kind = 2 * class + (1 if a diagonal pair)
+    const double* cw;   // [12][36]
+    double* chk;
 };
 
 struct StoreArgs {
@@ -412,6 +417,30 @@ __device__ __forceinline__ void eri_jk_body(const JKArgs& a) {
 
     R acc[S::NOUT];
     quartet<R, LA, LB, LC, LD, AC, CC, CN>(a.tb, a.pd, P, Q, ib0, ib1, acc);
+
+    if (a.cw) {
+        // canonical-stream checksum: the weights fold the spherical projection
+        // and the unique-index restriction of each pair into one factor
+        constexpr int XB = LA == 0 ? 0 : (LA == 1 ? (LB == 0 ? 1 : 2) : (LB == 0 ? 3 : (LB == 1 ? 4 : 5)));
+        constexpr int YK = LC == 0 ? 0 : (LC == 1 ? (LD == 0 ? 1 : 2) : (LD == 0 ? 3 : (LD == 1 ? 4 : 5)));
+        const double* wp = a.cw + (2 * XB + (a.pd.diag[P] ? 1 : 0)) * 36;
+        const double* wq = a.cw + (2 * YK + (a.pd.diag[Q] ? 1 : 0)) * 36;
+        double v = 0.0;
+        int o = 0;
+#pragma unroll
+        for (int ai = 0; ai < S::NAL; ++ai)
+#pragma unroll
+            for (int b = 0; b < S::NB; ++b)
+#pragma unroll
+                for (int ci = 0; ci < S::NCL; ++ci)
+#pragma unroll
+                    for (int d = 0; d < S::ND; ++d, ++o)
+                        v = fma(double(acc[o]) * wp[S::acomp(ai) * S::NB + b], wq[S::ccomp(ci) * S::ND + d], v);
+        if (!active || P == Q) v = 0.0;
+        v = wave_sum(v);
+        if (lane == 0) unsafeAtomicAdd(a.chk, v);
+        return;
+    }
 
     // degeneracy scale: 1/2 per diagonal pair, 1/2 if P == Q
     double s = 1.0;

@@ -40,6 +40,7 @@ SIGNATURES = {
     "tlno_gpu_boys": (C.c_int, [C.c_int, D, C.c_int, D]),
     "tlno_gpu_rys": (C.c_int, [C.c_int, C.c_int, D, D, D]),
     "tlno_gpu_onee": (C.c_int, [C.c_void_p, C.c_int, D, D, D, D, D]),
+    "tlno_gpu_eri_checksum": (C.c_int, [C.c_void_p, C.c_double, D]),
     "tlno_gpu_set_learned_screen": (C.c_int, [C.c_void_p, F, C.c_double]),
     "tlno_gpu_quartet_features": (C.c_int, [C.c_void_p, C.c_int, I, I, F]),
     "tlno_gpu_last_route_stats": (C.c_int, [C.c_void_p, LL]),

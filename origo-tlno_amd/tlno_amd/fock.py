@@ -155,6 +155,15 @@ class DirectPlan:
         FP32 recurrences (0 disables); J/K accumulation stays FP64."""
         _lib.check(_lib.load().tlno_gpu_set_mixed_precision(self._h, float(fp32_threshold)))
 
+    def eri_checksum(self, schwarz=None):
+        """Sum of the canonical unique spherical ERIs over Schwarz-surviving
+        shell quartets (the reference's packed-stream checksum)."""
+        thr = self.options.integral_threshold if schwarz is None else schwarz
+        _check_threshold("schwarz_threshold", thr)
+        v = C.c_double()
+        _lib.check(_lib.load().tlno_gpu_eri_checksum(self._h, float(thr), C.byref(v)))
+        return v.value
+
     def set_learned_screen(self, params=None, margin=0.0):
         """Turn the learned shell-quartet screen on (params: 6401 floats of a
         trained network, margin in decades) or off (params None)."""
@@ -301,6 +310,25 @@ def build_eri_packed_plan_for_G_rys(basis: Basis, options: EriPackedBuildOptions
 
 def adopt_eri_plan_for_transition(plan: DirectPlan) -> DirectPlan:
     return plan
+
+
+@dataclass
+class EriPackedGData:
+    """values = [checksum of the canonical spherical stream]
+    (twoe_integrals.hpp:37-39, rys_engine.ipp:17096-17103)."""
+    values: list = field(default_factory=list)
+
+
+def build_eri_packed_for_G_rys(plan: DirectPlan, options: EriPackedBuildOptions | None = None) -> EriPackedGData:
+    o = options or screening_to_eri_options(plan.options)
+    _check_threshold("schwarz_threshold", o.schwarz_threshold)
+    _check_threshold("decay_threshold", o.decay_threshold)
+    _check_threshold("primitive_fastschwarz_threshold", o.primitive_fastschwarz_threshold)
+    for name in ("primitive_fastschwarz_threshold", "decay_threshold"):
+        if getattr(o, name) != 0.0:
+            raise NotImplementedError(f"{name} screening is not available in the GPU engine")
+    # the stream ignores density_threshold: there is no density to bound against
+    return EriPackedGData([plan.eri_checksum(o.schwarz_threshold)])
 
 
 @dataclass

@@ -23,6 +23,9 @@ tests) and the reference's outputs:
                    configs.spherical_density(321, 5, 60): G = J - K/2 at
                    Schwarz 1e-12 in full, and 16 bilinear probes u_i^T X v_i
                    of J and K (python gen_golden.py organic30 rewrites only it)
+  checksum.npz     canonical spherical ERI stream checksums of organic30 at
+                   Schwarz 1e-12 and 1e-8 (python gen_golden.py checksum
+                   rewrites only it)
   density.npz      4-water cluster (96 spherical functions), density
                    configs.spherical_density(96, 9, 20): the reference's
                    density-screened J, K and G (density_threshold 1e-9, 1e-7,
@@ -60,6 +63,8 @@ def main(only=None):
         return organic30(ref)
     if only == "density":
         return density(ref)
+    if only == "checksum":
+        return checksums(ref)
     t0 = time.time()
     # ---------------------------------------------------------- Boys / Rys
     xs = np.concatenate([[0.0, 1e-10, 1e-3, 0.5, 5.0, 11.99, 12.0, 12.5, 25.0, 39.99, 40.0, 45.0, 80.0, 200.0],
@@ -122,7 +127,16 @@ def main(only=None):
 
     organic30(ref)
     density(ref)
+    checksums(ref)
     print(f"golden fixtures written in {time.time() - t0:.1f} s")
+
+
+def checksums(ref):
+    # ---------------------------------------------------------- ERI stream checksums
+    b30, _ = configs.organic30()
+    np.savez_compressed(os.path.join(HERE, "checksum.npz"),
+                        organic30_sha=basis_hash(b30), organic30_12=ref.checksum(b30, 1e-12),
+                        organic30_8=ref.checksum(b30, 1e-8))
 
 
 def density(ref):

@@ -119,3 +119,28 @@ def test_density_screen_vs_reference(tau, tag):
     assert np.abs(J - g[f"J_{tag}"]).max() < 1e-10
     assert np.abs(K - g[f"K_{tag}"]).max() < 1e-10
     assert np.abs(G - g[f"G_{tag}"]).max() < 1e-10
+
+
+def test_eri_stream_checksums_vs_reference():
+    """The reference's packed-stream checksum (sum of the unique spherical
+    ERIs): water7 and organic5 unscreened, organic30 at Schwarz 1e-12 and
+    1e-8.  Relative tolerance 1e-11 (a sum of ~1e8 terms in another order)."""
+    from tlno_amd.fock import build_eri_packed_for_G_rys, EriPackedBuildOptions, build_eri_packed_plan_for_G_rys
+    g7, g5, gc = load("water7.npz"), load("organic5.npz"), load("checksum.npz")
+    b7, _ = configs.water7()
+    p7 = build_eri_packed_plan_for_G_rys(b7, EriPackedBuildOptions(schwarz_threshold=0.0))
+    c7 = build_eri_packed_for_G_rys(p7, EriPackedBuildOptions(schwarz_threshold=0.0)).values[0]
+    b5, _ = configs.organic30(natoms=5)
+    p5 = DirectPlan(b5, ScreeningOptions(integral_threshold=0.0))
+    c5 = p5.eri_checksum(0.0)
+    b30, _ = configs.organic30()
+    p30 = DirectPlan(b30, ScreeningOptions(integral_threshold=1e-12))
+    c30 = {t: p30.eri_checksum(thr) for t, thr in (("12", 1e-12), ("8", 1e-8))}
+    print(f"water7 {c7!r} vs {float(g7['checksum'])!r}; organic5 {c5!r} vs {float(g5['checksum'])!r}; "
+          f"organic30 {c30} vs {float(gc['organic30_12'])!r} {float(gc['organic30_8'])!r}")
+    assert c7 == pytest.approx(float(g7["checksum"]), rel=1e-11)
+    assert c5 == pytest.approx(float(g5["checksum"]), rel=1e-11)
+    assert c30["12"] == pytest.approx(float(gc["organic30_12"]), rel=1e-11)
+    assert c30["8"] == pytest.approx(float(gc["organic30_8"]), rel=1e-11)
+    with pytest.raises(Exception, match="pure spherical"):
+        DirectPlan(b7, cartesian=True).eri_checksum(0.0)
