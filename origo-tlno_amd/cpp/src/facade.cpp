@@ -1,0 +1,526 @@
+// C++ facade over the C ABI (include/tlno_gpu.h): the reference's one- and
+// two-electron engine API (namespaces tlno::core, tlno::integrals::twoe,
+// tlno::integrals::twoe::fock, tlno::integrals::onee) implemented on the GPU
+// engine.  Errors raise std::runtime_error with the engine's message text.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "core/basis/types.hpp"
+#include "core/math/general_math.hpp"
+#include "integrals/one_electron/onee_matrices.hpp"
+#include "integrals/two_electron/fock/eri_direct_jk.hpp"
+#include "integrals/two_electron/fock/jk_direct.hpp"
+#include "integrals/two_electron/twoe_integrals.hpp"
+#include "tlno_gpu.h"
+
+namespace {
+
+using tlno::core::basis::Shell;
+
+void check(int rc) {
+    if (rc != 0) throw std::runtime_error(tlno_gpu_last_error());
+}
+
+bool finite_nonneg(double v) { return std::isfinite(v) && v >= 0.0; }
+
+// A GPU plan plus the lock that serializes builds on it (a plan owns one set
+// of device work buffers).
+struct GpuPlan {
+    tlno_gpu_plan* h = nullptr;
+    int M = 0, n_groups = 0, n_pairs = 0;
+    long long n_prim_pairs = 0;
+    std::mutex mu;
+    ~GpuPlan() {
+        if (h) tlno_gpu_plan_destroy(h);
+    }
+};
+
+std::shared_ptr<GpuPlan> make_gpu_plan(const std::vector<Shell>& basis, int spherical) {
+    const int n = static_cast<int>(basis.size());
+    std::vector<double> centers(3 * static_cast<std::size_t>(n)), exps, coefs;
+    std::vector<int> ang(3 * static_cast<std::size_t>(n)), nprim(n);
+    for (int i = 0; i < n; ++i) {
+        for (int x = 0; x < 3; ++x) {
+            centers[3 * i + x] = basis[i].center[x];
+            ang[3 * i + x] = basis[i].ang[x];
+        }
+        nprim[i] = static_cast<int>(basis[i].primitives.size());
+        for (const auto& p : basis[i].primitives) {
+            exps.push_back(p.exponent);
+            coefs.push_back(p.coefficient);
+        }
+    }
+    check(tlno_gpu_init(0));
+    auto g = std::make_shared<GpuPlan>();
+    check(tlno_gpu_plan_create(n, centers.data(), ang.data(), nprim.data(), exps.data(), coefs.data(), spherical,
+                               &g->h));
+    check(tlno_gpu_plan_info(g->h, &g->M, &g->n_groups, &g->n_pairs, &g->n_prim_pairs));
+    return g;
+}
+
+void require_gpu_screens(double fastschwarz, double decay) {
+    if (!finite_nonneg(fastschwarz))
+        throw std::runtime_error("eri_ao_tensor_hgp: primitive_fastschwarz_threshold must be finite and >= 0.");
+    if (!finite_nonneg(decay)) throw std::runtime_error("eri_ao_tensor_hgp: decay_threshold must be finite and >= 0.");
+    if (fastschwarz != 0.0 || decay != 0.0)
+        throw std::runtime_error(
+            "eri_ao_tensor_hgp: primitive FASTSCHWARZ and decay screening are not available in the GPU engine.");
+}
+
+// J and/or K with Schwarz and density thresholds on a locked plan
+void build_jk(GpuPlan& g, const std::vector<double>& density, double schwarz, double density_threshold, double* J,
+              double* K) {
+    if (density.size() != static_cast<std::size_t>(g.M) * g.M)
+        throw std::runtime_error("eri_ao_tensor_hgp: density must be spherical M x M row-major.");
+    if (!finite_nonneg(density_threshold))
+        throw std::runtime_error("eri_ao_tensor_hgp: density_threshold must be finite and >= 0.");
+    std::lock_guard<std::mutex> lk(g.mu);
+    check(tlno_gpu_set_density_threshold(g.h, density_threshold));
+    check(tlno_gpu_build_jk(g.h, density.data(), J, K, schwarz));
+}
+
+tlno::integrals::twoe::EriPackedPlanProfile profile_of(const GpuPlan& g) {
+    tlno::integrals::twoe::EriPackedPlanProfile p;
+    p.M = g.M;
+    p.shell_groups = static_cast<std::uint64_t>(g.n_groups);
+    p.pair_groups = static_cast<std::uint64_t>(g.n_pairs);
+    p.pair_group_quartets = p.pair_groups * (p.pair_groups + 1) / 2;
+    p.packed_pair_count = static_cast<std::uint64_t>(g.M) * (g.M + 1) / 2;
+    p.packed_value_count = p.packed_pair_count * (p.packed_pair_count + 1) / 2;
+    p.primitive_pairs = static_cast<std::uint64_t>(g.n_prim_pairs);
+    return p;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ Boys --
+namespace tlno::core::math {
+
+double boys_single(int order, double x) {
+    if (order < 0 || order > 16) throw std::runtime_error("boys_single: order must be in [0, 16].");
+    std::vector<double> F(static_cast<std::size_t>(order) + 1);
+    check(tlno_gpu_boys(1, &x, order, F.data()));
+    return F[order];
+}
+
+void boys_r000_all(int max_n, double x, double p, double* out) {
+    if (max_n < 0 || max_n > 16) throw std::runtime_error("boys_r000_all: max_n must be in [0, 16].");
+    std::vector<double> F(static_cast<std::size_t>(max_n) + 1);
+    check(tlno_gpu_boys(1, &x, max_n, F.data()));
+    double s = 1.0;
+    for (int n = 0; n <= max_n; ++n, s *= -2.0 * p) out[n] = s * F[n];
+}
+
+template <int W>
+void boys_r000_all_batch(int max_n, const double* xs, double p, double* out) {
+    boys_r000_all_batch_masked<W>(max_n, xs, p, out, (1u << W) - 1u);
+}
+
+template <int W>
+void boys_r000_all_batch_masked(int max_n, const double* xs, double p, double* out, unsigned active_mask) {
+    static_assert(W == 1 || W == 2 || W == 4 || W == 8, "W must be 1, 2, 4 or 8");
+    if (max_n < 0 || max_n > 16) throw std::runtime_error("boys_r000_all: max_n must be in [0, 16].");
+    std::vector<double> F(static_cast<std::size_t>(W) * (max_n + 1));
+    check(tlno_gpu_boys(W, xs, max_n, F.data()));
+    for (int lane = 0; lane < W; ++lane) {
+        const bool on = (active_mask >> lane) & 1u;
+        double s = 1.0;
+        for (int n = 0; n <= max_n; ++n, s *= -2.0 * p) out[n * W + lane] = on ? s * F[lane * (max_n + 1) + n] : 0.0;
+    }
+}
+
+void boys_r000_all_x4(int max_n, const double* x4, double p, double* out) {
+    boys_r000_all_batch<4>(max_n, x4, p, out);
+}
+
+template void boys_r000_all_batch<1>(int, const double*, double, double*);
+template void boys_r000_all_batch<2>(int, const double*, double, double*);
+template void boys_r000_all_batch<4>(int, const double*, double, double*);
+template void boys_r000_all_batch<8>(int, const double*, double, double*);
+template void boys_r000_all_batch_masked<1>(int, const double*, double, double*, unsigned);
+template void boys_r000_all_batch_masked<2>(int, const double*, double, double*, unsigned);
+template void boys_r000_all_batch_masked<4>(int, const double*, double, double*, unsigned);
+template void boys_r000_all_batch_masked<8>(int, const double*, double, double*, unsigned);
+
+}  // namespace tlno::core::math
+
+// -------------------------------------------------------- twoe (packed) --
+namespace tlno::integrals::twoe {
+
+struct EriPackedPlanBackend {
+    std::shared_ptr<GpuPlan> gpu;
+};
+
+struct EriPackedPlanBackendAccess {
+    static const std::shared_ptr<const EriPackedPlanBackend>& get(const EriPackedPlan& p) { return p.backend_; }
+    static void set(EriPackedPlan& p, std::shared_ptr<const EriPackedPlanBackend> b) { p.backend_ = std::move(b); }
+};
+
+namespace {
+GpuPlan& gpu_of(const EriPackedPlan& plan) {
+    const auto& b = EriPackedPlanBackendAccess::get(plan);
+    if (!b || !b->gpu) throw std::runtime_error("eri_ao_tensor_hgp: null packed ERI plan.");
+    return *b->gpu;
+}
+
+void check_schwarz(double v) {
+    if (!finite_nonneg(v)) throw std::runtime_error("eri_ao_tensor_hgp: schwarz_threshold must be finite and >= 0.");
+}
+}  // namespace
+
+EriPackedPlan build_eri_packed_plan_for_G_rys(const std::vector<tlno::core::basis::Shell>& basis,
+                                              const EriPackedBuildOptions& options) {
+    check_schwarz(options.schwarz_threshold);
+    auto b = std::make_shared<EriPackedPlanBackend>();
+    b->gpu = make_gpu_plan(basis, 1);
+    EriPackedPlan plan;
+    plan.profile = profile_of(*b->gpu);
+    EriPackedPlanBackendAccess::set(plan, std::move(b));
+    return plan;
+}
+
+EriPackedGData build_eri_packed_for_G_rys(const EriPackedPlan& plan, const EriPackedBuildOptions& options) {
+    check_schwarz(options.schwarz_threshold);
+    require_gpu_screens(options.primitive_fastschwarz_threshold, options.decay_threshold);
+    GpuPlan& g = gpu_of(plan);
+    std::lock_guard<std::mutex> lk(g.mu);
+    double v = 0.0;
+    check(tlno_gpu_eri_checksum(g.h, options.schwarz_threshold, &v));
+    return EriPackedGData{{v}};
+}
+
+EriDirectJKResult build_jk_direct_for_G_rys(const EriPackedPlan& plan, const std::vector<double>& density,
+                                            const EriPackedBuildOptions& options) {
+    check_schwarz(options.schwarz_threshold);
+    require_gpu_screens(options.primitive_fastschwarz_threshold, options.decay_threshold);
+    GpuPlan& g = gpu_of(plan);
+    const std::size_t n = static_cast<std::size_t>(g.M) * g.M;
+    std::vector<double> J(n), K(n);
+    build_jk(g, density, options.schwarz_threshold, options.density_threshold, J.data(), K.data());
+    EriDirectJKResult r;
+    r.M = g.M;
+    r.g.resize(n);
+    for (std::size_t i = 0; i < n; ++i) r.g[i] = J[i] - 0.5 * K[i];
+    return r;
+}
+
+// ---------------------------------------------------- incremental state --
+EriDirectJKIncrementalState::EriDirectJKIncrementalState(const EriPackedPlan& plan,
+                                                         const EriDirectJKIncrementalOptions& options)
+    : plan_(plan), options_(options) {
+    M_ = gpu_of(plan_).M;
+    if (options_.exact_rebuild_interval == 0u)
+        throw std::runtime_error("eri_ao_tensor_hgp: incremental exact_rebuild_interval must be >= 1.");
+    const EriPackedBuildOptions& b = options_.base_options;
+    if (!finite_nonneg(b.schwarz_threshold))
+        throw std::runtime_error("eri_ao_tensor_hgp: incremental schwarz_threshold must be finite and >= 0.");
+    if (!finite_nonneg(b.primitive_fastschwarz_threshold))
+        throw std::runtime_error(
+            "eri_ao_tensor_hgp: incremental primitive_fastschwarz_threshold must be finite and >= 0.");
+    if (!finite_nonneg(b.decay_threshold))
+        throw std::runtime_error("eri_ao_tensor_hgp: incremental decay_threshold must be finite and >= 0.");
+    if (!finite_nonneg(b.density_threshold))
+        throw std::runtime_error("eri_ao_tensor_hgp: incremental base density_threshold must be finite and >= 0.");
+    if (b.schwarz_threshold != 0.0 && b.decay_threshold != 0.0)
+        throw std::runtime_error(
+            "eri_ao_tensor_hgp: incremental base options cannot combine Schwarz and decay screening.");
+    if (!finite_nonneg(options_.delta_density_threshold))
+        throw std::runtime_error("eri_ao_tensor_hgp: incremental delta_density_threshold must be finite and >= 0.");
+    if (!finite_nonneg(options_.first_delta_density_threshold))
+        throw std::runtime_error(
+            "eri_ao_tensor_hgp: incremental first_delta_density_threshold must be finite and >= 0.");
+    if ((options_.delta_density_threshold > 0.0 ||
+         (options_.use_first_delta_density_threshold && options_.first_delta_density_threshold > 0.0)) &&
+        b.decay_threshold > 0.0)
+        throw std::runtime_error(
+            "eri_ao_tensor_hgp: incremental delta-density screening cannot be combined with the heuristic decay "
+            "metric.");
+    // exact rebuilds never screen with the (full) density
+    options_.base_options.density_threshold = 0.0;
+}
+
+EriDirectJKUpdateStatus EriDirectJKIncrementalState::update(const std::vector<double>& density,
+                                                            bool force_exact_rebuild) {
+    const std::size_t n = static_cast<std::size_t>(M_) * M_;
+    if (density.size() != n)
+        throw std::runtime_error("eri_ao_tensor_hgp: incremental density must be spherical M x M row-major.");
+    std::vector<double> delta(initialized_ ? n : 0);
+    double max_abs = 0.0;
+    bool unchanged = initialized_;
+    for (std::size_t i = 0; i < n; ++i) {
+        if (!std::isfinite(density[i]))
+            throw std::runtime_error("eri_ao_tensor_hgp: incremental density contains a non-finite value.");
+        const double d = initialized_ ? density[i] - density_[i] : density[i];
+        if (!std::isfinite(d)) throw std::runtime_error("eri_ao_tensor_hgp: incremental density delta overflowed.");
+        if (initialized_) {
+            delta[i] = d;
+            unchanged = unchanged && d == 0.0;
+        }
+        max_abs = std::max(max_abs, std::abs(d));
+    }
+    if (unchanged && !force_exact_rebuild) {
+        ++counters_.updates;
+        ++counters_.skipped_updates;
+        last_status_ = EriDirectJKUpdateStatus::skipped_unchanged;
+        last_max_abs_density_delta_ = 0.0;
+        last_effective_density_threshold_ = 0.0;
+        return last_status_;
+    }
+    const bool exact = !initialized_ || force_exact_rebuild ||
+                       updates_since_exact_rebuild_ + 1u >= options_.exact_rebuild_interval;
+    EriPackedBuildOptions bo = options_.base_options;
+    double eff = 0.0;
+    if (!exact) {
+        eff = (options_.use_first_delta_density_threshold && updates_since_exact_rebuild_ == 0u)
+                  ? options_.first_delta_density_threshold
+                  : options_.delta_density_threshold;
+        bo.density_threshold = eff;
+    }
+    std::vector<double> next_density(density);
+    EriDirectJKResult built = build_jk_direct_for_G_rys(plan_, exact ? density : delta, bo);
+    if (built.M != M_ || built.g.size() != n)
+        throw std::runtime_error("eri_ao_tensor_hgp: incremental direct J/K builder returned an inconsistent shape.");
+    std::vector<double> next_g;
+    if (exact) {
+        next_g = std::move(built.g);
+        for (double v : next_g)
+            if (!std::isfinite(v))
+                throw std::runtime_error("eri_ao_tensor_hgp: exact direct J/K rebuild produced a non-finite value.");
+    } else {
+        next_g = g_;
+        for (std::size_t i = 0; i < n; ++i) {
+            if (!std::isfinite(built.g[i]))
+                throw std::runtime_error("eri_ao_tensor_hgp: incremental dG build produced a non-finite value.");
+            next_g[i] += built.g[i];
+            if (!std::isfinite(next_g[i]))
+                throw std::runtime_error("eri_ao_tensor_hgp: incremental cached G overflowed.");
+        }
+    }
+    density_.swap(next_density);
+    g_.swap(next_g);
+    initialized_ = true;
+    ++counters_.updates;
+    last_max_abs_density_delta_ = max_abs;
+    last_effective_density_threshold_ = eff;
+    if (exact) {
+        ++counters_.exact_rebuilds;
+        updates_since_exact_rebuild_ = 0;
+        last_status_ = EriDirectJKUpdateStatus::exact_rebuild;
+    } else {
+        ++counters_.incremental_updates;
+        ++updates_since_exact_rebuild_;
+        last_status_ = EriDirectJKUpdateStatus::incremental_update;
+    }
+    return last_status_;
+}
+
+void EriDirectJKIncrementalState::reset() noexcept {
+    density_.clear();
+    g_.clear();
+    counters_ = {};
+    last_status_ = EriDirectJKUpdateStatus::uninitialized;
+    last_max_abs_density_delta_ = 0.0;
+    last_effective_density_threshold_ = 0.0;
+    updates_since_exact_rebuild_ = 0;
+    initialized_ = false;
+}
+
+}  // namespace tlno::integrals::twoe
+
+// ------------------------------------------------------------ twoe::fock --
+namespace tlno::integrals::twoe::fock {
+
+struct DirectPlanBackend {
+    std::shared_ptr<GpuPlan> gpu;
+};
+
+struct DirectPlanBackendAccess {
+    static const std::shared_ptr<const DirectPlanBackend>& get(const DirectPlan& p) { return p.backend_; }
+    static void set(DirectPlan& p, std::shared_ptr<const DirectPlanBackend> b) { p.backend_ = std::move(b); }
+};
+
+namespace {
+GpuPlan& gpu_of(const DirectPlan& plan) {
+    const auto& b = DirectPlanBackendAccess::get(plan);
+    if (!b || !b->gpu) throw std::runtime_error("jk_direct: null J/K plan backend.");
+    return *b->gpu;
+}
+
+void validate(const ScreeningOptions& o) {
+    if (!finite_nonneg(o.integral_threshold))
+        throw std::runtime_error("eri_ao_tensor_hgp: schwarz_threshold must be finite and >= 0.");
+    if (!finite_nonneg(o.density_threshold))
+        throw std::runtime_error("eri_ao_tensor_hgp: density_threshold must be finite and >= 0.");
+    if (o.density_threshold > 0.0 && o.decay_threshold > 0.0)
+        throw std::runtime_error(
+            "eri_ao_tensor_hgp: density screening requires Schwarz bounds and cannot be combined with the heuristic "
+            "decay metric.");
+    require_gpu_screens(o.primitive_fastschwarz_threshold, o.decay_threshold);
+}
+}  // namespace
+
+ScreeningOptions screening_from_eri_options(const tlno::integrals::twoe::EriPackedBuildOptions& o) noexcept {
+    return ScreeningOptions{o.schwarz_threshold, o.primitive_fastschwarz_threshold, o.decay_threshold,
+                            o.density_threshold};
+}
+
+tlno::integrals::twoe::EriPackedBuildOptions screening_to_eri_options(const ScreeningOptions& o) noexcept {
+    tlno::integrals::twoe::EriPackedBuildOptions e;
+    e.schwarz_threshold = o.integral_threshold;
+    e.primitive_fastschwarz_threshold = o.primitive_fastschwarz_threshold;
+    e.decay_threshold = o.decay_threshold;
+    e.density_threshold = o.density_threshold;
+    return e;
+}
+
+DirectPlan build_direct_plan(const std::vector<tlno::core::basis::Shell>& basis, const ScreeningOptions& options) {
+    validate(options);
+    auto b = std::make_shared<DirectPlanBackend>();
+    b->gpu = make_gpu_plan(basis, 1);
+    DirectPlan plan;
+    plan.profile = profile_of(*b->gpu);
+    DirectPlanBackendAccess::set(plan, std::move(b));
+    return plan;
+}
+
+DirectPlan adopt_eri_plan_for_transition(const tlno::integrals::twoe::EriPackedPlan& eri) {
+    const auto& eb = tlno::integrals::twoe::EriPackedPlanBackendAccess::get(eri);
+    if (!eb || !eb->gpu) throw std::runtime_error("eri_ao_tensor_hgp: null packed ERI plan.");
+    auto b = std::make_shared<DirectPlanBackend>();
+    b->gpu = eb->gpu;  // shares the immutable GPU plan
+    DirectPlan plan;
+    plan.profile = eri.profile;
+    DirectPlanBackendAccess::set(plan, std::move(b));
+    return plan;
+}
+
+CoulombJResult build_coulomb_j(const DirectPlan& plan, const std::vector<double>& density,
+                               const ScreeningOptions& options) {
+    validate(options);
+    GpuPlan& g = gpu_of(plan);
+    CoulombJResult r;
+    r.M = g.M;
+    r.j.resize(static_cast<std::size_t>(g.M) * g.M);
+    build_jk(g, density, options.integral_threshold, options.density_threshold, r.j.data(), nullptr);
+    return r;
+}
+
+ExchangeKResult build_exchange_k(const DirectPlan& plan, const std::vector<double>& density,
+                                 const ScreeningOptions& options) {
+    validate(options);
+    GpuPlan& g = gpu_of(plan);
+    ExchangeKResult r;
+    r.M = g.M;
+    r.k.resize(static_cast<std::size_t>(g.M) * g.M);
+    build_jk(g, density, options.integral_threshold, options.density_threshold, nullptr, r.k.data());
+    return r;
+}
+
+DirectResult build_rhf_g(const DirectPlan& plan, const std::vector<double>& density, const ScreeningOptions& options) {
+    validate(options);
+    GpuPlan& g = gpu_of(plan);
+    const std::size_t n = static_cast<std::size_t>(g.M) * g.M;
+    std::vector<double> J(n), K(n);
+    build_jk(g, density, options.integral_threshold, options.density_threshold, J.data(), K.data());
+    DirectResult r;
+    r.M = g.M;
+    r.g.resize(n);
+    for (std::size_t i = 0; i < n; ++i) r.g[i] = J[i] - 0.5 * K[i];
+    return r;
+}
+
+}  // namespace tlno::integrals::twoe::fock
+
+// ------------------------------------------------------------------ onee --
+namespace tlno::integrals::onee {
+
+struct OneElectronPlanBackend {
+    std::vector<basis::Shell> shells;
+    mutable std::mutex mu;
+    mutable std::shared_ptr<GpuPlan> cart, sph;
+    GpuPlan& get(bool spherical) const {
+        std::lock_guard<std::mutex> lk(mu);
+        auto& p = spherical ? sph : cart;
+        if (!p) p = make_gpu_plan(shells, spherical ? 1 : 0);
+        return *p;
+    }
+};
+
+namespace {
+bool g_screening = true;
+double g_decay_eps = 1e-15;
+
+OneElectronPlan make_plan(const basis::BasisBuildResult& build, bool spherical) {
+    auto b = std::make_shared<OneElectronPlanBackend>();
+    b->shells = build.basis;
+    OneElectronPlan plan;
+    plan.n_cart = static_cast<int>(build.basis.size());
+    plan.n_sph = b->get(true).M;
+    if (!spherical) (void)b->get(false);
+    plan.backend = std::move(b);
+    return plan;
+}
+
+void evaluate(const basis::BasisBuildResult& build, const OneElectronPlan& plan, bool spherical,
+              OneElectronMatrices& mats) {
+    if (!plan.backend) throw std::runtime_error("onee: null one-electron plan.");
+    if (build.nuclear_charges.size() != build.nuclear_centers.size())
+        throw std::runtime_error("onee: nuclear charges and centres differ in length.");
+    GpuPlan& g = plan.backend->get(spherical);
+    const int n = g.M;
+    std::vector<double> Z(build.nuclear_charges.begin(), build.nuclear_charges.end()), R;
+    for (const auto& c : build.nuclear_centers) R.insert(R.end(), c.begin(), c.end());
+    mats = OneElectronMatrices(n);
+    std::lock_guard<std::mutex> lk(g.mu);
+    check(tlno_gpu_onee(g.h, static_cast<int>(Z.size()), Z.data(), R.data(), mats.overlap.values.data(),
+                        mats.kinetic.values.data(), mats.nuclear.values.data()));
+}
+}  // namespace
+
+OneElectronPlan build_one_electron_plan(const basis::BasisBuildResult& build) { return make_plan(build, false); }
+OneElectronPlan build_one_electron_plan(const basis::BasisBuildResult& build, bool) { return make_plan(build, false); }
+OneElectronPlan build_one_electron_spherical_plan(const basis::BasisBuildResult& build, bool) {
+    return make_plan(build, true);
+}
+
+OneElectronMatrices evaluate_one_electron_matrices(const basis::BasisBuildResult& build, const OneElectronPlan& plan) {
+    OneElectronMatrices m;
+    evaluate(build, plan, false, m);
+    return m;
+}
+
+void evaluate_one_electron_matrices_into(const basis::BasisBuildResult& build, const OneElectronPlan& plan,
+                                         OneElectronMatrices& mats) {
+    evaluate(build, plan, false, mats);
+}
+
+void evaluate_one_electron_matrices_sph_into(const basis::BasisBuildResult& build, const OneElectronPlan& plan,
+                                             OneElectronMatrices& mats) {
+    evaluate(build, plan, true, mats);
+}
+
+OneElectronMatrices build_one_electron_matrices(const basis::BasisBuildResult& build) {
+    return evaluate_one_electron_matrices(build, build_one_electron_plan(build));
+}
+
+OneElectronMatrices build_one_electron_spherical_matrices(const basis::BasisBuildResult& build) {
+    OneElectronMatrices m;
+    evaluate(build, build_one_electron_spherical_plan(build), true, m);
+    return m;
+}
+
+void set_onee_screening_enabled(bool enabled) {
+    g_screening = enabled;
+    g_decay_eps = 1e-15;
+}
+bool onee_screening_enabled() { return g_screening; }
+void set_onee_group_pair_decay_eps(double eps) { g_decay_eps = eps; }
+double onee_group_pair_decay_eps() { return g_decay_eps; }
+
+}  // namespace tlno::integrals::onee

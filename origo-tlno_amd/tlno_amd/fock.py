@@ -341,3 +341,122 @@ def build_jk_direct_for_G_rys(plan: DirectPlan, density, options: EriPackedBuild
     o = screening_from_eri_options(options or EriPackedBuildOptions())
     r = build_rhf_g(plan, density, o)
     return EriDirectJKResult(r.g, r.M)
+
+
+# --- incremental SCF state (eri_direct_jk.hpp:41-135, direct_build.ipp:562-775) ---
+import enum
+
+
+class EriDirectJKUpdateStatus(enum.IntEnum):
+    uninitialized = 0
+    exact_rebuild = 1
+    incremental_update = 2
+    skipped_unchanged = 3
+
+
+@dataclass
+class EriDirectJKIncrementalOptions:
+    base_options: EriPackedBuildOptions = field(default_factory=EriPackedBuildOptions)
+    exact_rebuild_interval: int = 8
+    delta_density_threshold: float = 0.0
+    use_first_delta_density_threshold: bool = False
+    first_delta_density_threshold: float = 0.0
+
+
+@dataclass
+class EriDirectJKIncrementalCounters:
+    updates: int = 0
+    exact_rebuilds: int = 0
+    incremental_updates: int = 0
+    skipped_updates: int = 0
+
+
+class EriDirectJKIncrementalState:
+    """G for SCF iterations: exact rebuilds from the full density (first call,
+    forced, or every exact_rebuild_interval changed densities); in between
+    G += G(dP), dP optionally density-screened.  A failed update leaves the
+    state unchanged."""
+
+    def __init__(self, plan: DirectPlan, options: EriDirectJKIncrementalOptions | None = None):
+        import copy
+        self.plan = plan
+        self.options = copy.deepcopy(options or EriDirectJKIncrementalOptions())
+        o = self.options
+        if int(o.exact_rebuild_interval) < 1:
+            raise RuntimeError("eri_ao_tensor_hgp: incremental exact_rebuild_interval must be >= 1.")
+        b = o.base_options
+        for name in ("schwarz_threshold", "primitive_fastschwarz_threshold", "decay_threshold"):
+            v = getattr(b, name)
+            if not (math.isfinite(v) and v >= 0.0):
+                raise RuntimeError(f"eri_ao_tensor_hgp: incremental {name} must be finite and >= 0.")
+        if not (math.isfinite(b.density_threshold) and b.density_threshold >= 0.0):
+            raise RuntimeError("eri_ao_tensor_hgp: incremental base density_threshold must be finite and >= 0.")
+        if b.schwarz_threshold != 0.0 and b.decay_threshold != 0.0:
+            raise RuntimeError("eri_ao_tensor_hgp: incremental base options cannot combine Schwarz and decay "
+                               "screening.")
+        for name in ("delta_density_threshold", "first_delta_density_threshold"):
+            v = getattr(o, name)
+            if not (math.isfinite(v) and v >= 0.0):
+                raise RuntimeError(f"eri_ao_tensor_hgp: incremental {name} must be finite and >= 0.")
+        b.density_threshold = 0.0  # exact rebuilds never screen with the full density
+        self.M = plan.M
+        self.reset()
+
+    def reset(self):
+        self.density = None
+        self.g = None
+        self.counters = EriDirectJKIncrementalCounters()
+        self.last_status = EriDirectJKUpdateStatus.uninitialized
+        self.last_max_abs_density_delta = 0.0
+        self.last_effective_density_threshold = 0.0
+        self.updates_since_exact_rebuild = 0
+
+    @property
+    def initialized(self):
+        return self.density is not None
+
+    def update(self, density, force_exact_rebuild=False) -> EriDirectJKUpdateStatus:
+        D = np.asarray(density, dtype=np.float64)
+        if D.shape != (self.M, self.M):
+            raise RuntimeError("eri_ao_tensor_hgp: incremental density must be spherical M x M row-major.")
+        if not np.all(np.isfinite(D)):
+            raise RuntimeError("eri_ao_tensor_hgp: incremental density contains a non-finite value.")
+        delta = D - self.density if self.initialized else D
+        if not np.all(np.isfinite(delta)):
+            raise RuntimeError("eri_ao_tensor_hgp: incremental density delta overflowed.")
+        max_abs = float(np.abs(delta).max()) if delta.size else 0.0
+        if self.initialized and max_abs == 0.0 and not force_exact_rebuild:
+            self.counters.updates += 1
+            self.counters.skipped_updates += 1
+            self.last_status = EriDirectJKUpdateStatus.skipped_unchanged
+            self.last_max_abs_density_delta = 0.0
+            self.last_effective_density_threshold = 0.0
+            return self.last_status
+        o = self.options
+        exact = (not self.initialized or force_exact_rebuild
+                 or self.updates_since_exact_rebuild + 1 >= int(o.exact_rebuild_interval))
+        import copy
+        bo = copy.copy(o.base_options)
+        eff = 0.0
+        if not exact:
+            eff = (o.first_delta_density_threshold
+                   if o.use_first_delta_density_threshold and self.updates_since_exact_rebuild == 0
+                   else o.delta_density_threshold)
+            bo.density_threshold = eff
+        built = build_jk_direct_for_G_rys(self.plan, D if exact else delta, bo).g
+        if not np.all(np.isfinite(built)):
+            raise RuntimeError("eri_ao_tensor_hgp: direct J/K build produced a non-finite value.")
+        g = built if exact else self.g + built
+        self.density, self.g = D.copy(), g
+        self.counters.updates += 1
+        self.last_max_abs_density_delta = max_abs
+        self.last_effective_density_threshold = eff
+        if exact:
+            self.counters.exact_rebuilds += 1
+            self.updates_since_exact_rebuild = 0
+            self.last_status = EriDirectJKUpdateStatus.exact_rebuild
+        else:
+            self.counters.incremental_updates += 1
+            self.updates_since_exact_rebuild += 1
+            self.last_status = EriDirectJKUpdateStatus.incremental_update
+        return self.last_status

@@ -1,0 +1,79 @@
+"""GPU: the C++ facade (the reference's tlno::integrals / tlno::core API over
+the C ABI), driven by tests/cpp/facade_driver as a reference caller would use
+it: J, K, G, the packed-stream checksum, the incremental SCF state, S/T/V and
+the Boys seeds, against the golden fixtures and the Python mirror.
+Tolerances: 1e-10 for J/K/G and S/T/V against the reference, 1e-12 between
+the incremental and the exact G and against the Python mirror (same
+kernels; atomic accumulation order varies between runs)."""
+import os
+
+import numpy as np
+import pytest
+
+import cpp_facade_io as fio
+from oracle import oracle as O
+from tlno_amd import configs
+from tlno_amd.fock import DirectPlan, ScreeningOptions
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _densities(M, seed):
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((M, M)) * 1e-2
+    return A + A.T
+
+
+@pytest.mark.parametrize("name", ["water7", "organic5"])
+def test_facade_matches_reference_and_mirror(name, tmp_path):
+    if name == "water7":
+        b, D0 = configs.water7()
+    else:
+        b, _ = configs.organic30(natoms=5)
+        D0 = None
+    plan = DirectPlan(b, ScreeningOptions(integral_threshold=1e-12))
+    M = plan.M
+    if D0 is None:
+        D0 = np.load(os.path.join(GOLD, "organic5.npz"))["Dsph"]
+    D1 = D0 + _densities(M, 3)
+    r, res = fio.run(b, D0, D1, str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    J, K = plan.jk(D0)
+    # same kernels; the floating-point atomics add in a run-dependent order
+    assert np.abs(res["J"] - J.ravel()).max() < 1e-12 and np.abs(res["K"] - K.ravel()).max() < 1e-12
+    G0 = (J - 0.5 * K).ravel()
+    assert np.abs(res["G"] - G0).max() < 1e-12 and np.abs(res["G_twoe"] - G0).max() < 1e-12
+    J1, K1 = plan.jk(D1)
+    G1 = (J1 - 0.5 * K1).ravel()
+    assert np.abs(res["G_adopted"] - G1).max() < 1e-12
+    # reference fixtures
+    g = np.load(os.path.join(GOLD, f"{name}.npz"))
+    gref = g["G_12"] if name == "water7" else g["G_0"]
+    assert np.abs(res["G"] - gref.ravel()).max() < 1e-10
+    assert res["checksum"][0] == pytest.approx(plan.eri_checksum(1e-12), rel=1e-13)
+    assert res["checksum"][0] == pytest.approx(float(g["checksum"]), rel=1e-9)
+    # incremental state: exact, incremental, skipped, incremental, periodic exact, forced exact
+    assert res["status"].tolist() == [1, 2, 3, 2, 1, 1]
+    assert res["counters"].tolist() == [6, 3, 2, 1]
+    assert np.abs(res["G_incremental"] - G1).max() < 1e-12
+    assert np.abs(res["G_final"] - G1).max() < 1e-12
+    assert "schwarz_threshold must be finite" in res["error"]
+    # one-electron matrices
+    n = len(b.nprim)
+    assert np.abs(res["S_cart"] - g["S0"].ravel()).max() < 1e-10
+    # T against the Python mirror: the reference's one-sided kinetic formula
+    # carries ~1e-7 cancellation error for tight d primitives (organic5), the
+    # engine evaluates the symmetric form, checked against the oracle elsewhere
+    from tlno_amd import onee
+    T_mirror = onee.build_one_electron_matrices(b).kinetic
+    assert np.abs(res["T_cart"] - T_mirror.ravel()).max() < 1e-12
+    assert np.abs(res["T_cart"] - g["T0"].ravel()).max() < (1e-10 if name == "water7" else 1e-6)
+    assert np.abs(res["V_cart"] - g["V0"].ravel()).max() < 1e-10
+    assert np.abs(res["S_sph"] - g["S1"].ravel()).max() < 1e-10
+    assert np.abs(res["V_sph"] - g["V1"].ravel()).max() < 1e-10
+    assert res["S_cart"].size == n * n
+    # Boys seeds (-2p)^n F_n(x), F_3(17.25)
+    F = [O.boys(m, 2.5) for m in range(5)]
+    want = [(-1.4) ** m * F[m] for m in range(5)] + [O.boys(3, 17.25)]
+    np.testing.assert_allclose(res["boys"], want, rtol=1e-13)

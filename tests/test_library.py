@@ -111,3 +111,35 @@ def test_workload_sizes():
     w, Dw = configs.water40()
     assert w.n == 1000
     assert Basis.nsph(w) == 960  # O 3s2p1d = 14, H 2s1p = 5 spherical functions
+
+
+def test_cpp_facade_exports_reference_api():
+    """The C++ facade library defines the reference's API (demangled names)."""
+    import subprocess
+    so = os.path.join(ROOT, "origo-tlno_amd", "tlno_amd", "libtlno_amd_cpp.so")
+    out = subprocess.run(["nm", "-DC", "--defined-only", so], capture_output=True, text=True, check=True).stdout
+    for sym in ("tlno::integrals::twoe::fock::build_rhf_g(", "tlno::integrals::twoe::fock::build_coulomb_j(",
+                "tlno::integrals::twoe::fock::build_exchange_k(", "tlno::integrals::twoe::fock::build_direct_plan(",
+                "tlno::integrals::twoe::fock::adopt_eri_plan_for_transition(",
+                "tlno::integrals::twoe::build_eri_packed_plan_for_G_rys(",
+                "tlno::integrals::twoe::build_eri_packed_for_G_rys(",
+                "tlno::integrals::twoe::build_jk_direct_for_G_rys(",
+                "tlno::integrals::twoe::EriDirectJKIncrementalState::update(",
+                "tlno::integrals::twoe::EriDirectJKIncrementalState::reset(",
+                "tlno::integrals::onee::build_one_electron_matrices(",
+                "tlno::integrals::onee::build_one_electron_spherical_matrices(",
+                "tlno::core::math::boys_single(", "tlno::core::math::boys_r000_all("):
+        assert sym in out, sym
+
+
+def test_cpp_facade_fails_loudly_without_a_device(tmp_path):
+    """A reference-style caller gets the engine's error, not a silent result,
+    when no HIP device is usable (this container has none)."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    import cpp_facade_io as fio
+    b, D = configs.water7()
+    r, res = fio.run(b, D, D, str(tmp_path))
+    assert r.returncode == 1 and not res
+    assert "tlno_gpu" in r.stderr or "hip" in r.stderr.lower()
