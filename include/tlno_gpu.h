@@ -180,6 +180,38 @@ int tlno_gpu_rys(int N, int n, const double* T, double* roots, double* weights);
 int tlno_gpu_onee(tlno_gpu_plan* plan, int n_nuc, const double* Z, const double* R,
                   double* S, double* T, double* V);
 
+/* ---- molecule batches (no reference counterpart: the reference builds one
+ * molecule per call).  n_mol molecules in the Cartesian component basis,
+ * their component arrays concatenated in molecule order (n_comp[m]
+ * components each, same per-component layout as tlno_gpu_plan_create).
+ * D, J and K of molecule m are M_m x M_m blocks packed at offsets[m] of one
+ * array of `total` doubles (tlno_gpu_batch_info).  Every class kernel runs
+ * once for all molecules, persistent with the launch shape of its class
+ * (block size, waves per CU) chosen by tlno_gpu_batch_autotune or set by
+ * tlno_gpu_batch_set_shape (default: 256 threads, one wave per task).
+ * use_graph = 1 captures the whole build into a HIP graph on the first call
+ * for a given (density, J, K, schwarz) and replays it afterwards. */
+typedef struct tlno_gpu_batch tlno_gpu_batch;
+int tlno_gpu_batch_create(int n_mol, const int* n_comp, const double* centers, const int* ang,
+                          const int* nprim, const double* exps, const double* coefs,
+                          tlno_gpu_batch** out);
+void tlno_gpu_batch_destroy(tlno_gpu_batch* batch);
+int tlno_gpu_batch_info(const tlno_gpu_batch* batch, int* n_mol, long long* total, int* M,
+                        long long* offsets);
+int tlno_gpu_batch_build_jk(tlno_gpu_batch* batch, const double* density, double* J, double* K,
+                            double schwarz);
+int tlno_gpu_batch_build_jk_device(tlno_gpu_batch* batch, const double* d_density, double* d_J,
+                                   double* d_K, double schwarz, int use_graph);
+/* Times every candidate launch shape per class pair (repeats launches each)
+ * and keeps the fastest; report receives a JSON summary (may be NULL). */
+int tlno_gpu_batch_autotune(tlno_gpu_batch* batch, double schwarz, int repeats, char* report,
+                            int report_len);
+int tlno_gpu_batch_set_shape(tlno_gpu_batch* batch, int bra_class, int ket_class, int block,
+                             int waves_per_cu);
+/* Shell and primitive quartets and device milliseconds of the last build. */
+int tlno_gpu_batch_last_stats(const tlno_gpu_batch* batch, long long* shell_quartets,
+                              long long* prim_quartets, double* ms);
+
 /* ---- learned-screen network: 32 -> 64 -> 64 -> 1, ReLU, bf16 weights,
  * FP32 sums, trained on the GPU (no reference counterpart).  Parameters are
  * TLNO_GPU_MLP_NALL floats: W1[64][32], b1[64], W2[64][64], b2[64], w3[64],

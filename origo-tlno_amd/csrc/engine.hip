@@ -246,6 +246,10 @@ struct tlno_gpu_plan {
     std::vector<double> Q;
     long long nprim_pairs = 0;
     std::vector<int> pn_host;
+    // host mirror of the pair data in plan order
+    std::vector<double> h_A, h_AB, h_pw;
+    std::vector<int> h_ao, h_poff, h_diag;
+    std::vector<double4> h_pP;
     // device pair data
     double *dA = nullptr, *dAB = nullptr, *dpw = nullptr;
     int *dao = nullptr, *dpoff = nullptr, *dpn = nullptr, *ddiag = nullptr;
@@ -815,6 +819,14 @@ void upload_pairs(tlno_gpu_plan* pl, const std::vector<int>& order,
     pl->pgB = gB;
     pl->pn_host = pn;
     pl->nprim_pairs = static_cast<long long>(pP.size());
+    // host mirror (molecule batches concatenate the pair data of their plans)
+    pl->h_A = A;
+    pl->h_AB = AB;
+    pl->h_ao = ao;
+    pl->h_poff = poff;
+    pl->h_diag = diag;
+    pl->h_pP = pP;
+    pl->h_pw = pw;
     dev_upload(pl->dA, A);
     dev_upload(pl->dAB, AB);
     dev_upload(pl->dao, ao);
@@ -2130,6 +2142,533 @@ int tlno_gpu_rys(int N, int n, const double* T, double* roots, double* weights) 
         HIP_OK(hipFree(dT));
         HIP_OK(hipFree(dR));
         HIP_OK(hipFree(dW));
+    });
+}
+
+}  // extern "C"
+
+// ============================================================ molecule batches ===
+// Many molecules share one set of class launches: their pair data are
+// concatenated (class-major, then molecule, then descending Schwarz bound),
+// every task pairs a bra with kets of its own molecule, and the D / J / K
+// blocks of molecule m are packed at offset mbase[m] with row stride M_m
+// (Cartesian component basis).  Class kernels run persistent (a fixed grid of
+// waves striding over the class's tasks) with per-class launch shapes from
+// the autotuner; a build can be captured once into a HIP graph and replayed.
+namespace tlno_amd {
+
+// element idx of the packed arrays -> molecule (mbase has nmol + 1 entries)
+__device__ inline int batch_mol(const long long* mbase, int nmol, long long idx) {
+    int lo = 0, hi = nmol;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (mbase[mid] <= idx) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_prep_batch(const double* __restrict__ Din, double* __restrict__ Dint, const int* __restrict__ perm,
+                             const int* __restrict__ pbase, const long long* __restrict__ mbase,
+                             const int* __restrict__ mM, int nmol, long long total) {
+    const long long idx = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const int m = batch_mol(mbase, nmol, idx);
+    const long long base = mbase[m];
+    const int M = mM[m];
+    const long long loc = idx - base;
+    const int i = static_cast<int>(loc / M), j = static_cast<int>(loc % M);
+    const int* pm = perm + pbase[m];
+    Dint[idx] = Din[base + static_cast<long long>(pm[i]) * M + pm[j]];
+}
+
+__global__ void k_fin_batch(const double* __restrict__ Xacc, double* __restrict__ Xout, const int* __restrict__ perm,
+                            const int* __restrict__ pbase, const long long* __restrict__ mbase,
+                            const int* __restrict__ mM, int nmol, long long total) {
+    const long long idx = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const int m = batch_mol(mbase, nmol, idx);
+    const long long base = mbase[m];
+    const int M = mM[m];
+    const long long loc = idx - base;
+    const int i = static_cast<int>(loc / M), j = static_cast<int>(loc % M);
+    const int* pm = perm + pbase[m];
+    Xout[base + static_cast<long long>(pm[i]) * M + pm[j]] = Xacc[idx] + Xacc[base + static_cast<long long>(j) * M + i];
+}
+
+// launch shape of one class: block size and persistent waves per CU (0: one
+// wave per work item)
+struct LaunchShape {
+    int block = 256;
+    int waves_per_cu = 0;
+};
+
+}  // namespace tlno_amd
+
+struct tlno_gpu_batch {
+    int device = 0;
+    int nmol = 0;
+    std::vector<std::unique_ptr<tlno_gpu_plan>> mols;
+    std::vector<long long> mbase;  // nmol + 1
+    std::vector<int> mM, pbase;
+    long long total = 0;
+    int npair = 0;
+    int cls_begin[7] = {0};
+    std::vector<int> pmol, pn;
+    std::vector<double> Q;
+    std::vector<int> mcls[7];  // mcls[c][m]: first pair of molecule m in class c (size nmol + 1)
+    // device data
+    double *dA = nullptr, *dAB = nullptr, *dpw = nullptr;
+    int *dao = nullptr, *dpoff = nullptr, *dpn = nullptr, *ddiag = nullptr, *dpmol = nullptr;
+    double4* dpP = nullptr;
+    long long* dmbase = nullptr;
+    int *dmM = nullptr, *dperm = nullptr, *dpbase = nullptr;
+    double *dDint = nullptr, *dJacc = nullptr, *dKacc = nullptr;
+    unsigned long long* dcounter = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int n_cu = 256;
+    // work lists per Schwarz threshold
+    struct Tasks {
+        int4* d = nullptr;
+        long long off[6][6] = {}, cnt[6][6] = {};
+        long long shell_quartets = 0, prim_quartets = 0;
+    };
+    std::map<double, Tasks> tasks;
+    tlno_amd::LaunchShape shape[6][6];
+    // captured build
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    const void* graph_key[3] = {nullptr, nullptr, nullptr};
+    double graph_thr = -1.0;
+    long long last_sq = 0, last_pq = 0;
+    double last_ms = 0.0;
+
+    tlno_amd::PairData pair_data() const {
+        tlno_amd::PairData p;
+        p.A = dA;
+        p.AB = dAB;
+        p.ao = dao;
+        p.poff = dpoff;
+        p.pn = dpn;
+        p.pP = dpP;
+        p.pw = dpw;
+        p.diag = ddiag;
+        return p;
+    }
+    void drop_graph() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+        graph_key[0] = graph_key[1] = graph_key[2] = nullptr;
+        graph_thr = -1.0;
+    }
+    ~tlno_gpu_batch() {
+        drop_graph();
+        for (void* p : {(void*)dA, (void*)dAB, (void*)dpw, (void*)dao, (void*)dpoff, (void*)dpn, (void*)ddiag,
+                        (void*)dpmol, (void*)dpP, (void*)dmbase, (void*)dmM, (void*)dperm, (void*)dpbase,
+                        (void*)dDint, (void*)dJacc, (void*)dKacc, (void*)dcounter})
+            if (p) (void)hipFree(p);
+        for (auto& kv : tasks)
+            if (kv.second.d) (void)hipFree(kv.second.d);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace tlno_amd {
+
+void batch_build_tables(tlno_gpu_batch* bt) {
+    const int nm = bt->nmol;
+    // class-major, then molecule: pair order of the batch
+    std::vector<double> A, AB, pw;
+    std::vector<int> ao, poff, pn, diag, pmol, perm;
+    std::vector<double4> pP;
+    for (int c = 0; c < 6; ++c) {
+        bt->cls_begin[c] = static_cast<int>(pn.size());
+        bt->mcls[c].assign(nm + 1, 0);
+        for (int m = 0; m < nm; ++m) {
+            const tlno_gpu_plan& pl = *bt->mols[m];
+            bt->mcls[c][m] = static_cast<int>(pn.size());
+            for (int P = pl.cls_begin[c]; P < pl.cls_begin[c + 1]; ++P) {
+                for (int x = 0; x < 3; ++x) {
+                    A.push_back(pl.h_A[3 * P + x]);
+                    AB.push_back(pl.h_AB[3 * P + x]);
+                }
+                ao.push_back(pl.h_ao[2 * P]);
+                ao.push_back(pl.h_ao[2 * P + 1]);
+                poff.push_back(static_cast<int>(pP.size()));
+                pn.push_back(pl.pn_host[P]);
+                diag.push_back(pl.h_diag[P]);
+                pmol.push_back(m);
+                bt->Q.push_back(pl.Q[P]);
+                for (int k = 0; k < pl.pn_host[P]; ++k) {
+                    pP.push_back(pl.h_pP[pl.h_poff[P] + k]);
+                    pw.push_back(pl.h_pw[pl.h_poff[P] + k]);
+                }
+            }
+        }
+        bt->mcls[c][nm] = static_cast<int>(pn.size());
+    }
+    bt->cls_begin[6] = static_cast<int>(pn.size());
+    bt->npair = static_cast<int>(pn.size());
+    bt->pmol = pmol;
+    bt->pn = pn;
+    bt->mbase.assign(nm + 1, 0);
+    bt->mM.assign(nm, 0);
+    bt->pbase.assign(nm, 0);
+    for (int m = 0; m < nm; ++m) {
+        const tlno_gpu_plan& pl = *bt->mols[m];
+        bt->mM[m] = pl.Mcart;
+        bt->mbase[m + 1] = bt->mbase[m] + static_cast<long long>(pl.Mcart) * pl.Mcart;
+        bt->pbase[m] = static_cast<int>(perm.size());
+        perm.insert(perm.end(), pl.int2basis.begin(), pl.int2basis.end());
+    }
+    bt->total = bt->mbase[nm];
+    dev_upload(bt->dA, A);
+    dev_upload(bt->dAB, AB);
+    dev_upload(bt->dpw, pw);
+    dev_upload(bt->dao, ao);
+    dev_upload(bt->dpoff, poff);
+    dev_upload(bt->dpn, pn);
+    dev_upload(bt->ddiag, diag);
+    dev_upload(bt->dpmol, pmol);
+    dev_upload(bt->dpP, pP);
+    dev_upload(bt->dmbase, bt->mbase);
+    dev_upload(bt->dmM, bt->mM);
+    dev_upload(bt->dperm, perm);
+    dev_upload(bt->dpbase, bt->pbase);
+    const std::size_t tot = static_cast<std::size_t>(std::max(1ll, bt->total));
+    HIP_OK(hipMalloc(&bt->dDint, tot * sizeof(double)));
+    HIP_OK(hipMalloc(&bt->dJacc, tot * sizeof(double)));
+    HIP_OK(hipMalloc(&bt->dKacc, tot * sizeof(double)));
+    HIP_OK(hipMalloc(&bt->dcounter, 4 * sizeof(unsigned long long)));
+}
+
+tlno_gpu_batch::Tasks& batch_tasks(tlno_gpu_batch* bt, double thr) {
+    auto it = bt->tasks.find(thr);
+    if (it != bt->tasks.end()) return it->second;
+    tlno_gpu_batch::Tasks ts;
+    std::vector<int4> all;
+    std::vector<long long> pref(bt->npair + 1, 0);
+    for (int i = 0; i < bt->npair; ++i) pref[i + 1] = pref[i] + bt->pn[i];
+    for (int X = 0; X < 6; ++X)
+        for (int Y = 0; Y <= X; ++Y) {
+            ts.off[X][Y] = static_cast<long long>(all.size());
+            std::vector<int4> base;
+            for (int m = 0; m < bt->nmol; ++m) {
+                const int k0 = bt->mcls[Y][m], k1 = bt->mcls[Y][m + 1];
+                if (k1 <= k0) continue;
+                for (int P = bt->mcls[X][m]; P < bt->mcls[X][m + 1]; ++P) {
+                    int end = k1;
+                    if (thr > 0.0) {
+                        int lo = k0, hi = k1;
+                        while (lo < hi) {
+                            const int mid = (lo + hi) / 2;
+                            if (bt->Q[P] * bt->Q[mid] >= thr) lo = mid + 1;
+                            else hi = mid;
+                        }
+                        end = lo;
+                    }
+                    if (X == Y) end = std::min(end, P + 1);
+                    for (int q0 = k0; q0 < end; q0 += 64) base.push_back(make_int4(P, q0, end, 0));
+                    if (end > k0) {
+                        ts.shell_quartets += end - k0;
+                        ts.prim_quartets += (pref[end] - pref[k0]) * bt->pn[P];
+                    }
+                }
+            }
+            const long long waves = static_cast<long long>(base.size()) * g_kernels[X][Y].nchunk;
+            constexpr long long kTargetWaves = 16384;
+            int split = 1;
+            if (waves > 0 && waves < kTargetWaves)
+                split = static_cast<int>(std::min<long long>(256, (kTargetWaves + waves - 1) / waves));
+            for (const int4& t : base) {
+                const int pbn = bt->pn[t.x];
+                const int s = std::max(1, std::min(split, pbn));
+                const int step = (pbn + s - 1) / s;
+                for (int i0 = 0; i0 < pbn; i0 += step)
+                    all.push_back(make_int4(t.x, t.y, t.z, i0 | (std::min(pbn, i0 + step) << 16)));
+            }
+            ts.cnt[X][Y] = static_cast<long long>(all.size()) - ts.off[X][Y];
+        }
+    if (!all.empty()) {
+        HIP_OK(hipMalloc(&ts.d, all.size() * sizeof(int4)));
+        HIP_OK(hipMemcpy(ts.d, all.data(), all.size() * sizeof(int4), hipMemcpyHostToDevice));
+    }
+    return bt->tasks.emplace(thr, ts).first->second;
+}
+
+JKArgs batch_args(tlno_gpu_batch* bt, bool do_j, bool do_k) {
+    DeviceCtx& ctx = device_ctx(bt->device);
+    JKArgs a{};
+    a.pd = bt->pair_data();
+    a.tb = ctx.tb;
+    a.D = bt->dDint;
+    a.J = bt->dJacc;
+    a.K = bt->dKacc;
+    a.M = 0;
+    a.do_j = do_j;
+    a.do_k = do_k;
+    a.split = 1;
+    a.counter = bt->dcounter;
+    a.pmol = bt->dpmol;
+    a.mbase = bt->dmbase;
+    a.mM = bt->dmM;
+    return a;
+}
+
+// one class's kernels with a given launch shape (range-mode tasks)
+void batch_launch_class(tlno_gpu_batch* bt, const tlno_gpu_batch::Tasks& ts, int X, int Y, const LaunchShape& sh,
+                        JKArgs a, hipStream_t st) {
+    const long long nt = ts.cnt[X][Y];
+    if (nt == 0) return;
+    a.tasks = ts.d + ts.off[X][Y];
+    a.ntasks = static_cast<int>(nt);
+    long long waves = nt;
+    if (sh.waves_per_cu > 0) {
+        const long long grid_waves = std::min<long long>(nt, static_cast<long long>(bt->n_cu) * sh.waves_per_cu);
+        a.stride = static_cast<int>(grid_waves);
+        a.nwork = static_cast<int>(nt);
+        waves = grid_waves;
+    } else {
+        a.stride = 0;
+        a.nwork = 0;
+    }
+    const ClassKernels& k = g_kernels[X][Y];
+    void* params[] = {&a};
+    const int block = sh.block;
+    const unsigned grid = grid_for(waves * 64, block);
+    for (int g = 0; g < k.ngroup; ++g)
+        HIP_OK(hipLaunchKernel(reinterpret_cast<const void*>(k.grp[g].jk), dim3(grid, k.grp[g].n), dim3(block), params,
+                               0, st));
+}
+
+// the whole build on the batch stream (capturable: kernels only)
+void batch_enqueue(tlno_gpu_batch* bt, const double* dD, double* dJ, double* dK, double thr) {
+    const tlno_gpu_batch::Tasks& ts = batch_tasks(bt, thr);
+    const hipStream_t st = bt->stream;
+    const unsigned g = grid_for(bt->total, 256);
+    k_prep_batch<<<g, 256, 0, st>>>(dD, bt->dDint, bt->dperm, bt->dpbase, bt->dmbase, bt->dmM, bt->nmol, bt->total);
+    if (dJ) k_fill<<<g, 256, 0, st>>>(bt->dJacc, bt->total, 0.0);
+    if (dK) k_fill<<<g, 256, 0, st>>>(bt->dKacc, bt->total, 0.0);
+    k_fill<<<1, 64, 0, st>>>(reinterpret_cast<double*>(bt->dcounter), 4, 0.0);
+    const JKArgs a = batch_args(bt, dJ != nullptr, dK != nullptr);
+    for (int X = 5; X >= 0; --X)
+        for (int Y = X; Y >= 0; --Y) batch_launch_class(bt, ts, X, Y, bt->shape[X][Y], a, st);
+    if (dJ) k_fin_batch<<<g, 256, 0, st>>>(bt->dJacc, dJ, bt->dperm, bt->dpbase, bt->dmbase, bt->dmM, bt->nmol, bt->total);
+    if (dK) k_fin_batch<<<g, 256, 0, st>>>(bt->dKacc, dK, bt->dperm, bt->dpbase, bt->dmbase, bt->dmM, bt->nmol, bt->total);
+    HIP_OK(hipGetLastError());
+}
+
+void batch_build(tlno_gpu_batch* bt, const double* dD, double* dJ, double* dK, double thr, bool use_graph) {
+    if (!std::isfinite(thr) || thr < 0.0)
+        throw std::runtime_error("eri_ao_tensor_hgp: schwarz_threshold must be finite and >= 0.");
+    HIP_OK(hipSetDevice(bt->device));
+    const tlno_gpu_batch::Tasks& ts = batch_tasks(bt, thr);  // host work stays outside any capture
+    HIP_OK(hipEventRecord(bt->ev0, bt->stream));
+    if (use_graph) {
+        const bool same = bt->graph_exec && bt->graph_key[0] == dD && bt->graph_key[1] == dJ &&
+                          bt->graph_key[2] == dK && bt->graph_thr == thr;
+        if (!same) {
+            bt->drop_graph();
+            HIP_OK(hipStreamBeginCapture(bt->stream, hipStreamCaptureModeThreadLocal));
+            try {
+                batch_enqueue(bt, dD, dJ, dK, thr);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(bt->stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            HIP_OK(hipStreamEndCapture(bt->stream, &bt->graph));
+            HIP_OK(hipGraphInstantiate(&bt->graph_exec, bt->graph, nullptr, nullptr, 0));
+            bt->graph_key[0] = dD;
+            bt->graph_key[1] = dJ;
+            bt->graph_key[2] = dK;
+            bt->graph_thr = thr;
+        }
+        HIP_OK(hipGraphLaunch(bt->graph_exec, bt->stream));
+    } else {
+        batch_enqueue(bt, dD, dJ, dK, thr);
+    }
+    HIP_OK(hipEventRecord(bt->ev1, bt->stream));
+    HIP_OK(hipStreamSynchronize(bt->stream));
+    float ms = 0.0f;
+    HIP_OK(hipEventElapsedTime(&ms, bt->ev0, bt->ev1));
+    bt->last_ms = ms;
+    unsigned long long c[4] = {0, 0, 0, 0};
+    HIP_OK(hipMemcpy(c, bt->dcounter, sizeof(c), hipMemcpyDeviceToHost));
+    bt->last_sq = static_cast<long long>(c[0]);
+    bt->last_pq = static_cast<long long>(c[1]);
+    (void)ts;
+}
+
+// Per class: time every candidate launch shape on the class's own kernels
+// (J and K digestion on scratch accumulators, the densities left as they
+// are) and keep the fastest.  Candidates: block 64 / 128 / 256 threads x one
+// wave per work item or 4 / 8 / 16 / 32 persistent waves per CU.
+std::string batch_autotune(tlno_gpu_batch* bt, double thr, int repeats) {
+    HIP_OK(hipSetDevice(bt->device));
+    const tlno_gpu_batch::Tasks& ts = batch_tasks(bt, thr);
+    const hipStream_t st = bt->stream;
+    const unsigned g = grid_for(bt->total, 256);
+    k_fill<<<g, 256, 0, st>>>(bt->dDint, bt->total, 1e-3);  // a smooth density for timing
+    const JKArgs a = batch_args(bt, true, true);
+    std::string rep = "{";
+    bool first = true;
+    for (int X = 5; X >= 0; --X)
+        for (int Y = X; Y >= 0; --Y) {
+            if (ts.cnt[X][Y] == 0) continue;
+            LaunchShape best;
+            double best_ms = 1e300, default_ms = 0.0;
+            for (int block : {64, 128, 256})
+                for (int wpc : {0, 4, 8, 16, 32}) {
+                    const LaunchShape sh{block, wpc};
+                    batch_launch_class(bt, ts, X, Y, sh, a, st);  // warm-up
+                    HIP_OK(hipEventRecord(bt->ev0, st));
+                    for (int r = 0; r < repeats; ++r) batch_launch_class(bt, ts, X, Y, sh, a, st);
+                    HIP_OK(hipEventRecord(bt->ev1, st));
+                    HIP_OK(hipEventSynchronize(bt->ev1));
+                    float ms = 0.0f;
+                    HIP_OK(hipEventElapsedTime(&ms, bt->ev0, bt->ev1));
+                    const double per = ms / repeats;
+                    if (block == 256 && wpc == 0) default_ms = per;
+                    if (per < best_ms) {
+                        best_ms = per;
+                        best = sh;
+                    }
+                }
+            bt->shape[X][Y] = best;
+            static const char* names[6] = {"ss", "ps", "pp", "ds", "dp", "dd"};
+            char buf[256];
+            std::snprintf(buf, sizeof(buf), "%s\"(%s|%s)\": {\"block\": %d, \"waves_per_cu\": %d, \"ms\": %.4f, "
+                          "\"default_ms\": %.4f}", first ? "" : ", ", names[X], names[Y], best.block,
+                          best.waves_per_cu, best_ms, default_ms);
+            rep += buf;
+            first = false;
+        }
+    rep += "}";
+    bt->drop_graph();  // captured launches used the previous shapes
+    return rep;
+}
+
+}  // namespace tlno_amd
+
+extern "C" {
+
+int tlno_gpu_batch_create(int n_mol, const int* n_comp, const double* centers, const int* ang, const int* nprim,
+                          const double* exps, const double* coefs, tlno_gpu_batch** out) {
+    return guarded([&] {
+        if (!out) throw std::runtime_error("tlno_gpu: null output handle");
+        *out = nullptr;
+        if (n_mol <= 0 || !n_comp || !centers || !ang || !nprim || !exps || !coefs)
+            throw std::runtime_error("eri_ao_tensor_hgp: empty or null batch.");
+        auto bt = std::make_unique<tlno_gpu_batch>();
+        bt->device = g_device;
+        bt->nmol = n_mol;
+        long long ci = 0, pi = 0;
+        for (int m = 0; m < n_mol; ++m) {
+            if (n_comp[m] <= 0) throw std::runtime_error("eri_ao_tensor_hgp: empty or null basis.");
+            auto pl = std::make_unique<tlno_gpu_plan>();
+            pl->device = g_device;
+            pl->spherical = 0;
+            long long np = 0;
+            for (int i = 0; i < n_comp[m]; ++i) np += nprim[ci + i];
+            build_plan(pl.get(), n_comp[m], centers + 3 * ci, ang + 3 * ci, nprim + ci, exps + pi, coefs + pi);
+            ci += n_comp[m];
+            pi += np;
+            bt->mols.push_back(std::move(pl));
+        }
+        HIP_OK(hipSetDevice(bt->device));
+        HIP_OK(hipStreamCreateWithFlags(&bt->stream, hipStreamNonBlocking));
+        HIP_OK(hipEventCreate(&bt->ev0));
+        HIP_OK(hipEventCreate(&bt->ev1));
+        hipDeviceProp_t prop;
+        HIP_OK(hipGetDeviceProperties(&prop, bt->device));
+        bt->n_cu = std::max(1, prop.multiProcessorCount);
+        batch_build_tables(bt.get());
+        *out = bt.release();
+    });
+}
+
+void tlno_gpu_batch_destroy(tlno_gpu_batch* batch) { delete batch; }
+
+int tlno_gpu_batch_info(const tlno_gpu_batch* bt, int* n_mol, long long* total, int* M, long long* offsets) {
+    return guarded([&] {
+        if (!bt) throw std::runtime_error("tlno_gpu: null batch");
+        if (n_mol) *n_mol = bt->nmol;
+        if (total) *total = bt->total;
+        for (int m = 0; m < bt->nmol; ++m) {
+            if (M) M[m] = bt->mM[m];
+            if (offsets) offsets[m] = bt->mbase[m];
+        }
+    });
+}
+
+int tlno_gpu_batch_build_jk_device(tlno_gpu_batch* bt, const double* d_density, double* d_J, double* d_K,
+                                   double schwarz, int use_graph) {
+    return guarded([&] {
+        if (!bt) throw std::runtime_error("tlno_gpu: null batch");
+        if (!d_density) throw std::runtime_error("eri_ao_tensor_hgp: null density.");
+        batch_build(bt, d_density, d_J, d_K, schwarz, use_graph != 0);
+    });
+}
+
+int tlno_gpu_batch_build_jk(tlno_gpu_batch* bt, const double* density, double* J, double* K, double schwarz) {
+    return guarded([&] {
+        if (!bt) throw std::runtime_error("tlno_gpu: null batch");
+        if (!density) throw std::runtime_error("eri_ao_tensor_hgp: null density.");
+        for (int m = 0; m < bt->nmol; ++m) validate_density(density + bt->mbase[m], bt->mM[m]);
+        const std::size_t bytes = static_cast<std::size_t>(bt->total) * sizeof(double);
+        double *dD = nullptr, *dJ = nullptr, *dK = nullptr;
+        HIP_OK(hipSetDevice(bt->device));
+        HIP_OK(hipMalloc(&dD, bytes));
+        if (J) HIP_OK(hipMalloc(&dJ, bytes));
+        if (K) HIP_OK(hipMalloc(&dK, bytes));
+        HIP_OK(hipMemcpy(dD, density, bytes, hipMemcpyHostToDevice));
+        batch_build(bt, dD, dJ, dK, schwarz, false);
+        if (J) HIP_OK(hipMemcpy(J, dJ, bytes, hipMemcpyDeviceToHost));
+        if (K) HIP_OK(hipMemcpy(K, dK, bytes, hipMemcpyDeviceToHost));
+        HIP_OK(hipFree(dD));
+        if (dJ) HIP_OK(hipFree(dJ));
+        if (dK) HIP_OK(hipFree(dK));
+    });
+}
+
+int tlno_gpu_batch_autotune(tlno_gpu_batch* bt, double schwarz, int repeats, char* report, int report_len) {
+    return guarded([&] {
+        if (!bt) throw std::runtime_error("tlno_gpu: null batch");
+        if (!std::isfinite(schwarz) || schwarz < 0.0)
+            throw std::runtime_error("eri_ao_tensor_hgp: schwarz_threshold must be finite and >= 0.");
+        const std::string r = batch_autotune(bt, schwarz, std::max(1, repeats));
+        if (report && report_len > 0) {
+            std::strncpy(report, r.c_str(), static_cast<std::size_t>(report_len) - 1);
+            report[report_len - 1] = '\0';
+        }
+    });
+}
+
+int tlno_gpu_batch_set_shape(tlno_gpu_batch* bt, int X, int Y, int block, int waves_per_cu) {
+    return guarded([&] {
+        if (!bt) throw std::runtime_error("tlno_gpu: null batch");
+        if (X < 0 || X > 5 || Y < 0 || Y > X) throw std::runtime_error("tlno_gpu: class pair out of range");
+        if (block != 64 && block != 128 && block != 256) throw std::runtime_error("tlno_gpu: block must be 64, 128 or 256");
+        if (waves_per_cu < 0 || waves_per_cu > 64) throw std::runtime_error("tlno_gpu: waves per CU must be 0..64");
+        bt->shape[X][Y] = LaunchShape{block, waves_per_cu};
+        bt->drop_graph();
+    });
+}
+
+int tlno_gpu_batch_last_stats(const tlno_gpu_batch* bt, long long* shell_quartets, long long* prim_quartets,
+                              double* ms) {
+    return guarded([&] {
+        if (!bt) throw std::runtime_error("tlno_gpu: null batch");
+        if (shell_quartets) *shell_quartets = bt->last_sq;
+        if (prim_quartets) *prim_quartets = bt->last_pq;
+        if (ms) *ms = bt->last_ms;
     });
 }
 

@@ -341,6 +341,15 @@ struct JKArgs {
     // P != Q; pair kind = 2 * class + (1 if a diagonal pair)
     const double* cw;   // [12][36]
     double* chk;
+    // molecule batches (pmol != nullptr): pair P belongs to molecule pmol[P],
+    // whose D / J / K blocks start at mbase[mol] with row stride mM[mol]
+    const int* pmol;
+    const long long* mbase;
+    const int* mM;
+    // persistent launch (stride > 0): each wave takes work items
+    // w, w + stride, ... below nwork (range mode: tasks; list mode: tasks x split)
+    int stride;
+    int nwork;
 };
 
 struct StoreArgs {
@@ -358,11 +367,11 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Task = {P, q_begin, q_end, bra primitive slice ib0 | ib1 << 16}.
+// One work item of a wave.  Task = {P, q_begin, q_end, bra primitive slice
+// ib0 | ib1 << 16} (range mode) or a list-mode task (see JKArgs).
 template <typename R, int LA, int LB, int LC, int LD, int AC, int CC, int CN>
-__device__ __forceinline__ void eri_jk_body(const JKArgs& a) {
+__device__ __forceinline__ void eri_jk_task(const JKArgs& a, const int wave) {
     using S = Shape<LA, LB, LC, LD, AC, CC, CN>;
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     const int task = a.qlist ? wave / a.split : wave;
     if (task >= a.ntasks) return;  // whole wave leaves together
@@ -449,10 +458,20 @@ __device__ __forceinline__ void eri_jk_body(const JKArgs& a) {
     if (P == Q) s *= 0.5;
     if (!active) s = 0.0;
 
-    const long M = a.M;
+    long M = a.M;
+    const double* __restrict__ Dm = a.D;
+    double* __restrict__ Jm = a.J;
+    double* __restrict__ Km = a.K;
+    if (a.pmol) {  // both pairs of a task belong to P's molecule
+        const int mol = __builtin_amdgcn_readfirstlane(a.pmol[P]);
+        const long long base = a.mbase[mol];
+        M = a.mM[mol];
+        Dm += base;
+        Jm += base;
+        Km += base;
+    }
     const int A0 = a.pd.ao[2 * P], B0 = a.pd.ao[2 * P + 1];
     const int C0 = a.pd.ao[2 * Q], D0 = a.pd.ao[2 * Q + 1];
-    const double* __restrict__ Dm = a.D;
 
     if (a.do_j) {
         // ket side: Jacc[c][d] += 2 s sum_ab (ab|cd) D[a][b]
@@ -482,14 +501,14 @@ __device__ __forceinline__ void eri_jk_body(const JKArgs& a) {
                         jcd = fma(v, dab[ai][b], jcd);
                         jab[ai][b] = fma(v, dcd, jab[ai][b]);
                     }
-                if (active) unsafeAtomicAdd(a.J + row, 2.0 * s * jcd);
+                if (active) unsafeAtomicAdd(Jm + row, 2.0 * s * jcd);
             }
 #pragma unroll
         for (int ai = 0; ai < S::NAL; ++ai)
 #pragma unroll
             for (int b = 0; b < S::NB; ++b) {
                 const double v = wave_sum(2.0 * s * jab[ai][b]);
-                if (lane == 0) unsafeAtomicAdd(a.J + (A0 + S::acomp(ai)) * M + B0 + b, v);
+                if (lane == 0) unsafeAtomicAdd(Jm + (A0 + S::acomp(ai)) * M + B0 + b, v);
             }
     }
     if (a.do_k && active) {
@@ -507,7 +526,7 @@ __device__ __forceinline__ void eri_jk_body(const JKArgs& a) {
 #pragma unroll
                     for (int d = 0; d < S::ND; ++d)
                         kac = fma(double(acc[((ai * S::NB + b) * S::NCL + ci) * S::ND + d]), Dm[(B0 + b) * M + D0 + d], kac);
-                unsafeAtomicAdd(a.K + ga * M + gc, s * kac);
+                unsafeAtomicAdd(Km + ga * M + gc, s * kac);
             }
 #pragma unroll
             for (int d = 0; d < S::ND; ++d) {
@@ -519,7 +538,7 @@ __device__ __forceinline__ void eri_jk_body(const JKArgs& a) {
                     for (int ci = 0; ci < S::NCL; ++ci)
                         kad = fma(double(acc[((ai * S::NB + b) * S::NCL + ci) * S::ND + d]),
                                   Dm[(B0 + b) * M + C0 + S::ccomp(ci)], kad);
-                unsafeAtomicAdd(a.K + ga * M + gd, s * kad);
+                unsafeAtomicAdd(Km + ga * M + gd, s * kad);
             }
         }
 #pragma unroll
@@ -535,7 +554,7 @@ __device__ __forceinline__ void eri_jk_body(const JKArgs& a) {
                     for (int d = 0; d < S::ND; ++d)
                         kbc = fma(double(acc[((ai * S::NB + b) * S::NCL + ci) * S::ND + d]),
                                   Dm[(A0 + S::acomp(ai)) * M + D0 + d], kbc);
-                unsafeAtomicAdd(a.K + gb * M + gc, s * kbc);
+                unsafeAtomicAdd(Km + gb * M + gc, s * kbc);
             }
 #pragma unroll
             for (int d = 0; d < S::ND; ++d) {
@@ -547,10 +566,19 @@ __device__ __forceinline__ void eri_jk_body(const JKArgs& a) {
                     for (int ci = 0; ci < S::NCL; ++ci)
                         kbd = fma(double(acc[((ai * S::NB + b) * S::NCL + ci) * S::ND + d]),
                                   Dm[(A0 + S::acomp(ai)) * M + C0 + S::ccomp(ci)], kbd);
-                unsafeAtomicAdd(a.K + gb * M + gd, s * kbd);
+                unsafeAtomicAdd(Km + gb * M + gd, s * kbd);
             }
         }
     }
+}
+
+// Grid entry: one work item per wave, or a persistent loop over work items.
+template <typename R, int LA, int LB, int LC, int LD, int AC, int CC, int CN>
+__device__ __forceinline__ void eri_jk_body(const JKArgs& a) {
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int end = a.stride > 0 ? a.nwork : wave + 1;
+    const int step = a.stride > 0 ? a.stride : 1;
+    for (int w = wave; w < end; w += step) eri_jk_task<R, LA, LB, LC, LD, AC, CC, CN>(a, w);
 }
 
 // Writes the lane's slice of the unit-diagonal Cartesian block of (P|Q) into

@@ -41,6 +41,15 @@ SIGNATURES = {
     "tlno_gpu_rys": (C.c_int, [C.c_int, C.c_int, D, D, D]),
     "tlno_gpu_onee": (C.c_int, [C.c_void_p, C.c_int, D, D, D, D, D]),
     "tlno_gpu_eri_checksum": (C.c_int, [C.c_void_p, C.c_double, D]),
+    "tlno_gpu_batch_create": (C.c_int, [C.c_int, I, D, I, I, D, D, C.POINTER(C.c_void_p)]),
+    "tlno_gpu_batch_destroy": (None, [C.c_void_p]),
+    "tlno_gpu_batch_info": (C.c_int, [C.c_void_p, I, LL, I, LL]),
+    "tlno_gpu_batch_build_jk": (C.c_int, [C.c_void_p, D, D, D, C.c_double]),
+    "tlno_gpu_batch_build_jk_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                                 C.c_int]),
+    "tlno_gpu_batch_autotune": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_char_p, C.c_int]),
+    "tlno_gpu_batch_set_shape": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "tlno_gpu_batch_last_stats": (C.c_int, [C.c_void_p, LL, LL, D]),
     "tlno_gpu_set_learned_screen": (C.c_int, [C.c_void_p, F, C.c_double]),
     "tlno_gpu_quartet_features": (C.c_int, [C.c_void_p, C.c_int, I, I, F]),
     "tlno_gpu_last_route_stats": (C.c_int, [C.c_void_p, LL]),

@@ -121,13 +121,13 @@ def organic30(seed=SEED0 + 2, natoms=30, box_A=12.0):
     return b, D
 
 
-def batch256(seed=SEED0 + 3, nmol=256):
+def batch256(seed=SEED0 + 3, nmol=256, atom_range=(3, 40)):
     rng = np.random.default_rng(seed)
     eb = _element_bases(rng, ["C", "N", "O", "H"], [0, 0, 0, 1, 1, 2], [0, 0, 1],
                         (3, 3), (0.1, 150.0))
     mols = []
     for i in range(nmol):
-        na = int(rng.integers(3, 41))
+        na = int(rng.integers(atom_range[0], atom_range[1] + 1))
         box = 12.0 * (na / 30.0) ** (1.0 / 3.0)
         atoms = _random_atoms(rng, na, box, 1.0)
         b = _build(atoms, eb, f"batch{i}")

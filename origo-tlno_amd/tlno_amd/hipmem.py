@@ -52,6 +52,13 @@ class DeviceArray:
         _ok(hip().hipMemcpy(d.ptr, a.ctypes.data_as(C.c_void_p), d.nbytes, 1), "hipMemcpy H2D")
         return d
 
+    def upload(self, a):
+        """Copy a host array of the same size into this buffer."""
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.size * 8 != self.nbytes:
+            raise ValueError("size mismatch")
+        _ok(hip().hipMemcpy(self.ptr, a.ctypes.data_as(C.c_void_p), self.nbytes, 1), "hipMemcpy H2D")
+
     def to_host(self):
         out = np.empty(self.shape)
         _ok(hip().hipMemcpy(out.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes, 2), "hipMemcpy D2H")
