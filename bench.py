@@ -148,6 +148,57 @@ def mixed_run(steps=3, gate=1e-6, tau=SCREEN_TAU):
     return out
 
 
+def batched_run(steps=5):
+    """batch256: 256 molecules of 3-40 atoms (25-460 Cartesian functions,
+    every class up to (dd|dd)), FP64, Schwarz 1e-12: one batched build = J and
+    K of all 256 molecules.  Reports default launch shapes, autotuned shapes,
+    autotuned + HIP-graph replay, and 256 separate single-molecule builds."""
+    from tlno_amd.batch import BatchPlan
+    lib = _lib.load()
+    mols = configs.batch256()
+    t0 = time.perf_counter()
+    bp = BatchPlan([b for b, _ in mols])
+    plan_s = time.perf_counter() - t0
+    dD = hipmem.DeviceArray.from_host(bp.pack([D for _, D in mols]))
+    dJ = hipmem.DeviceArray((bp.total,))
+    dK = hipmem.DeviceArray((bp.total,))
+
+    def timed(use_graph):
+        bp.jk_device(dD, dJ, dK, use_graph=use_graph)
+        t = time.perf_counter()
+        for _ in range(steps):
+            bp.jk_device(dD, dJ, dK, use_graph=use_graph)
+        return (time.perf_counter() - t) / steps
+
+    t_default = timed(False)
+    ref_J = dJ.to_host()
+    tune = bp.autotune(repeats=2)
+    t_tuned = timed(False)
+    t_graph = timed(True)
+    st = bp.last_stats()
+    err = float(np.abs(dJ.to_host() - ref_J).max())
+    # the same molecules one plan per molecule, device-resident densities
+    singles = []
+    for b, D in mols:
+        p = DirectPlan(b, ScreeningOptions(integral_threshold=1e-12), cartesian=True)
+        singles.append((p, hipmem.DeviceArray.from_host(D), hipmem.DeviceArray(D.shape), hipmem.DeviceArray(D.shape)))
+    for p, d, j, k in singles:
+        _lib.check(lib.tlno_gpu_build_jk_device(p.handle, d.ptr, j.ptr, k.ptr, 1e-12))
+    t = time.perf_counter()
+    for p, d, j, k in singles:
+        _lib.check(lib.tlno_gpu_build_jk_device(p.handle, d.ptr, j.ptr, k.ptr, 1e-12))
+    hipmem.synchronize()
+    t_single = time.perf_counter() - t
+    return {"workload": "batch256: 256 molecules, 3-40 atoms, heavy 3s2p1d / H 2s1p Cartesian, Schwarz 1e-12",
+            "molecules": bp.n_mol, "functions_min_max": [int(bp.M.min()), int(bp.M.max())],
+            "plan_seconds": plan_s, "shell_quartets_per_build": st["shell_quartets"],
+            "prim_quartets_per_build": st["prim_quartets"],
+            "ms_per_batched_build": {"default_shapes": t_default * 1e3, "autotuned": t_tuned * 1e3,
+                                     "autotuned_graph": t_graph * 1e3, "256_single_molecule_builds": t_single * 1e3},
+            "batched_builds_per_s": 1.0 / t_graph, "molecule_builds_per_s": bp.n_mol / t_graph,
+            "graph_vs_default_max_abs_diff_J": err, "autotuner": tune}
+
+
 def cpu_baseline(b, seconds_cap=60.0):
     try:
         from oracle import oracle as O
@@ -181,6 +232,7 @@ def main():
     ap.add_argument("--workload", default="organic30", choices=["organic30", "water40"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the J/K build shards by "
@@ -225,6 +277,8 @@ def main():
     }
     if not args.no_mixed:
         result["mixed_precision"] = mixed_run()
+    if not args.no_batch:
+        result["batched"] = batched_run()
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(b)
         if result["cpu_baseline"]["value"]:
