@@ -128,6 +128,22 @@ int tlno_gpu_quartet_features(tlno_gpu_plan* plan, int n, const int* pair_p, con
  * [3] of those sent to FP32. */
 int tlno_gpu_last_route_stats(const tlno_gpu_plan* plan, long long* stats);
 
+/* Kernel policy (no reference counterpart): FP64 builds evaluate classes
+ * with at least min_outputs Cartesian values per shell quartet with the team
+ * evaluator (one wave per quartet, horizontal recurrence after the primitive
+ * contraction); smaller classes, and every class when min_outputs = 0, use the
+ * lane evaluator (one lane per quartet).  Default 0 (tlno_gpu_plan_autotune
+ * chooses per class). */
+int tlno_gpu_set_team_threshold(tlno_gpu_plan* plan, int min_outputs);
+
+/* Autotuner of the plan's kernels (no reference counterpart): per class pair,
+ * times the lane and the team evaluator on this plan (repeats serialized FP64
+ * builds each) and keeps the faster for following FP64 builds; report
+ * receives a JSON summary (may be NULL).  tlno_gpu_set_team_threshold
+ * discards the choices. */
+int tlno_gpu_plan_autotune(tlno_gpu_plan* plan, double schwarz, int repeats, char* report,
+                           int report_len);
+
 /* Mixed precision (no reference counterpart): with fp32_threshold > 0,
  * following builds of this plan evaluate the 2D recurrences and the Cartesian
  * contraction of every shell quartet with Q_ab Q_cd max|D| < fp32_threshold in

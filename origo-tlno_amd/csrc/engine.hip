@@ -32,6 +32,14 @@
 #include "onee.hpp"
 #include "tlno_gpu.h"
 
+// Classes with at least this many Cartesian values per shell quartet run the
+// team evaluator (one wave per quartet, HRR after contraction) in FP64 builds;
+// 0 (default): the lane evaluator unless the plan's autotuner chose the team
+// evaluator for a class.
+#ifndef TLNO_TEAM_MIN_OUT
+#define TLNO_TEAM_MIN_OUT 0
+#endif
+
 namespace tlno_amd {
 
 thread_local std::string g_err;
@@ -303,6 +311,8 @@ struct tlno_gpu_plan {
     unsigned* d_tcount = nullptr;            // [2][36] tasks emitted per route and class pair
     unsigned long long* d_rstats = nullptr;  // [4]
     long long last_route[4] = {0, 0, 0, 0};  // candidates, density-kept, kept, FP32 (last routed build)
+    int team_min_out = TLNO_TEAM_MIN_OUT;    // 0: lane evaluator for every class
+    signed char team_sel[6][6];              // autotuned choice per class pair: -1 none, 0 lane, 1 team
     // optional per-class kernel timing with HIP events on the plan stream
     int timing = 0;
     hipEvent_t ev[6][6][2] = {};
@@ -503,9 +513,26 @@ __global__ void k_rys(DevTables tb, const double* T, double* r, double* w, int n
 
 
 // grid: x covers the tasks (one wave each, 4 waves per block), y the chunks
-// nwaves waves per chunk, every chunk group of the class (FP32 or FP64 kernels)
-void launch_jk(const ClassKernels& k, bool fp32, JKArgs& args, long long nwaves, hipStream_t st) {
+bool use_team(const ClassKernels& k, bool fp32, const JKArgs& args, int min_out) {
+    return !fp32 && k.team && !args.cw && min_out > 0 && k.nout >= min_out;
+}
+
+// the team threshold to apply to class pair (X, Y): the autotuned choice when
+// there is one (1 = team, expressed as threshold 1; 0 = lane, threshold 0)
+inline int class_team_min(const tlno_gpu_plan* pl, int X, int Y);
+
+// nwaves waves per chunk, every chunk group of the class (FP32 or FP64
+// kernels), or the team kernel with one 64-thread block per work item
+void launch_jk(const ClassKernels& k, bool fp32, JKArgs& args, long long nwaves, hipStream_t st,
+               int team_min_out = 0) {
     void* params[] = {&args};
+    if (use_team(k, fp32, args, team_min_out)) {  // one block per shell quartet (64 per task)
+        args.stride = 0;
+        HIP_OK(hipLaunchKernel(reinterpret_cast<const void*>(k.team),
+                               dim3(static_cast<unsigned>(static_cast<long long>(args.ntasks) * 64)), dim3(64),
+                               params, 0, st));
+        return;
+    }
     const unsigned grid = grid_for(nwaves * 64, 256);
     for (int g = 0; g < k.ngroup; ++g) {
         const JKKernel kern = fp32 ? k.grp[g].jk32 : k.grp[g].jk;
@@ -887,6 +914,7 @@ void upload_pair_features(tlno_gpu_plan* pl, const std::vector<int>& order,
 
 void build_plan(tlno_gpu_plan* pl, int n, const double* centers, const int* ang, const int* nprim,
                 const double* exps, const double* coefs) {
+    std::memset(pl->team_sel, -1, sizeof(pl->team_sel));
     // input validation first: invalid bases fail the same way with or without a device
     pl->groups = build_groups(n, centers, ang, nprim, exps, coefs);
     DeviceCtx& ctx = device_ctx(pl->device);
@@ -1191,6 +1219,11 @@ TaskSet& get_tasks(tlno_gpu_plan* pl, double thr) {
     return pl->tasks.emplace(thr, ts).first->second;
 }
 
+inline int class_team_min(const tlno_gpu_plan* pl, int X, int Y) {
+    const int sel = pl->team_sel[X][Y];
+    return sel < 0 ? pl->team_min_out : (sel ? 1 : 0);
+}
+
 // Candidate rows for the per-quartet routing kernel (once per threshold):
 // every Schwarz-surviving (P | ket range) split into rows of <= kRouteSeg
 // kets, each owning a qlist region of its length; per class pair the task
@@ -1381,8 +1414,8 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
                 if (n64 > 0) {
                     a.tasks = pl->d_t64 + ts.toff[xy];
                     a.ntasks = static_cast<int>(n64);
-                    a.split = list_split(n64, k.nchunk);
-                    launch_jk(k, false, a, n64 * a.split, cs);
+                    a.split = use_team(k, false, a, class_team_min(pl, X, Y)) ? 1 : list_split(n64, k.nchunk);
+                    launch_jk(k, false, a, n64 * a.split, cs, class_team_min(pl, X, Y));
                 }
                 if (n32 > 0) {
                     // the FP32 share goes to a stream of its own so that it
@@ -1416,7 +1449,7 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
                 if (n64 > 0) {
                     a.tasks = ts.d_tasks + ts.off[X][Y];
                     a.ntasks = static_cast<int>(n64);
-                    launch_jk(k, false, a, n64, cs);
+                    launch_jk(k, false, a, n64, cs, class_team_min(pl, X, Y));
                 }
                 if (nt > n64) {
                     a.tasks = ts.d_tasks + ts.off[X][Y] + n64;
@@ -1610,6 +1643,68 @@ double eri_checksum(tlno_gpu_plan* pl, double thr) {
     return total;
 }
 
+// Per class pair, time the lane and the team evaluator on this plan (FP64,
+// J and K, a constant density, serialized launches bracketed by events) and
+// keep the faster one; returns a JSON summary.
+std::string plan_autotune(tlno_gpu_plan* pl, double thr, int repeats) {
+    HIP_OK(hipSetDevice(pl->device));
+    const long mo2 = static_cast<long>(pl->Mout) * pl->Mout;
+    double *dD = nullptr, *dJ = nullptr, *dK = nullptr;
+    HIP_OK(hipMalloc(&dD, mo2 * sizeof(double)));
+    HIP_OK(hipMalloc(&dJ, mo2 * sizeof(double)));
+    HIP_OK(hipMalloc(&dK, mo2 * sizeof(double)));
+    k_fill<<<grid_for(mo2, 256), 256, 0, pl->stream>>>(dD, mo2, 1e-3);
+    const int saved_timing = pl->timing;
+    const double saved_gate = pl->fp32_gate, saved_dthr = pl->density_thr;
+    const int saved_learned = pl->learned;
+    pl->fp32_gate = 0.0;
+    pl->density_thr = 0.0;
+    pl->learned = 0;
+    if (!pl->timing)
+        for (int X = 0; X < 6; ++X)
+            for (int Y = 0; Y < 6; ++Y)
+                for (int e = 0; e < 2; ++e)
+                    if (!pl->ev[X][Y][e]) HIP_OK(hipEventCreate(&pl->ev[X][Y][e]));
+    pl->timing = 1;
+    double ms[2][6][6] = {};
+    for (int mode = 0; mode < 2; ++mode) {
+        for (int X = 0; X < 6; ++X)
+            for (int Y = 0; Y < 6; ++Y) pl->team_sel[X][Y] = static_cast<signed char>(mode);
+        jk_device(pl, dD, dJ, dK, thr);  // warm-up
+        std::memset(pl->cls_ms, 0, sizeof(pl->cls_ms));
+        pl->timed_builds = 0;
+        for (int r = 0; r < repeats; ++r) jk_device(pl, dD, dJ, dK, thr);
+        for (int X = 0; X < 6; ++X)
+            for (int Y = 0; Y < 6; ++Y) ms[mode][X][Y] = pl->cls_ms[X][Y] / std::max<long long>(1, pl->timed_builds);
+    }
+    std::string rep = "{";
+    bool first = true;
+    static const char* names[6] = {"ss", "ps", "pp", "ds", "dp", "dd"};
+    for (int X = 5; X >= 0; --X)
+        for (int Y = X; Y >= 0; --Y) {
+            const bool has_team = g_kernels[X][Y].team != nullptr;
+            const bool team = has_team && ms[1][X][Y] < ms[0][X][Y];
+            pl->team_sel[X][Y] = static_cast<signed char>(team ? 1 : 0);
+            if (!has_team || ms[0][X][Y] == 0.0) continue;
+            char buf[200];
+            std::snprintf(buf, sizeof(buf), "%s\"(%s|%s)\": {\"lane_ms\": %.4f, \"team_ms\": %.4f, \"choice\": \"%s\"}",
+                          first ? "" : ", ", names[X], names[Y], ms[0][X][Y], ms[1][X][Y], team ? "team" : "lane");
+            rep += buf;
+            first = false;
+        }
+    rep += "}";
+    pl->timing = saved_timing;
+    std::memset(pl->cls_ms, 0, sizeof(pl->cls_ms));
+    pl->timed_builds = 0;
+    pl->fp32_gate = saved_gate;
+    pl->density_thr = saved_dthr;
+    pl->learned = saved_learned;
+    HIP_OK(hipFree(dD));
+    HIP_OK(hipFree(dJ));
+    HIP_OK(hipFree(dK));
+    return rep;
+}
+
 void validate_density(const double* D, int M) {
     const double tol = 128.0 * 2.220446049250313e-16;
     for (int i = 0; i < M; ++i)
@@ -1797,6 +1892,28 @@ int tlno_gpu_set_density_threshold(tlno_gpu_plan* pl, double density_threshold) 
         if (!std::isfinite(density_threshold) || density_threshold < 0.0)
             throw std::runtime_error("eri_ao_tensor_hgp: density_threshold must be finite and >= 0.");
         pl->density_thr = density_threshold;
+    });
+}
+
+int tlno_gpu_set_team_threshold(tlno_gpu_plan* pl, int min_outputs) {
+    return guarded([&] {
+        if (!pl) throw std::runtime_error("jk_direct: null J/K plan backend.");
+        if (min_outputs < 0) throw std::runtime_error("tlno_gpu: team threshold must be >= 0");
+        pl->team_min_out = min_outputs;
+        std::memset(pl->team_sel, -1, sizeof(pl->team_sel));  // an explicit policy replaces autotuned choices
+    });
+}
+
+int tlno_gpu_plan_autotune(tlno_gpu_plan* pl, double schwarz, int repeats, char* report, int report_len) {
+    return guarded([&] {
+        if (!pl) throw std::runtime_error("jk_direct: null J/K plan backend.");
+        if (!std::isfinite(schwarz) || schwarz < 0.0)
+            throw std::runtime_error("eri_ao_tensor_hgp: schwarz_threshold must be finite and >= 0.");
+        const std::string r = plan_autotune(pl, schwarz, std::max(1, repeats));
+        if (report && report_len > 0) {
+            std::strncpy(report, r.c_str(), static_cast<std::size_t>(report_len) - 1);
+            report[report_len - 1] = '\0';
+        }
     });
 }
 
@@ -2236,6 +2353,7 @@ struct tlno_gpu_batch {
     };
     std::map<double, Tasks> tasks;
     tlno_amd::LaunchShape shape[6][6];
+    int team_min_out = TLNO_TEAM_MIN_OUT;
     // captured build
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -2439,6 +2557,19 @@ void batch_launch_class(tlno_gpu_batch* bt, const tlno_gpu_batch::Tasks& ts, int
     }
     const ClassKernels& k = g_kernels[X][Y];
     void* params[] = {&a};
+    if (use_team(k, false, a, bt->team_min_out)) {  // one block per shell quartet, or a persistent grid
+        const long long items = nt * 64;
+        long long blocks = items;
+        if (sh.waves_per_cu > 0) {
+            blocks = std::min<long long>(items, static_cast<long long>(bt->n_cu) * sh.waves_per_cu);
+            a.stride = static_cast<int>(blocks);
+        } else {
+            a.stride = 0;
+        }
+        HIP_OK(hipLaunchKernel(reinterpret_cast<const void*>(k.team), dim3(static_cast<unsigned>(blocks)), dim3(64),
+                               params, 0, st));
+        return;
+    }
     const int block = sh.block;
     const unsigned grid = grid_for(waves * 64, block);
     for (int g = 0; g < k.ngroup; ++g)

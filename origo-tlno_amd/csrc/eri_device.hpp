@@ -149,6 +149,55 @@ __device__ __forceinline__ void rys_roots(const DevTables& tb, double T, double*
     }
 }
 
+// Rys roots and weights one root at a time (the team evaluator gives each
+// lane a single root of a primitive quartet).
+template <int N>
+struct RootEval {
+    const double* c = nullptr;  // table branch: this T's interval (nullptr: asymptotic branch)
+    double u = 0.0, iT = 0.0, isT = 0.0;
+    double F0 = 0.0, F1 = 0.0;  // N == 1
+    __device__ __forceinline__ RootEval(const DevTables& tb, double T) {
+        if constexpr (N == 1) {
+            double F[2];
+            boys_all<1>(tb, T, F);
+            F0 = F[0];
+            F1 = F[1];
+        } else {
+            constexpr int ttab[6] = {0, 41, 49, 55, 61, 67};
+            if (T < ttab[N]) {
+                const int k = static_cast<int>(T);
+                u = T - k - 0.5;
+                c = tb.rys + tb.rys_off[N] + k * (2 * N * RYS_NCOEF);
+            } else {
+                iT = 1.0 / T;
+                isT = sqrt(iT);
+            }
+        }
+    }
+    __device__ __forceinline__ void get(const DevTables& tb, int r, double& t2, double& w) const {
+        if constexpr (N == 1) {
+            t2 = F1 / F0;
+            w = F0;
+        } else {
+            if (c) {
+                const double* cr = c + r * RYS_NCOEF;
+                const double* cw = c + (N + r) * RYS_NCOEF;
+                double s = cr[RYS_NCOEF - 1], sw = cw[RYS_NCOEF - 1];
+#pragma unroll
+                for (int p = RYS_NCOEF - 2; p >= 0; --p) {
+                    s = fma(s, u, cr[p]);
+                    sw = fma(sw, u, cw[p]);
+                }
+                t2 = s;
+                w = sw;
+            } else {
+                t2 = tb.herm_r2[N][r] * iT;
+                w = tb.herm_w[N][r] * isT;
+            }
+        }
+    }
+};
+
 // ------------------------------------------------------------ pair data ---
 struct PairData {
     const double* A;        // [npair][3] centre of the bra-side (higher-L) shell
@@ -622,6 +671,8 @@ struct ClassKernels {
     int ngroup = 0;
     JKGroup grp[kMaxChunkGroups];
     StoreKernel store = nullptr;
+    JKKernel team = nullptr;  // one wave per shell quartet, FP64 (classes with L_total >= 4)
+    int nout = 0;             // Cartesian values per shell quartet
 };
 
 // class index: 0 ss, 1 ps, 2 pp, 3 ds, 4 dp, 5 dd (pair orientation: higher L first)

@@ -14,6 +14,7 @@
 #include <utility>
 
 #include "eri_device.hpp"
+#include "eri_team.hpp"
 
 #ifndef TLNO_CHUNK_MAXOUT
 #define TLNO_CHUNK_MAXOUT 108
@@ -92,6 +93,8 @@ void reg_class(ClassKernels (&tab)[6][6]) {
     k.ngroup = 0;
     reg_groups<LA, LB, LC, LD>(k, std::make_integer_sequence<int, ngroup>{});
     reg_store<LA, LB, LC, LD>(k, std::make_integer_sequence<int, CP::nchunk>{});
+    k.nout = CP::nc;
+    if constexpr (LA + LB + LC + LD >= 4) k.team = k_eri_team<LA, LB, LC, LD>;
 }
 
 }  // namespace tlno_amd

@@ -1,0 +1,332 @@
+// Team evaluator for high angular momentum classes: one wave per shell
+// quartet, with the horizontal recurrence moved after the contraction.
+//
+// For bra shells (LA, LB) and ket shells (LC, LD) the contracted quartet is
+//   (ab|cd) = HRR_bra HRR_ket [e|f],   [e|f] = sum_prim sum_root Gx Gy (w Gz),
+// with e over the Cartesian triples LA <= |e| <= LA+LB, f over LC <= |f| <= LC+LD
+// and G the 2D Rys tables of each axis (G[ex][fx], ex <= LA+LB, fx <= LC+LD).
+// The HRR coefficients (AB, CD) do not depend on the primitives, so the
+// transfer runs once per shell quartet instead of once per primitive root:
+//   phase A  each lane takes one (ket primitive, bra primitive, root) combination
+//            of a batch, evaluates its root and the three G tables and writes
+//            them to LDS;
+//   phase B  each lane owns a slice of the [e|f] intermediates and accumulates
+//            Gx Gy Gz over the batch from LDS;
+//   phase C  [e|f] goes to LDS, each lane expands its slice of the outputs by
+//            the binomial form of the HRR, (a, b| = sum_{t <= b} C(b, t) AB^(b-t) [a+t|
+//            (per axis), the same on the ket;
+//   phase D  the block sits in LDS; lanes reduce J and K blocks with the
+//            density blocks of the six shell pairs (one atomic per element).
+// The digestion contract is that of the lane evaluator (eri_jk_task).
+#pragma once
+
+#include "eri_device.hpp"
+
+namespace tlno_amd {
+
+constexpr int ncart_cum(int L) { return (L + 1) * (L + 2) * (L + 3) / 6; }  // sum_{l <= L} ncart(l)
+
+// index of triple (x, y, z), |e| = x + y + z, in the list of all triples with
+// LMIN <= |e| ordered by |e| then descending lex
+__host__ __device__ constexpr int tri_index(int LMIN, int x, int y, int z) {
+    const int L = x + y + z;
+    const int dx = L - x;
+    return (ncart_cum(L - 1) - (LMIN > 0 ? ncart_cum(LMIN - 1) : 0)) + dx * (dx + 1) / 2 + (dx - y);
+}
+
+template <int LA, int LB, int LC, int LD>
+struct TeamShape {
+    static constexpr int LAB = LA + LB, LCD = LC + LD;
+    static constexpr int NR = (LAB + LCD) / 2 + 1;
+    static constexpr int NE = ncart_cum(LAB) - (LA > 0 ? ncart_cum(LA - 1) : 0);
+    static constexpr int NF = ncart_cum(LCD) - (LC > 0 ? ncart_cum(LC - 1) : 0);
+    static constexpr int NI = NE * NF;                     // [e|f] intermediates
+    static constexpr int IPL = (NI + 63) / 64;             // per lane
+    static constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    static constexpr int NOUT = NA * NB * NC * ND;
+    static constexpr int OPL = (NOUT + 63) / 64;
+    static constexpr int GS = (LAB + 1) * (LCD + 1);       // one axis table
+    static constexpr int CS = 3 * GS + 1;                  // one combination (padded)
+    static constexpr int CB = 32;                          // combinations per batch
+    static constexpr int LDS_A = CB * CS;
+    static constexpr int LDS_C = NI + NOUT;
+    static constexpr int LDS_D = NI + NOUT + NA * NB + NC * ND + NA * NC + NA * ND + NB * NC + NB * ND;
+    static constexpr int LDS = LDS_A > (LDS_C > LDS_D ? LDS_C : LDS_D) ? LDS_A : (LDS_C > LDS_D ? LDS_C : LDS_D);
+};
+
+// triple of list position p (|e| from LMIN), decoded arithmetically
+__device__ inline void tri_decode(int LMIN, int p, int& x, int& y, int& z) {
+    int L = LMIN, base = 0;
+    while (p >= base + ncart(L)) {
+        base += ncart(L);
+        ++L;
+    }
+    int k = p - base, dx = 0;
+    while (k >= dx + 1) {
+        k -= dx + 1;
+        ++dx;
+    }
+    x = L - dx;
+    y = dx - k;
+    z = L - x - y;
+}
+
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void eri_team_quartet(const JKArgs& a, double* __restrict__ lds, int P, int Q,
+                                                 double* __restrict__ Jm, double* __restrict__ Km,
+                                                 const double* __restrict__ Dm, long M) {
+    using T = TeamShape<LA, LB, LC, LD>;
+    constexpr CartT<LA> ca = make_cart<LA>();
+    constexpr CartT<LB> cb = make_cart<LB>();
+    constexpr CartT<LC> cc = make_cart<LC>();
+    constexpr CartT<LD> cd = make_cart<LD>();
+    const int lane = threadIdx.x & 63;
+    const PairData& pd = a.pd;
+
+    // owned intermediates: LDS offsets of their G factors inside a combination
+    int ox[T::IPL], oy[T::IPL], oz[T::IPL];
+#pragma unroll
+    for (int k = 0; k < T::IPL; ++k) {
+        int i = lane + 64 * k;
+        if (i >= T::NI) i = T::NI - 1;
+        int ex, ey, ez, fx, fy, fz;
+        tri_decode(LA, i / T::NF, ex, ey, ez);
+        tri_decode(LC, i % T::NF, fx, fy, fz);
+        ox[k] = ex * (T::LCD + 1) + fx;
+        oy[k] = T::GS + ey * (T::LCD + 1) + fy;
+        oz[k] = 2 * T::GS + ez * (T::LCD + 1) + fz;
+    }
+    double acc[T::IPL];
+#pragma unroll
+    for (int k = 0; k < T::IPL; ++k) acc[k] = 0.0;
+
+    const double Ax = pd.A[3 * P], Ay = pd.A[3 * P + 1], Az = pd.A[3 * P + 2];
+    const double Cx = pd.A[3 * Q], Cy = pd.A[3 * Q + 1], Cz = pd.A[3 * Q + 2];
+    const int pb0 = pd.poff[P], pbn = pd.pn[P];
+    const int pk0 = pd.poff[Q], pkn = pd.pn[Q];
+    const int ncomb = pkn * pbn * T::NR;
+
+    for (int c0 = 0; c0 < ncomb; c0 += T::CB) {
+        // ---- phase A: one combination per lane (lanes >= CB idle)
+        const int c = c0 + lane;
+        if (lane < T::CB && c < ncomb) {
+            const int r = c % T::NR;
+            const int pq = c / T::NR;
+            const int ib = pq % pbn, ik = pq / pbn;
+            const double4 Pp = pd.pP[pb0 + ib];
+            const double4 Qp = pd.pP[pk0 + ik];
+            const double p = Pp.w, q = Qp.w;
+            const double ipq = 1.0 / (p + q);
+            const double PQx = Pp.x - Qp.x, PQy = Pp.y - Qp.y, PQz = Pp.z - Qp.z;
+            const double Tv = p * q * ipq * (PQx * PQx + PQy * PQy + PQz * PQz);
+            const double pref = kTwoPi52 * pd.pw[pb0 + ib] * pd.pw[pk0 + ik] * sqrt(ipq);
+            const RootEval<T::NR> re(a.tb, Tv);
+            double t2, wr;
+            re.get(a.tb, r, t2, wr);
+            const double qo = q * ipq, po = p * ipq;
+            const double B00 = 0.5 * ipq * t2;
+            const double B10 = (0.5 / p) * (1.0 - qo * t2);
+            const double B01 = (0.5 / q) * (1.0 - po * t2);
+            const double qt = qo * t2, pt = po * t2;
+            double* g = lds + lane * T::CS;
+            // one axis at a time (x, y, z), so a single 2D table is live
+#pragma unroll 1
+            for (int ax = 0; ax < 3; ++ax) {
+                const double c00 = ax == 0 ? Pp.x - Ax - qt * PQx : (ax == 1 ? Pp.y - Ay - qt * PQy : Pp.z - Az - qt * PQz);
+                const double d00 = ax == 0 ? Qp.x - Cx + pt * PQx : (ax == 1 ? Qp.y - Cy + pt * PQy : Qp.z - Cz + pt * PQz);
+                double G[T::LAB + 1][T::LCD + 1];
+                G[0][0] = ax == 2 ? pref * wr : 1.0;
+                if constexpr (T::LAB > 0) G[1][0] = c00 * G[0][0];
+#pragma unroll
+                for (int e = 1; e < T::LAB; ++e) G[e + 1][0] = c00 * G[e][0] + (e * B10) * G[e - 1][0];
+#pragma unroll
+                for (int f = 0; f < T::LCD; ++f)
+#pragma unroll
+                    for (int e = 0; e <= T::LAB; ++e) {
+                        double v = d00 * G[e][f];
+                        if (e > 0) v = fma(e * B00, G[e - 1][f], v);
+                        if (f > 0) v = fma(f * B01, G[e][f - 1], v);
+                        G[e][f + 1] = v;
+                    }
+                double* ga = g + ax * T::GS;
+#pragma unroll
+                for (int e = 0; e <= T::LAB; ++e)
+#pragma unroll
+                    for (int f = 0; f <= T::LCD; ++f) ga[e * (T::LCD + 1) + f] = G[e][f];
+            }
+        }
+        __syncthreads();
+        // ---- phase B: accumulate the owned intermediates over the batch
+        const int nb = min(T::CB, ncomb - c0);
+        for (int j = 0; j < nb; ++j) {
+            const double* g = lds + j * T::CS;
+#pragma unroll
+            for (int k = 0; k < T::IPL; ++k) acc[k] = fma(g[ox[k]] * g[oy[k]], g[oz[k]], acc[k]);
+        }
+        __syncthreads();
+    }
+
+    // ---- phase C: HRR from [e|f] to the unit-diagonal Cartesian block
+    double* ef = lds;             // NI
+    double* blk = lds + T::NI;    // NOUT, [a][b][c][d]
+#pragma unroll
+    for (int k = 0; k < T::IPL; ++k) {
+        const int i = lane + 64 * k;
+        if (i < T::NI) ef[i] = acc[k];
+    }
+    __syncthreads();
+    const double AB[3] = {pd.AB[3 * P], pd.AB[3 * P + 1], pd.AB[3 * P + 2]};
+    const double CD[3] = {pd.AB[3 * Q], pd.AB[3 * Q + 1], pd.AB[3 * Q + 2]};
+    // binomial HRR weight C(n, t) X^(n - t) for n <= 2
+    auto hw = [](int n, int t, double X) { return t == n ? 1.0 : (n - t == 1 ? n * X : X * X); };
+    for (int o = lane; o < T::NOUT; o += 64) {
+        const int d = o % T::ND, c = (o / T::ND) % T::NC, b = (o / (T::ND * T::NC)) % T::NB,
+                  a0 = o / (T::ND * T::NC * T::NB);
+        const int axc = ca.e[0][a0], ayc = ca.e[1][a0], azc = ca.e[2][a0];
+        const int bx = cb.e[0][b], by = cb.e[1][b], bz = cb.e[2][b];
+        const int cxc = cc.e[0][c], cyc = cc.e[1][c], czc = cc.e[2][c];
+        const int dx = cd.e[0][d], dy = cd.e[1][d], dz = cd.e[2][d];
+        double v = 0.0;
+        for (int tx = 0; tx <= bx; ++tx)
+            for (int ty = 0; ty <= by; ++ty)
+                for (int tz = 0; tz <= bz; ++tz) {
+                    const double wb = hw(bx, tx, AB[0]) * hw(by, ty, AB[1]) * hw(bz, tz, AB[2]);
+                    const double* row = ef + tri_index(LA, axc + tx, ayc + ty, azc + tz) * T::NF;
+                    double u = 0.0;
+                    for (int ux = 0; ux <= dx; ++ux)
+                        for (int uy = 0; uy <= dy; ++uy)
+                            for (int uz = 0; uz <= dz; ++uz)
+                                u = fma(hw(dx, ux, CD[0]) * hw(dy, uy, CD[1]) * hw(dz, uz, CD[2]),
+                                        row[tri_index(LC, cxc + ux, cyc + uy, czc + uz)], u);
+                    v = fma(wb, u, v);
+                }
+        blk[o] = v * (ca.nang[a0] * cb.nang[b] * cc.nang[c] * cd.nang[d]);
+    }
+    __syncthreads();
+
+    // ---- phase D: digestion (J and K element sums from the LDS block)
+    double s = 1.0;
+    if (pd.diag[P]) s *= 0.5;
+    if (pd.diag[Q]) s *= 0.5;
+    if (P == Q) s *= 0.5;
+    const int A0 = pd.ao[2 * P], B0 = pd.ao[2 * P + 1];
+    const int C0 = pd.ao[2 * Q], D0 = pd.ao[2 * Q + 1];
+    constexpr int NA = T::NA, NB = T::NB, NC = T::NC, ND = T::ND;
+    // the six density blocks of the quartet's shells, staged in LDS after the block
+    double* dab = blk + T::NOUT;
+    double* dcd = dab + NA * NB;
+    double* dac = dcd + NC * ND;
+    double* dad = dac + NA * NC;
+    double* dbc = dad + NA * ND;
+    double* dbd = dbc + NB * NC;
+    for (int t = lane; t < NA * NB; t += 64) dab[t] = Dm[(A0 + t / NB) * M + B0 + t % NB];
+    for (int t = lane; t < NC * ND; t += 64) dcd[t] = Dm[(C0 + t / ND) * M + D0 + t % ND];
+    for (int t = lane; t < NA * NC; t += 64) dac[t] = Dm[(A0 + t / NC) * M + C0 + t % NC];
+    for (int t = lane; t < NA * ND; t += 64) dad[t] = Dm[(A0 + t / ND) * M + D0 + t % ND];
+    for (int t = lane; t < NB * NC; t += 64) dbc[t] = Dm[(B0 + t / NC) * M + C0 + t % NC];
+    for (int t = lane; t < NB * ND; t += 64) dbd[t] = Dm[(B0 + t / ND) * M + D0 + t % ND];
+    __syncthreads();
+    auto B = [&](int ai, int bi, int ci, int di) { return blk[((ai * NB + bi) * NC + ci) * ND + di]; };
+    if (a.do_j) {
+        // J[a][b] += 2 s sum_cd (ab|cd) D[c][d];  J[c][d] += 2 s sum_ab (ab|cd) D[a][b]
+        for (int t = lane; t < NA * NB + NC * ND; t += 64) {
+            double v = 0.0;
+            if (t < NA * NB) {
+                const int ai = t / NB, bi = t % NB;
+                for (int ci = 0; ci < NC; ++ci)
+                    for (int di = 0; di < ND; ++di) v = fma(B(ai, bi, ci, di), dcd[ci * ND + di], v);
+                unsafeAtomicAdd(Jm + (A0 + ai) * M + B0 + bi, 2.0 * s * v);
+            } else {
+                const int u = t - NA * NB, ci = u / ND, di = u % ND;
+                for (int ai = 0; ai < NA; ++ai)
+                    for (int bi = 0; bi < NB; ++bi) v = fma(B(ai, bi, ci, di), dab[ai * NB + bi], v);
+                unsafeAtomicAdd(Jm + (C0 + ci) * M + D0 + di, 2.0 * s * v);
+            }
+        }
+    }
+    if (a.do_k) {
+        // K[a][c] += s sum_bd v D[b][d]; K[a][d] += s sum_bc v D[b][c];
+        // K[b][c] += s sum_ad v D[a][d]; K[b][d] += s sum_ac v D[a][c]
+        constexpr int n0 = NA * NC, n1 = NA * ND, n2 = NB * NC, n3 = NB * ND;
+        for (int t = lane; t < n0 + n1 + n2 + n3; t += 64) {
+            double v = 0.0;
+            if (t < n0) {
+                const int ai = t / NC, ci = t % NC;
+                for (int bi = 0; bi < NB; ++bi)
+                    for (int di = 0; di < ND; ++di) v = fma(B(ai, bi, ci, di), dbd[bi * ND + di], v);
+                unsafeAtomicAdd(Km + (A0 + ai) * M + C0 + ci, s * v);
+            } else if (t < n0 + n1) {
+                const int u = t - n0, ai = u / ND, di = u % ND;
+                for (int bi = 0; bi < NB; ++bi)
+                    for (int ci = 0; ci < NC; ++ci) v = fma(B(ai, bi, ci, di), dbc[bi * NC + ci], v);
+                unsafeAtomicAdd(Km + (A0 + ai) * M + D0 + di, s * v);
+            } else if (t < n0 + n1 + n2) {
+                const int u = t - n0 - n1, bi = u / NC, ci = u % NC;
+                for (int ai = 0; ai < NA; ++ai)
+                    for (int di = 0; di < ND; ++di) v = fma(B(ai, bi, ci, di), dad[ai * ND + di], v);
+                unsafeAtomicAdd(Km + (B0 + bi) * M + C0 + ci, s * v);
+            } else {
+                const int u = t - n0 - n1 - n2, bi = u / ND, di = u % ND;
+                for (int ai = 0; ai < NA; ++ai)
+                    for (int ci = 0; ci < NC; ++ci) v = fma(B(ai, bi, ci, di), dac[ai * NC + ci], v);
+                unsafeAtomicAdd(Km + (B0 + bi) * M + D0 + di, s * v);
+            }
+        }
+    }
+    __syncthreads();  // the LDS block is reused by the next quartet
+}
+
+// One wave (= one block of 64 threads) per shell quartet: work item w is ket
+// w % 64 of task w / 64.  Same task formats, counters, molecule offsets and
+// persistent stride (over work items) as the lane evaluator; FP64 only.
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(64) void k_eri_team(JKArgs a) {
+    using T = TeamShape<LA, LB, LC, LD>;
+    __shared__ double lds[T::LDS];
+    const int lane = threadIdx.x & 63;
+    const long long w0 = blockIdx.x;
+    const long long nwork = static_cast<long long>(a.ntasks) * 64;
+    const long long end = a.stride > 0 ? nwork : w0 + 1;
+    const long long step = a.stride > 0 ? a.stride : 1;
+    for (long long w = w0; w < end; w += step) {
+        const int task = static_cast<int>(w >> 6), k = static_cast<int>(w & 63);
+        if (task >= a.ntasks) return;
+        const int4 t = a.tasks[task];
+        const int P = __builtin_amdgcn_readfirstlane(t.x);
+        int Q, nq;
+        if (a.qlist) {
+            const long long off = static_cast<long long>(static_cast<unsigned>(__builtin_amdgcn_readfirstlane(t.y))) |
+                                  (static_cast<long long>(__builtin_amdgcn_readfirstlane(t.z) >> 8) << 32);
+            nq = __builtin_amdgcn_readfirstlane(t.z) & 255;
+            if (k >= nq) continue;
+            Q = __builtin_amdgcn_readfirstlane(a.qlist[off + k]);
+        } else {
+            if ((__builtin_amdgcn_readfirstlane(t.w) & 0xffff) != 0) continue;  // slice 0 takes all primitives
+            const int qb = __builtin_amdgcn_readfirstlane(t.y);
+            int qe = __builtin_amdgcn_readfirstlane(t.z);
+            if (a.cut) qe = min(qe, a.cut[6 * P]);
+            nq = min(qe - qb, 64);
+            if (k >= nq) continue;
+            Q = qb + k;
+        }
+        long M = a.M;
+        const double* __restrict__ Dm = a.D;
+        double* __restrict__ Jm = a.J;
+        double* __restrict__ Km = a.K;
+        if (a.pmol) {
+            const int mol = __builtin_amdgcn_readfirstlane(a.pmol[P]);
+            const long long base = a.mbase[mol];
+            M = a.mM[mol];
+            Dm += base;
+            Jm += base;
+            Km += base;
+        }
+        if (a.counter && lane == 0) {
+            atomicAdd(a.counter, 1ull);
+            atomicAdd(a.counter + 1, static_cast<unsigned long long>(a.pd.pn[Q]) * a.pd.pn[P]);
+        }
+        eri_team_quartet<LA, LB, LC, LD>(a, lds, P, Q, Jm, Km, Dm, M);
+    }
+}
+
+}  // namespace tlno_amd

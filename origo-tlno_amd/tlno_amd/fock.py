@@ -150,6 +150,20 @@ class DirectPlan:
                                                  K.ctypes.data_as(_lib.D) if want_k else None, float(thr)))
         return J, K
 
+    def set_team_threshold(self, min_outputs):
+        """FP64 classes with >= min_outputs Cartesian values per shell quartet
+        use the one-wave-per-quartet evaluator (0: lane evaluator everywhere)."""
+        _lib.check(_lib.load().tlno_gpu_set_team_threshold(self._h, int(min_outputs)))
+
+    def autotune(self, schwarz=None, repeats=2):
+        """Pick the faster of the lane and team evaluators per class pair on
+        this plan; returns the timings and choices."""
+        import json
+        thr = self.options.integral_threshold if schwarz is None else schwarz
+        buf = C.create_string_buffer(8192)
+        _lib.check(_lib.load().tlno_gpu_plan_autotune(self._h, float(thr), int(repeats), buf, len(buf)))
+        return json.loads(buf.value.decode())
+
     def set_mixed_precision(self, fp32_threshold):
         """Route shell quartets with Q_ab Q_cd max|D| < fp32_threshold to the
         FP32 recurrences (0 disables); J/K accumulation stays FP64."""

@@ -166,3 +166,28 @@ def test_onee(small_organic, cartesian):
         d = np.sqrt(np.abs(np.diag(r)))
         ok, worst = _close(x, r, floor=1e-13 * np.outer(d, d))
         assert ok, worst
+
+
+@pytest.mark.parametrize("spherical", [False, True])
+def test_team_evaluator_matches_lane_evaluator(spherical):
+    """The one-wave-per-quartet evaluator (HRR after contraction) against the
+    one-lane-per-quartet evaluator and the oracle, every class with d shells."""
+    from tlno_amd import configs
+    from tlno_amd.fock import DirectPlan, ScreeningOptions
+    b, D = configs.organic30(natoms=6, box_A=5.0)
+    plan = DirectPlan(b, ScreeningOptions(integral_threshold=0.0), cartesian=not spherical)
+    Dx = D if not spherical else configs.spherical_density(plan.M, 4, 10)
+    plan.set_team_threshold(0)
+    J0, K0 = plan.jk(Dx)
+    plan.set_team_threshold(1)  # every class with a team kernel
+    J1, K1 = plan.jk(Dx)
+    plan.set_team_threshold(216)
+    assert np.abs(J1 - J0).max() < 1e-11 and np.abs(K1 - K0).max() < 1e-11
+    if not spherical:
+        from oracle import oracle as O
+        Jo, Ko = O.OracleBasis(b).jk(D, spherical=False, schwarz=0.0)
+        assert np.abs(J1 - Jo).max() < 1e-10 and np.abs(K1 - Ko).max() < 1e-10
+    rep = plan.autotune(schwarz=0.0, repeats=1)
+    assert rep and all(v["choice"] in ("lane", "team") for v in rep.values())
+    J2, K2 = plan.jk(Dx)
+    assert np.abs(J2 - J0).max() < 1e-11 and np.abs(K2 - K0).max() < 1e-11
