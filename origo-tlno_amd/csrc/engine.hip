@@ -1134,6 +1134,17 @@ void build_plan(tlno_gpu_plan* pl, int n, const double* centers, const int* ang,
     }
 }
 
+// Waves per task for a class whose tasks should be split `split` ways: a task
+// with nq <= 32 kets already spreads its bra primitive pairs over the 64 / ks
+// lanes of each ket (ks = nq rounded up to a power of two; device side
+// ket_stride), so it needs that many times fewer waves.
+int wave_split(int split, int nq) {
+    int ks = 1;
+    while (ks < nq && ks < 64) ks <<= 1;
+    const int lanes_per_ket = 64 / ks;
+    return (split + lanes_per_ket - 1) / lanes_per_ket;
+}
+
 TaskSet& get_tasks(tlno_gpu_plan* pl, double thr) {
     auto it = pl->tasks.find(thr);
     if (it != pl->tasks.end()) return it->second;
@@ -1181,7 +1192,7 @@ TaskSet& get_tasks(tlno_gpu_plan* pl, double thr) {
                 std::vector<int4> cls;
                 for (const int4& t : base) {
                     const int pbn = pl->pn_host[t.x];
-                    const int s = std::min(split, pbn);
+                    const int s = std::max(1, std::min(pbn, wave_split(split, std::min(t.z - t.y, 64))));
                     const int step = (pbn + s - 1) / s;
                     for (int i0 = 0; i0 < pbn; i0 += step) {
                         const int i1 = std::min(pbn, i0 + step);
@@ -2505,7 +2516,7 @@ tlno_gpu_batch::Tasks& batch_tasks(tlno_gpu_batch* bt, double thr) {
                 split = static_cast<int>(std::min<long long>(256, (kTargetWaves + waves - 1) / waves));
             for (const int4& t : base) {
                 const int pbn = bt->pn[t.x];
-                const int s = std::max(1, std::min(split, pbn));
+                const int s = std::max(1, std::min(pbn, wave_split(split, std::min(t.z - t.y, 64))));
                 const int step = (pbn + s - 1) / s;
                 for (int i0 = 0; i0 < pbn; i0 += step)
                     all.push_back(make_int4(t.x, t.y, t.z, i0 | (std::min(pbn, i0 + step) << 16)));

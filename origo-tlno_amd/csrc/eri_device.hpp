@@ -3,9 +3,10 @@
 // One wave (64 lanes) owns one task: a fixed bra shell pair P and up to 64
 // consecutive ket shell pairs Q of one class, Schwarz-sorted so that the
 // surviving kets of P form a prefix.  Each lane evaluates one contracted
-// shell quartet (P|Q) entirely in registers:
+// shell quartet (P|Q) entirely in registers (a task with at most 32 kets
+// gives each ket several lanes, which split its bra primitive pairs):
 //
-//   for bra primitive pair (wave-uniform) x ket primitive pair (lane):
+//   for bra primitive pair x ket primitive pair:
 //     T = rho |PQ|^2, Rys roots/weights (Boys for N = 1, piecewise
 //     polynomials + Hermite asymptote for N >= 2),
 //     per root and axis: 2D VRR G[e][f], ket HRR, bra HRR (all unrolled at
@@ -19,7 +20,7 @@
 //
 // Classes whose full Cartesian block would not fit in registers are split
 // into compile-time "chunks": a fixed bra component AC and/or ket component
-// CC per kernel instance, so a lane accumulates at most 81 outputs.
+// CC per kernel instance, so a lane accumulates at most 108 outputs.
 //
 // The per-primitive-quartet formulas follow the reference's generic Rys path
 // (integrals/two_electron/rys/rys_engine.ipp:13540-13628, HRR 3155-3363):
@@ -279,10 +280,18 @@ struct Shape {
 
 constexpr double kTwoPi52 = 34.986836655249725693;  // 2 pi^{5/2}
 
-// Accumulates the lane's block slice (unit-diagonal components) into acc.
+// Classes whose per-primitive work is large enough to pay for per-lane
+// primitive indexing walk the flattened (ket primitive, bra primitive) grid.
+__host__ __device__ constexpr bool flat_prims(int LA, int LB, int LC, int LD) { return LA + LB + LC + LD >= 4; }
+
+// Accumulates the lane's block slice (unit-diagonal components) into acc over
+// its share of the primitive quartets: bra primitive pairs [ib0, ib1) of P
+// (a task's slice) times all ket primitive pairs of Q, of which the lane takes
+// every js-th starting at j0 -- of the flattened grid (bra index fastest) for
+// flat_prims classes, else of the bra pairs (with every ket pair).
 template <typename R, int LA, int LB, int LC, int LD, int AC, int CC, int CN>
 __device__ __forceinline__ void quartet(const DevTables& tb, const PairData& pd, int P, int Q, int ib0, int ib1,
-                                        R (&acc)[Shape<LA, LB, LC, LD, AC, CC, CN>::NOUT]) {
+                                        int j0, int js, R (&acc)[Shape<LA, LB, LC, LD, AC, CC, CN>::NOUT]) {
     using S = Shape<LA, LB, LC, LD, AC, CC, CN>;
     constexpr CartT<LA> ca = make_cart<LA>();
     constexpr CartT<LB> cb = make_cart<LB>();
@@ -295,63 +304,78 @@ __device__ __forceinline__ void quartet(const DevTables& tb, const PairData& pd,
     const R ABx = R(pd.AB[3 * P]), ABy = R(pd.AB[3 * P + 1]), ABz = R(pd.AB[3 * P + 2]);
     const double Cx = pd.A[3 * Q], Cy = pd.A[3 * Q + 1], Cz = pd.A[3 * Q + 2];
     const R CDx = R(pd.AB[3 * Q]), CDy = R(pd.AB[3 * Q + 1]), CDz = R(pd.AB[3 * Q + 2]);
-    // bra primitive pairs [ib0, ib1) of P (a task may own a slice of them;
-    // the block is linear in the primitives, so slices digest independently)
+    // the block is linear in the primitives, so the shares of a quartet held
+    // by different lanes or waves digest independently
     const int pb0 = pd.poff[P];
     const int pk0 = pd.poff[Q], pkn = pd.pn[Q];
 
-    // ket primitives (per lane) outside, bra primitives inside: in the J/K
-    // kernel P is wave-uniform, so the inner loads are scalar-cache loads and
-    // each lane reads its ket primitive data once
-    for (int ik = 0; ik < pkn; ++ik) {
-        const double4 Qp = pd.pP[pk0 + ik];
-        const double wcd = pd.pw[pk0 + ik];
+    auto prim = [&](const double4 Pp, const double wab, const double4 Qp, const double wcd) {
         const double q = Qp.w;
-        for (int ib = ib0; ib < ib1; ++ib) {
-            const double4 Pp = pd.pP[pb0 + ib];
-            const double wab = pd.pw[pb0 + ib];
-            const double p = Pp.w;
-            const double pq = p + q;
-            const double ipq = 1.0 / pq;
-            const double PQx = Pp.x - Qp.x, PQy = Pp.y - Qp.y, PQz = Pp.z - Qp.z;
-            const double T = p * q * ipq * (PQx * PQx + PQy * PQy + PQz * PQz);
-            const double pref = kTwoPi52 * wab * wcd * sqrt(ipq);
-            double rt[S::NR], wt[S::NR];
-            rys_roots<S::NR>(tb, T, rt, wt);
-            const R qo = R(q * ipq), po = R(p * ipq);
-            const R hpq = R(0.5 * ipq), i2p = R(0.5 / p), i2q = R(0.5 / q);
-            const R PAx = R(Pp.x - Ax), PAy = R(Pp.y - Ay), PAz = R(Pp.z - Az);
-            const R QCx = R(Qp.x - Cx), QCy = R(Qp.y - Cy), QCz = R(Qp.z - Cz);
-            const R pqx = R(PQx), pqy = R(PQy), pqz = R(PQz);
+        const double p = Pp.w;
+        const double pq = p + q;
+        const double ipq = 1.0 / pq;
+        const double PQx = Pp.x - Qp.x, PQy = Pp.y - Qp.y, PQz = Pp.z - Qp.z;
+        const double T = p * q * ipq * (PQx * PQx + PQy * PQy + PQz * PQz);
+        const double pref = kTwoPi52 * wab * wcd * sqrt(ipq);
+        double rt[S::NR], wt[S::NR];
+        rys_roots<S::NR>(tb, T, rt, wt);
+        const R qo = R(q * ipq), po = R(p * ipq);
+        const R hpq = R(0.5 * ipq), i2p = R(0.5 / p), i2q = R(0.5 / q);
+        const R PAx = R(Pp.x - Ax), PAy = R(Pp.y - Ay), PAz = R(Pp.z - Az);
+        const R QCx = R(Qp.x - Cx), QCy = R(Qp.y - Cy), QCz = R(Qp.z - Cz);
+        const R pqx = R(PQx), pqy = R(PQy), pqz = R(PQz);
 #pragma unroll
-            for (int r = 0; r < S::NR; ++r) {
-                const R t2 = R(rt[r]);
-                const R B00 = hpq * t2;
-                const R B10 = i2p * (R(1) - qo * t2);
-                const R B01 = i2q * (R(1) - po * t2);
-                const R qt = qo * t2, pt = po * t2;
-                R Ix[LA + 1][LB + 1][LC + 1][LD + 1];
-                R Iy[LA + 1][LB + 1][LC + 1][LD + 1];
-                R Iz[LA + 1][LB + 1][LC + 1][LD + 1];
-                axis_table<R, LA, LB, LC, LD>(PAx - qt * pqx, QCx + pt * pqx, B00, B10, B01, R(1), ABx, CDx, Ix);
-                axis_table<R, LA, LB, LC, LD>(PAy - qt * pqy, QCy + pt * pqy, B00, B10, B01, R(1), ABy, CDy, Iy);
-                axis_table<R, LA, LB, LC, LD>(PAz - qt * pqz, QCz + pt * pqz, B00, B10, B01, R(pref * wt[r]), ABz,
-                                              CDz, Iz);
-                int o = 0;
+        for (int r = 0; r < S::NR; ++r) {
+            const R t2 = R(rt[r]);
+            const R B00 = hpq * t2;
+            const R B10 = i2p * (R(1) - qo * t2);
+            const R B01 = i2q * (R(1) - po * t2);
+            const R qt = qo * t2, pt = po * t2;
+            R Ix[LA + 1][LB + 1][LC + 1][LD + 1];
+            R Iy[LA + 1][LB + 1][LC + 1][LD + 1];
+            R Iz[LA + 1][LB + 1][LC + 1][LD + 1];
+            axis_table<R, LA, LB, LC, LD>(PAx - qt * pqx, QCx + pt * pqx, B00, B10, B01, R(1), ABx, CDx, Ix);
+            axis_table<R, LA, LB, LC, LD>(PAy - qt * pqy, QCy + pt * pqy, B00, B10, B01, R(1), ABy, CDy, Iy);
+            axis_table<R, LA, LB, LC, LD>(PAz - qt * pqz, QCz + pt * pqz, B00, B10, B01, R(pref * wt[r]), ABz,
+                                          CDz, Iz);
+            int o = 0;
 #pragma unroll
-                for (int ai = 0; ai < S::NAL; ++ai)
+            for (int ai = 0; ai < S::NAL; ++ai)
 #pragma unroll
-                    for (int b = 0; b < S::NB; ++b)
+                for (int b = 0; b < S::NB; ++b)
 #pragma unroll
-                        for (int ci = 0; ci < S::NCL; ++ci)
+                    for (int ci = 0; ci < S::NCL; ++ci)
 #pragma unroll
-                            for (int d = 0; d < S::ND; ++d, ++o) {
-                                const int a = S::acomp(ai), c = S::ccomp(ci);
-                                const R xy = Ix[ca.e[0][a]][cb.e[0][b]][cc.e[0][c]][cd.e[0][d]] *
-                                             Iy[ca.e[1][a]][cb.e[1][b]][cc.e[1][c]][cd.e[1][d]];
-                                acc[o] = fma(xy, Iz[ca.e[2][a]][cb.e[2][b]][cc.e[2][c]][cd.e[2][d]], acc[o]);
-                            }
+                        for (int d = 0; d < S::ND; ++d, ++o) {
+                            const int a = S::acomp(ai), c = S::ccomp(ci);
+                            const R xy = Ix[ca.e[0][a]][cb.e[0][b]][cc.e[0][c]][cd.e[0][d]] *
+                                         Iy[ca.e[1][a]][cb.e[1][b]][cc.e[1][c]][cd.e[1][d]];
+                            acc[o] = fma(xy, Iz[ca.e[2][a]][cb.e[2][b]][cc.e[2][c]][cd.e[2][d]], acc[o]);
+                        }
+        }
+    };
+    if constexpr (flat_prims(LA, LB, LC, LD)) {
+        const int nb = ib1 - ib0;
+        if (nb > 0) {
+            const int ntot = pkn * nb;
+            int ik = j0 / nb, ib = j0 - ik * nb;
+            const int dk = js / nb, db = js - dk * nb;
+            for (int j = j0; j < ntot; j += js) {
+                prim(pd.pP[pb0 + ib0 + ib], pd.pw[pb0 + ib0 + ib], pd.pP[pk0 + ik], pd.pw[pk0 + ik]);
+                ik += dk;
+                ib += db;
+                if (ib >= nb) {
+                    ib -= nb;
+                    ++ik;
+                }
             }
+        }
+    } else {
+        // ket primitives outside: each lane reads its ket primitive data once
+        for (int ik = 0; ik < pkn; ++ik) {
+            const double4 Qp = pd.pP[pk0 + ik];
+            const double wcd = pd.pw[pk0 + ik];
+            for (int ib = ib0 + j0; ib < ib1; ib += js) prim(pd.pP[pb0 + ib], pd.pw[pb0 + ib], Qp, wcd);
         }
     }
     // unit-diagonal normalization of each component
@@ -416,8 +440,19 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// Lanes per ket of a task with nq kets: tasks with at most 32 kets give
+// each ket ks lanes' worth of room (ks = nq rounded up to a power of two)
+// and the 64 / ks lanes of a ket split its bra primitive pairs by stride.
+__device__ __forceinline__ int ket_stride(int nq) {
+    int ks = 1;
+    while (ks < nq && ks < 64) ks <<= 1;
+    return ks;
+}
+
 // One work item of a wave.  Task = {P, q_begin, q_end, bra primitive slice
-// ib0 | ib1 << 16} (range mode) or a list-mode task (see JKArgs).
+// ib0 | ib1 << 16} (range mode) or a list-mode task (see JKArgs).  Lane l
+// takes ket l % ks of the task (ks = ket_stride(kets)) and share l / ks of
+// the 64 / ks shares of its primitive quartets (see quartet).
 template <typename R, int LA, int LB, int LC, int LD, int AC, int CC, int CN>
 __device__ __forceinline__ void eri_jk_task(const JKArgs& a, const int wave) {
     using S = Shape<LA, LB, LC, LD, AC, CC, CN>;
@@ -428,7 +463,7 @@ __device__ __forceinline__ void eri_jk_task(const JKArgs& a, const int wave) {
     // the task is wave-uniform: keep it in scalar registers
     const int P = __builtin_amdgcn_readfirstlane(t.x);
     bool active;
-    int Q, ib0, ib1, nq;
+    int Q, ib0, ib1, nq, ks;
     if (a.qlist) {
         // list mode: up to 64 kets at qlist[offset ..], bra primitive slice
         // (wave % split) of split near-equal slices
@@ -443,23 +478,31 @@ __device__ __forceinline__ void eri_jk_task(const JKArgs& a, const int wave) {
         ib0 = slice * step;
         ib1 = min(pbn, ib0 + step);
         if (ib0 >= ib1) return;
-        active = lane < nq;
-        Q = a.qlist[off + (active ? lane : nq - 1)];
+        ks = ket_stride(nq);
+        const int kq = lane & (ks - 1);
+        active = kq < nq;
+        Q = a.qlist[off + (active ? kq : nq - 1)];
     } else {
         const int qb = __builtin_amdgcn_readfirstlane(t.y);
         int qe = __builtin_amdgcn_readfirstlane(t.z);
         const int tw = __builtin_amdgcn_readfirstlane(t.w);
         if (a.cut) qe = min(qe, a.cut[6 * P]);  // density screen (per bra pair, this ket class)
         if (qe <= qb) return;                     // whole wave leaves together
-        const int qraw = qb + lane;
+        nq = min(qe - qb, 64);
+        ks = ket_stride(nq);
+        const int qraw = qb + (lane & (ks - 1));
         active = qraw < qe;
         Q = active ? qraw : qe - 1;
-        nq = min(qe - qb, 64);
         ib0 = tw & 0xffff;
         ib1 = (tw >> 16) & 0xffff;
     }
+    const int ibs = 64 / ks;    // lanes per ket
+    const int sub = lane / ks;  // this lane's share of the ket's primitive quartets
+    constexpr bool flat = flat_prims(LA, LB, LC, LD);
+    const int nshare = flat ? a.pd.pn[Q] * (ib1 - ib0) : ib1 - ib0;  // units the lanes of a ket split
+    active = active && sub < nshare;
     if (a.counter && AC <= 0 && CC <= 0) {  // the class's first output chunk counts
-        int npq = active ? a.pd.pn[Q] * (ib1 - ib0) : 0;
+        int npq = active ? (nshare - sub + ibs - 1) / ibs * (flat ? 1 : a.pd.pn[Q]) : 0;
 #pragma unroll
         for (int m = 32; m >= 1; m >>= 1) npq += __shfl_xor(npq, m, 64);
         if (lane == 0) {
@@ -474,7 +517,18 @@ __device__ __forceinline__ void eri_jk_task(const JKArgs& a, const int wave) {
     }
 
     R acc[S::NOUT];
-    quartet<R, LA, LB, LC, LD, AC, CC, CN>(a.tb, a.pd, P, Q, ib0, ib1, acc);
+    quartet<R, LA, LB, LC, LD, AC, CC, CN>(a.tb, a.pd, P, Q, ib0, active ? ib1 : ib0, sub, ibs, acc);
+    if (ibs > 1) {
+        // the ibs lanes of a ket (lane bits above ks) hold parts of one block:
+        // sum them into the first, which alone digests
+#pragma unroll
+        for (int o = 0; o < S::NOUT; ++o) {
+            R v = acc[o];
+            for (int m = ks; m < 64; m <<= 1) v += __shfl_xor(v, m, 64);
+            acc[o] = v;
+        }
+        active = (lane < ks) && (lane < nq);
+    }
 
     if (a.cw) {
         // canonical-stream checksum: the weights fold the spherical projection
@@ -639,7 +693,7 @@ __device__ __forceinline__ void eri_store_body(const StoreArgs& a) {
     if (i >= a.n) return;
     const int2 pq = a.quartets[i];
     double acc[S::NOUT];
-    quartet<double, LA, LB, LC, LD, AC, CC, CN>(a.tb, a.pd, pq.x, pq.y, 0, a.pd.pn[pq.x], acc);
+    quartet<double, LA, LB, LC, LD, AC, CC, CN>(a.tb, a.pd, pq.x, pq.y, 0, a.pd.pn[pq.x], 0, 1, acc);
     double* out = a.out + a.off[i];
     int o = 0;
 #pragma unroll
