@@ -526,10 +526,11 @@ inline int class_team_min(const tlno_gpu_plan* pl, int X, int Y);
 void launch_jk(const ClassKernels& k, bool fp32, JKArgs& args, long long nwaves, hipStream_t st,
                int team_min_out = 0) {
     void* params[] = {&args};
-    if (use_team(k, fp32, args, team_min_out)) {  // one block per shell quartet (64 per task)
+    if (use_team(k, fp32, args, team_min_out)) {  // one block per shell quartet and slice (64 per task)
         args.stride = 0;
+        const long long ns = std::max(1, args.team_slices);
         HIP_OK(hipLaunchKernel(reinterpret_cast<const void*>(k.team),
-                               dim3(static_cast<unsigned>(static_cast<long long>(args.ntasks) * 64)), dim3(64),
+                               dim3(static_cast<unsigned>(static_cast<long long>(args.ntasks) * 64 * ns)), dim3(64),
                                params, 0, st));
         return;
     }
@@ -1235,6 +1236,21 @@ inline int class_team_min(const tlno_gpu_plan* pl, int X, int Y) {
     return sel < 0 ? pl->team_min_out : (sel ? 1 : 0);
 }
 
+// Team evaluator waves per shell quartet for a class with nq quartets: a class
+// too small to give every SIMD a few waves splits each quartet's primitive
+// combinations over several waves (each repeats the transfer and digestion).
+#ifndef TLNO_TEAM_TARGET_WAVES
+#define TLNO_TEAM_TARGET_WAVES 4096
+#endif
+#ifndef TLNO_TEAM_MAX_SLICES
+#define TLNO_TEAM_MAX_SLICES 8
+#endif
+int team_slices_for(long long nq) {
+    constexpr long long kTeamTargetWaves = TLNO_TEAM_TARGET_WAVES;
+    if (nq <= 0 || nq >= kTeamTargetWaves) return 1;
+    return static_cast<int>(std::min<long long>(TLNO_TEAM_MAX_SLICES, (kTeamTargetWaves + nq - 1) / nq));
+}
+
 // Candidate rows for the per-quartet routing kernel (once per threshold):
 // every Schwarz-surviving (P | ket range) split into rows of <= kRouteSeg
 // kets, each owning a qlist region of its length; per class pair the task
@@ -1426,6 +1442,7 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
                     a.tasks = pl->d_t64 + ts.toff[xy];
                     a.ntasks = static_cast<int>(n64);
                     a.split = use_team(k, false, a, class_team_min(pl, X, Y)) ? 1 : list_split(n64, k.nchunk);
+                    a.team_slices = team_slices_for(ts.cls_sq[X][Y]);
                     launch_jk(k, false, a, n64 * a.split, cs, class_team_min(pl, X, Y));
                 }
                 if (n32 > 0) {
@@ -1460,6 +1477,7 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
                 if (n64 > 0) {
                     a.tasks = ts.d_tasks + ts.off[X][Y];
                     a.ntasks = static_cast<int>(n64);
+                    a.team_slices = team_slices_for(ts.cls_sq[X][Y]);
                     launch_jk(k, false, a, n64, cs, class_team_min(pl, X, Y));
                 }
                 if (nt > n64) {

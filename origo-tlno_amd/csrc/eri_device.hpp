@@ -423,6 +423,9 @@ struct JKArgs {
     // w, w + stride, ... below nwork (range mode: tasks; list mode: tasks x split)
     int stride;
     int nwork;
+    // team evaluator: waves per shell quartet, each taking a slice of its
+    // primitive combinations (0 or 1: one wave)
+    int team_slices;
 };
 
 struct StoreArgs {

@@ -74,7 +74,7 @@ __device__ inline void tri_decode(int LMIN, int p, int& x, int& y, int& z) {
 template <int LA, int LB, int LC, int LD>
 __device__ __forceinline__ void eri_team_quartet(const JKArgs& a, double* __restrict__ lds, int P, int Q,
                                                  double* __restrict__ Jm, double* __restrict__ Km,
-                                                 const double* __restrict__ Dm, long M) {
+                                                 const double* __restrict__ Dm, long M, int slice, int ns) {
     using T = TeamShape<LA, LB, LC, LD>;
     constexpr CartT<LA> ca = make_cart<LA>();
     constexpr CartT<LB> cb = make_cart<LB>();
@@ -104,9 +104,16 @@ __device__ __forceinline__ void eri_team_quartet(const JKArgs& a, double* __rest
     const double Cx = pd.A[3 * Q], Cy = pd.A[3 * Q + 1], Cz = pd.A[3 * Q + 2];
     const int pb0 = pd.poff[P], pbn = pd.pn[P];
     const int pk0 = pd.poff[Q], pkn = pd.pn[Q];
-    const int ncomb = pkn * pbn * T::NR;
+    // slice `slice` of ns of the (ket primitive, bra primitive, root)
+    // combinations, in whole batches; the block is linear in them, so the
+    // slices of a quartet digest independently
+    const int ntot = pkn * pbn * T::NR;
+    const int per = ((ntot + ns - 1) / ns + T::CB - 1) / T::CB * T::CB;
+    const int cbeg = slice * per;
+    const int ncomb = min(ntot, cbeg + per);
+    if (cbeg >= ncomb) return;  // the whole block leaves together
 
-    for (int c0 = 0; c0 < ncomb; c0 += T::CB) {
+    for (int c0 = cbeg; c0 < ncomb; c0 += T::CB) {
         // ---- phase A: one combination per lane (lanes >= CB idle)
         const int c = c0 + lane;
         if (lane < T::CB && c < ncomb) {
@@ -276,8 +283,9 @@ __device__ __forceinline__ void eri_team_quartet(const JKArgs& a, double* __rest
     __syncthreads();  // the LDS block is reused by the next quartet
 }
 
-// One wave (= one block of 64 threads) per shell quartet: work item w is ket
-// w % 64 of task w / 64.  Same task formats, counters, molecule offsets and
+// One wave (= one block of 64 threads) per shell quartet and combination
+// slice: work item w is slice w % S of ket (w / S) % 64 of task w / (64 S)
+// (S = team_slices).  Same task formats, counters, molecule offsets and
 // persistent stride (over work items) as the lane evaluator; FP64 only.
 template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(64) void k_eri_team(JKArgs a) {
@@ -285,11 +293,14 @@ __global__ __launch_bounds__(64) void k_eri_team(JKArgs a) {
     __shared__ double lds[T::LDS];
     const int lane = threadIdx.x & 63;
     const long long w0 = blockIdx.x;
-    const long long nwork = static_cast<long long>(a.ntasks) * 64;
+    const int ns = a.team_slices > 1 ? a.team_slices : 1;
+    const long long nwork = static_cast<long long>(a.ntasks) * 64 * ns;
     const long long end = a.stride > 0 ? nwork : w0 + 1;
     const long long step = a.stride > 0 ? a.stride : 1;
     for (long long w = w0; w < end; w += step) {
-        const int task = static_cast<int>(w >> 6), k = static_cast<int>(w & 63);
+        const int slice = static_cast<int>(w % ns);
+        const long long item = w / ns;
+        const int task = static_cast<int>(item >> 6), k = static_cast<int>(item & 63);
         if (task >= a.ntasks) return;
         const int4 t = a.tasks[task];
         const int P = __builtin_amdgcn_readfirstlane(t.x);
@@ -321,11 +332,11 @@ __global__ __launch_bounds__(64) void k_eri_team(JKArgs a) {
             Jm += base;
             Km += base;
         }
-        if (a.counter && lane == 0) {
+        if (a.counter && lane == 0 && slice == 0) {
             atomicAdd(a.counter, 1ull);
             atomicAdd(a.counter + 1, static_cast<unsigned long long>(a.pd.pn[Q]) * a.pd.pn[P]);
         }
-        eri_team_quartet<LA, LB, LC, LD>(a, lds, P, Q, Jm, Km, Dm, M);
+        eri_team_quartet<LA, LB, LC, LD>(a, lds, P, Q, Jm, Km, Dm, M, slice, ns);
     }
 }
 
