@@ -86,6 +86,26 @@ struct DevTables {
     double herm_w[RYS_NMAX + 1][RYS_NMAX];
 };
 
+// Reciprocal and reciprocal square root of a positive normal double: the
+// hardware estimates (v_rcp_f64, v_rsq_f64) refined by two Newton-Raphson
+// steps each, r <- r + r (1 - x r) and y <- y + (y/2)(1 - x y^2); the error
+// of each step is about the square of the previous one, which leaves a few
+// ulp.  They replace the IEEE-exact division and square root sequences in
+// the per-primitive work (every argument there is a sum of Gaussian
+// exponents or a Boys argument T >= 40).
+__device__ __forceinline__ double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return r;
+}
+__device__ __forceinline__ double rsqrt_nr(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    y = fma(0.5 * y, fma(-x * y, y, 1.0), y);
+    y = fma(0.5 * y, fma(-x * y, y, 1.0), y);
+    return y;
+}
+
 // F_0..F_MMAX at T
 template <int MMAX>
 __device__ __forceinline__ void boys_all(const DevTables& tb, double T, double* F) {
@@ -101,8 +121,9 @@ __device__ __forceinline__ void boys_all(const DevTables& tb, double T, double* 
             F[m] = v;
         }
     } else {
-        const double iT = 1.0 / T;
-        double f = 0.88622692545275801365 * sqrt(iT);  // sqrt(pi)/2
+        const double isT = rsqrt_nr(T);
+        const double iT = isT * isT;
+        double f = 0.88622692545275801365 * isT;  // sqrt(pi)/2
         F[0] = f;
         if (MMAX > 0) {
             const double ex = exp(-T);
@@ -121,7 +142,7 @@ __device__ __forceinline__ void rys_roots(const DevTables& tb, double T, double*
     if constexpr (N == 1) {
         double F[2];
         boys_all<1>(tb, T, F);
-        r[0] = F[1] / F[0];
+        r[0] = F[1] * rcp_nr(F[0]);
         w[0] = F[0];
     } else {
         constexpr int ttab[6] = {0, 41, 49, 55, 61, 67};
@@ -139,8 +160,8 @@ __device__ __forceinline__ void rys_roots(const DevTables& tb, double T, double*
                 else w[v - N] = s;
             }
         } else {
-            const double iT = 1.0 / T;
-            const double isT = sqrt(iT);
+            const double isT = rsqrt_nr(T);
+            const double iT = isT * isT;
 #pragma unroll
             for (int i = 0; i < N; ++i) {
                 r[i] = tb.herm_r2[N][i] * iT;
@@ -170,14 +191,14 @@ struct RootEval {
                 u = T - k - 0.5;
                 c = tb.rys + tb.rys_off[N] + k * (2 * N * RYS_NCOEF);
             } else {
-                iT = 1.0 / T;
-                isT = sqrt(iT);
+                isT = rsqrt_nr(T);
+                iT = isT * isT;
             }
         }
     }
     __device__ __forceinline__ void get(const DevTables& tb, int r, double& t2, double& w) const {
         if constexpr (N == 1) {
-            t2 = F1 / F0;
+            t2 = F1 * rcp_nr(F0);
             w = F0;
         } else {
             if (c) {
@@ -313,14 +334,15 @@ __device__ __forceinline__ void quartet(const DevTables& tb, const PairData& pd,
         const double q = Qp.w;
         const double p = Pp.w;
         const double pq = p + q;
-        const double ipq = 1.0 / pq;
+        const double sipq = rsqrt_nr(pq);
+        const double ipq = sipq * sipq;
         const double PQx = Pp.x - Qp.x, PQy = Pp.y - Qp.y, PQz = Pp.z - Qp.z;
         const double T = p * q * ipq * (PQx * PQx + PQy * PQy + PQz * PQz);
-        const double pref = kTwoPi52 * wab * wcd * sqrt(ipq);
+        const double pref = kTwoPi52 * wab * wcd * sipq;
         double rt[S::NR], wt[S::NR];
         rys_roots<S::NR>(tb, T, rt, wt);
         const R qo = R(q * ipq), po = R(p * ipq);
-        const R hpq = R(0.5 * ipq), i2p = R(0.5 / p), i2q = R(0.5 / q);
+        const R hpq = R(0.5 * ipq), i2p = R(0.5 * rcp_nr(p)), i2q = R(0.5 * rcp_nr(q));
         const R PAx = R(Pp.x - Ax), PAy = R(Pp.y - Ay), PAz = R(Pp.z - Az);
         const R QCx = R(Qp.x - Cx), QCy = R(Qp.y - Cy), QCz = R(Qp.z - Cz);
         const R pqx = R(PQx), pqy = R(PQy), pqz = R(PQz);

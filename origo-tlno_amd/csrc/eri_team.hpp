@@ -123,17 +123,18 @@ __device__ __forceinline__ void eri_team_quartet(const JKArgs& a, double* __rest
             const double4 Pp = pd.pP[pb0 + ib];
             const double4 Qp = pd.pP[pk0 + ik];
             const double p = Pp.w, q = Qp.w;
-            const double ipq = 1.0 / (p + q);
+            const double sipq = rsqrt_nr(p + q);
+            const double ipq = sipq * sipq;
             const double PQx = Pp.x - Qp.x, PQy = Pp.y - Qp.y, PQz = Pp.z - Qp.z;
             const double Tv = p * q * ipq * (PQx * PQx + PQy * PQy + PQz * PQz);
-            const double pref = kTwoPi52 * pd.pw[pb0 + ib] * pd.pw[pk0 + ik] * sqrt(ipq);
+            const double pref = kTwoPi52 * pd.pw[pb0 + ib] * pd.pw[pk0 + ik] * sipq;
             const RootEval<T::NR> re(a.tb, Tv);
             double t2, wr;
             re.get(a.tb, r, t2, wr);
             const double qo = q * ipq, po = p * ipq;
             const double B00 = 0.5 * ipq * t2;
-            const double B10 = (0.5 / p) * (1.0 - qo * t2);
-            const double B01 = (0.5 / q) * (1.0 - po * t2);
+            const double B10 = (0.5 * rcp_nr(p)) * (1.0 - qo * t2);
+            const double B01 = (0.5 * rcp_nr(q)) * (1.0 - po * t2);
             const double qt = qo * t2, pt = po * t2;
             double* g = lds + lane * T::CS;
             // one axis at a time (x, y, z), so a single 2D table is live
