@@ -268,7 +268,8 @@ def main():
         "data": "synthetic (seeded generator, tlno_amd.configs)",
         "config": {"workload": f"{args.workload}: direct J+K build, Cartesian d, Schwarz 1e-12",
                    "n_basis": plan.M, "shell_pairs": plan.profile.pair_groups,
-                   "primitive_pairs": plan.profile.primitive_pairs, "parallelism": "single GPU"},
+                   "primitive_pairs": plan.profile.primitive_pairs, "parallelism": "single GPU",
+                   "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))},
         "eri_throughput": {"shell_quartets_per_s": sq_tot / (wall / args.steps),
                            "primitive_quartets_per_s": pq_tot / (wall / args.steps),
                            "cartesian_integrals_per_s": ints_tot / (wall / args.steps),
@@ -294,5 +295,23 @@ def main():
     print(json.dumps(result), flush=True)
 
 
+HW_QUEUES = 8
+
+
+def relaunch_with_hw_queues():
+    """The plan runs the class kernels of a build on up to 8 concurrent
+    streams, which HIP spreads over GPU_MAX_HW_QUEUES hardware queues (4 by
+    default), read when the HIP runtime loads.  The benchmark runs with 8
+    queues: if the environment says otherwise it reruns itself as a child
+    process with GPU_MAX_HW_QUEUES=8 (nothing has touched the GPU yet) and
+    exits with the child's status."""
+    if os.environ.get("GPU_MAX_HW_QUEUES") == str(HW_QUEUES):
+        return
+    import subprocess
+    env = dict(os.environ, GPU_MAX_HW_QUEUES=str(HW_QUEUES))
+    sys.exit(subprocess.run([sys.executable, os.path.abspath(__file__)] + sys.argv[1:], env=env).returncode)
+
+
 if __name__ == "__main__":
+    relaunch_with_hw_queues()
     main()

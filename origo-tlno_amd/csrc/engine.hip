@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -214,8 +215,24 @@ struct GroupH {
     int sph_base = 0;
 };
 
-#define tlno_amd_side_streams 4
+// Class-pair kernels of one build run concurrently on up to this many
+// streams of the plan.  HIP maps a process's streams onto GPU_MAX_HW_QUEUES
+// hardware queues (4 unless the process environment sets the variable
+// before the HIP runtime loads); a plan uses min(that, kSideStreams) streams,
+// since more streams than queues only serialize behind each other.
+#ifndef tlno_amd_side_streams
+#define tlno_amd_side_streams 8
+#endif
 constexpr int kSideStreams = tlno_amd_side_streams;
+
+int side_streams_for_queues() {
+    int q = 4;  // HIP's default
+    if (const char* v = std::getenv("GPU_MAX_HW_QUEUES")) {
+        const int n = std::atoi(v);
+        if (n > 0) q = n;
+    }
+    return std::max(1, std::min(kSideStreams, q));
+}
 
 struct TaskSet {
     int4* d_tasks = nullptr;
@@ -272,6 +289,7 @@ struct tlno_gpu_plan {
     double *dJout = nullptr, *dKout = nullptr;
     hipStream_t stream = nullptr;
     hipStream_t side[tlno_amd_side_streams] = {};
+    int nside = 1;  // side streams in use
     hipEvent_t ev_fork = nullptr, ev_join[tlno_amd_side_streams] = {};
     std::map<double, tlno_amd::TaskSet> tasks;
     long long last_shell_quartets = 0, last_prim_quartets = 0;
@@ -922,7 +940,8 @@ void build_plan(tlno_gpu_plan* pl, int n, const double* centers, const int* ang,
     (void)ctx;
     HIP_OK(hipSetDevice(pl->device));
     HIP_OK(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking));
-    for (int s = 0; s < kSideStreams; ++s) {
+    pl->nside = side_streams_for_queues();
+    for (int s = 0; s < pl->nside; ++s) {
         HIP_OK(hipStreamCreateWithFlags(&pl->side[s], hipStreamNonBlocking));
         HIP_OK(hipEventCreateWithFlags(&pl->ev_join[s], hipEventDisableTiming));
     }
@@ -1425,7 +1444,7 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
         const bool concurrent = !pl->timing;
         if (concurrent) {
             HIP_OK(hipEventRecord(pl->ev_fork, st));
-            for (int s = 0; s < kSideStreams; ++s) HIP_OK(hipStreamWaitEvent(pl->side[s], pl->ev_fork, 0));
+            for (int s = 0; s < pl->nside; ++s) HIP_OK(hipStreamWaitEvent(pl->side[s], pl->ev_fork, 0));
         }
         int rr = 0;
         for (int X = 5; X >= 0 && route; --X)
@@ -1434,7 +1453,7 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
                 const long long n64 = tcount[xy], n32 = tcount[36 + xy];
                 if (n64 + n32 == 0) continue;
                 const ClassKernels& k = g_kernels[X][Y];
-                const hipStream_t cs = concurrent ? pl->side[rr++ % kSideStreams] : st;
+                const hipStream_t cs = concurrent ? pl->side[rr++ % pl->nside] : st;
                 if (pl->timing) HIP_OK(hipEventRecord(pl->ev[X][Y][0], cs));
                 a.cut = nullptr;
                 a.qlist = pl->d_qlist;
@@ -1448,7 +1467,7 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
                 if (n32 > 0) {
                     // the FP32 share goes to a stream of its own so that it
                     // overlaps the class's FP64 share
-                    const hipStream_t cs32 = (concurrent && n64 > 0) ? pl->side[rr++ % kSideStreams] : cs;
+                    const hipStream_t cs32 = (concurrent && n64 > 0) ? pl->side[rr++ % pl->nside] : cs;
                     a.tasks = pl->d_t32 + ts.toff[xy];
                     a.ntasks = static_cast<int>(n32);
                     a.split = list_split(n32, k.nchunk);
@@ -1471,7 +1490,7 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
                           bm.begin();
                 }
                 const ClassKernels& k = g_kernels[X][Y];
-                const hipStream_t cs = concurrent ? pl->side[rr++ % kSideStreams] : st;
+                const hipStream_t cs = concurrent ? pl->side[rr++ % pl->nside] : st;
                 if (pl->timing) HIP_OK(hipEventRecord(pl->ev[X][Y][0], cs));
                 a.cut = nullptr;  // density screening always takes the routed path
                 if (n64 > 0) {
@@ -1488,7 +1507,7 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
                 if (pl->timing) HIP_OK(hipEventRecord(pl->ev[X][Y][1], cs));
             }
         if (concurrent)
-            for (int s = 0; s < kSideStreams; ++s) {
+            for (int s = 0; s < pl->nside; ++s) {
                 HIP_OK(hipEventRecord(pl->ev_join[s], pl->side[s]));
                 HIP_OK(hipStreamWaitEvent(st, pl->ev_join[s], 0));
             }

@@ -22,6 +22,10 @@
 
 #include "eri_device.hpp"
 
+#ifndef TLNO_TEAM_CB
+#define TLNO_TEAM_CB 64
+#endif
+
 namespace tlno_amd {
 
 constexpr int ncart_cum(int L) { return (L + 1) * (L + 2) * (L + 3) / 6; }  // sum_{l <= L} ncart(l)
@@ -47,7 +51,7 @@ struct TeamShape {
     static constexpr int OPL = (NOUT + 63) / 64;
     static constexpr int GS = (LAB + 1) * (LCD + 1);       // one axis table
     static constexpr int CS = 3 * GS + 1;                  // one combination (padded)
-    static constexpr int CB = 32;                          // combinations per batch
+    static constexpr int CB = TLNO_TEAM_CB;                // combinations per batch
     static constexpr int LDS_A = CB * CS;
     static constexpr int LDS_C = NI + NOUT;
     static constexpr int LDS_D = NI + NOUT + NA * NB + NC * ND + NA * NC + NA * ND + NB * NC + NB * ND;
