@@ -246,6 +246,8 @@ def main():
         sys.exit(2)
     thr = 1e-12
     b, D = configs.organic30() if args.workload == "organic30" else configs.water40()
+    print(f"bench: {args.workload} J/K builds ({args.warmup} warm-up, {args.steps} timed)", file=sys.stderr,
+          flush=True)
     plan, wall, classes, J, K = gpu_run(b, D, args.steps, args.warmup, thr)
     ms_per = wall / args.steps * 1e3
     sq_tot = sum(c["shell_quartets"] for c in classes)
@@ -285,10 +287,13 @@ def main():
         "autotuner": {"seconds": plan.tuning_seconds, "classes": plan.tuning},
     }
     if not args.no_mixed:
+        print("bench: water40 mixed-precision and screened builds", file=sys.stderr, flush=True)
         result["mixed_precision"] = mixed_run()
     if not args.no_batch:
+        print("bench: 256-molecule batch builds", file=sys.stderr, flush=True)
         result["batched"] = batched_run()
     if not args.no_cpu:
+        print("bench: reference CPU build", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(b)
         if result["cpu_baseline"]["value"]:
             result["gpu_over_cpu_baseline"] = result["value"] / result["cpu_baseline"]["value"]

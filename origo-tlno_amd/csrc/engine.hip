@@ -46,12 +46,50 @@ namespace tlno_amd {
 thread_local std::string g_err;
 
 // ------------------------------------------------------------- device ctx --
+// Class-pair kernels of one build run concurrently on up to this many
+// streams, shared by the plans of a device (created on the first build).
+// HIP maps a process's streams onto GPU_MAX_HW_QUEUES hardware queues (4
+// unless the process environment sets the variable before the HIP runtime
+// loads); min(that, kSideStreams) streams are used, since more streams than
+// queues only serialize behind each other.
+#ifndef tlno_amd_side_streams
+#define tlno_amd_side_streams 8
+#endif
+constexpr int kSideStreams = tlno_amd_side_streams;
+
+int side_streams_for_queues() {
+    int q = 4;  // HIP's default
+    if (const char* v = std::getenv("GPU_MAX_HW_QUEUES")) {
+        const int n = std::atoi(v);
+        if (n > 0) q = n;
+    }
+    return std::max(1, std::min(kSideStreams, q));
+}
+
 struct DeviceCtx {
     bool ready = false;
     double* boys = nullptr;
     double* rys = nullptr;
     DevTables tb{};
+    // side streams of concurrent builds; a build holds side_mu from its fork
+    // to its join, so the fork/join events are never re-recorded mid-build
+    std::mutex side_mu;
+    int nside = 0;  // 0: not created yet
+    hipStream_t side[kSideStreams] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[kSideStreams] = {};
 };
+
+// creates the device's side streams and events (caller holds side_mu)
+void ensure_side_streams(DeviceCtx& c) {
+    if (c.nside > 0) return;
+    const int n = side_streams_for_queues();
+    for (int s = 0; s < n; ++s) {
+        HIP_OK(hipStreamCreateWithFlags(&c.side[s], hipStreamNonBlocking));
+        HIP_OK(hipEventCreateWithFlags(&c.ev_join[s], hipEventDisableTiming));
+    }
+    HIP_OK(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
+    c.nside = n;
+}
 
 ClassKernels g_kernels[6][6];
 std::once_flag g_kernels_once;
@@ -215,24 +253,6 @@ struct GroupH {
     int sph_base = 0;
 };
 
-// Class-pair kernels of one build run concurrently on up to this many
-// streams of the plan.  HIP maps a process's streams onto GPU_MAX_HW_QUEUES
-// hardware queues (4 unless the process environment sets the variable
-// before the HIP runtime loads); a plan uses min(that, kSideStreams) streams,
-// since more streams than queues only serialize behind each other.
-#ifndef tlno_amd_side_streams
-#define tlno_amd_side_streams 8
-#endif
-constexpr int kSideStreams = tlno_amd_side_streams;
-
-int side_streams_for_queues() {
-    int q = 4;  // HIP's default
-    if (const char* v = std::getenv("GPU_MAX_HW_QUEUES")) {
-        const int n = std::atoi(v);
-        if (n > 0) q = n;
-    }
-    return std::max(1, std::min(kSideStreams, q));
-}
 
 struct TaskSet {
     int4* d_tasks = nullptr;
@@ -288,9 +308,6 @@ struct tlno_gpu_plan {
     double *dDin = nullptr, *dDint = nullptr, *dJacc = nullptr, *dKacc = nullptr;
     double *dJout = nullptr, *dKout = nullptr;
     hipStream_t stream = nullptr;
-    hipStream_t side[tlno_amd_side_streams] = {};
-    int nside = 1;  // side streams in use
-    hipEvent_t ev_fork = nullptr, ev_join[tlno_amd_side_streams] = {};
     std::map<double, tlno_amd::TaskSet> tasks;
     long long last_shell_quartets = 0, last_prim_quartets = 0;
     double last_thr = -1.0;
@@ -940,12 +957,6 @@ void build_plan(tlno_gpu_plan* pl, int n, const double* centers, const int* ang,
     (void)ctx;
     HIP_OK(hipSetDevice(pl->device));
     HIP_OK(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking));
-    pl->nside = side_streams_for_queues();
-    for (int s = 0; s < pl->nside; ++s) {
-        HIP_OK(hipStreamCreateWithFlags(&pl->side[s], hipStreamNonBlocking));
-        HIP_OK(hipEventCreateWithFlags(&pl->ev_join[s], hipEventDisableTiming));
-    }
-    HIP_OK(hipEventCreateWithFlags(&pl->ev_fork, hipEventDisableTiming));
     const int ng = static_cast<int>(pl->groups.size());
     int ib = 0, sb = 0;
     for (auto& G : pl->groups) {
@@ -1442,9 +1453,12 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
         // heaviest first; timing mode keeps them on one stream so each
         // class's events bracket only its own kernel.
         const bool concurrent = !pl->timing;
+        std::unique_lock<std::mutex> side_lk;
         if (concurrent) {
-            HIP_OK(hipEventRecord(pl->ev_fork, st));
-            for (int s = 0; s < pl->nside; ++s) HIP_OK(hipStreamWaitEvent(pl->side[s], pl->ev_fork, 0));
+            side_lk = std::unique_lock<std::mutex>(ctx.side_mu);
+            ensure_side_streams(ctx);
+            HIP_OK(hipEventRecord(ctx.ev_fork, st));
+            for (int s = 0; s < ctx.nside; ++s) HIP_OK(hipStreamWaitEvent(ctx.side[s], ctx.ev_fork, 0));
         }
         int rr = 0;
         for (int X = 5; X >= 0 && route; --X)
@@ -1453,7 +1467,7 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
                 const long long n64 = tcount[xy], n32 = tcount[36 + xy];
                 if (n64 + n32 == 0) continue;
                 const ClassKernels& k = g_kernels[X][Y];
-                const hipStream_t cs = concurrent ? pl->side[rr++ % pl->nside] : st;
+                const hipStream_t cs = concurrent ? ctx.side[rr++ % ctx.nside] : st;
                 if (pl->timing) HIP_OK(hipEventRecord(pl->ev[X][Y][0], cs));
                 a.cut = nullptr;
                 a.qlist = pl->d_qlist;
@@ -1467,7 +1481,7 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
                 if (n32 > 0) {
                     // the FP32 share goes to a stream of its own so that it
                     // overlaps the class's FP64 share
-                    const hipStream_t cs32 = (concurrent && n64 > 0) ? pl->side[rr++ % pl->nside] : cs;
+                    const hipStream_t cs32 = (concurrent && n64 > 0) ? ctx.side[rr++ % ctx.nside] : cs;
                     a.tasks = pl->d_t32 + ts.toff[xy];
                     a.ntasks = static_cast<int>(n32);
                     a.split = list_split(n32, k.nchunk);
@@ -1490,7 +1504,7 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
                           bm.begin();
                 }
                 const ClassKernels& k = g_kernels[X][Y];
-                const hipStream_t cs = concurrent ? pl->side[rr++ % pl->nside] : st;
+                const hipStream_t cs = concurrent ? ctx.side[rr++ % ctx.nside] : st;
                 if (pl->timing) HIP_OK(hipEventRecord(pl->ev[X][Y][0], cs));
                 a.cut = nullptr;  // density screening always takes the routed path
                 if (n64 > 0) {
@@ -1507,9 +1521,9 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
                 if (pl->timing) HIP_OK(hipEventRecord(pl->ev[X][Y][1], cs));
             }
         if (concurrent)
-            for (int s = 0; s < pl->nside; ++s) {
-                HIP_OK(hipEventRecord(pl->ev_join[s], pl->side[s]));
-                HIP_OK(hipStreamWaitEvent(st, pl->ev_join[s], 0));
+            for (int s = 0; s < ctx.nside; ++s) {
+                HIP_OK(hipEventRecord(ctx.ev_join[s], ctx.side[s]));
+                HIP_OK(hipStreamWaitEvent(st, ctx.ev_join[s], 0));
             }
     }
     if (dJ) {
@@ -1789,11 +1803,6 @@ tlno_gpu_plan::~tlno_gpu_plan() {
             for (int Y = 0; Y < 6; ++Y)
                 for (int e = 0; e < 2; ++e)
                     if (ev[X][Y][e]) (void)hipEventDestroy(ev[X][Y][e]);
-    for (int s = 0; s < tlno_amd::kSideStreams; ++s) {
-        if (side[s]) (void)hipStreamDestroy(side[s]);
-        if (ev_join[s]) (void)hipEventDestroy(ev_join[s]);
-    }
-    if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
