@@ -42,6 +42,17 @@ from tlno_amd.fock import DirectPlan, ScreeningOptions  # noqa: E402
 FP64_PEAK_TFLOPS = 78.6
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of a kernel as measured by the FETCH_SIZE and
+    WRITE_SIZE counter runs recorded under profiles/ (None when not measured)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "jk_organic30_traffic.json")) as f:
+            rec = json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+    return rec["bytes_per_launch"] if rec else None
+
+
 def gpu_run(b, D, steps, warmup, thr):
     lib = _lib.load()
     plan = DirectPlan(b, ScreeningOptions(integral_threshold=thr), cartesian=True)
@@ -239,12 +250,22 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first copy a buffer of this many MiB device to device (a copy kernel of known traffic "
+                         "for calibrating hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the J/K build shards by "
               "quartet tasks with one all-reduce of J and K; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
     thr = 1e-12
+    if args.copy_probe_mib > 0:
+        n = args.copy_probe_mib * (1 << 20) // 8
+        src, dst = hipmem.DeviceArray((n,)), hipmem.DeviceArray((n,))
+        dst.copy_from(src)
+        hipmem.synchronize()
+        src.free()
+        dst.free()
     b, D = configs.organic30() if args.workload == "organic30" else configs.water40()
     print(f"bench: {args.workload} J/K builds ({args.warmup} warm-up, {args.steps} timed)", file=sys.stderr,
           flush=True)
@@ -254,6 +275,8 @@ def main():
     pq_tot = sum(c["prim_quartets"] for c in classes)
     ints_tot = sum(c["integrals"] for c in classes)
     dom = max(classes, key=lambda c: c["ms"])
+    dom_kernel = (f"k_eri_team {dom['class']}" if plan.tuning.get(dom["class"], {}).get("choice") == "team"
+                  else f"k_eri_jk {dom['class']} (all chunk groups)")
     total_gflop = sum(c["gflop"] for c in classes)
     result = {
         "metric": "J/K Fock builds/s",
@@ -277,10 +300,9 @@ def main():
                            "cartesian_integrals_per_s": ints_tot / (wall / args.steps),
                            "shell_quartets_per_build": sq_tot, "primitive_quartets_per_build": pq_tot},
         "roofline": {"bound": "mfma", "pipe": "fp64 VALU (fp64 vector peak = fp64 matrix peak)",
-                     "kernel": (f"k_eri_team {dom['class']}" if plan.tuning.get(dom["class"], {}).get("choice") == "team"
-                                else f"k_eri_jk {dom['class']} (all chunk groups)"), "achieved": dom["tflops"],
+                     "kernel": dom_kernel, "achieved": dom["tflops"],
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": dom["tflops"] / FP64_PEAK_TFLOPS,
-                     "traffic": None, "algorithmic_gflop_per_launch": dom["gflop"],
+                     "traffic": measured_traffic(dom_kernel), "algorithmic_gflop_per_launch": dom["gflop"],
                      "avg_launch_ms": dom["ms"],
                      "whole_build_tflops": total_gflop / (ms_per * 1e-3) / 1e3},
         "classes": [{k: (round(v, 6) if isinstance(v, float) else v) for k, v in c.items()} for c in classes],

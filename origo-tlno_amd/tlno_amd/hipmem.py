@@ -59,6 +59,12 @@ class DeviceArray:
             raise ValueError("size mismatch")
         _ok(hip().hipMemcpy(self.ptr, a.ctypes.data_as(C.c_void_p), self.nbytes, 1), "hipMemcpy H2D")
 
+    def copy_from(self, other):
+        """Device-to-device copy of a buffer of the same size."""
+        if other.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        _ok(hip().hipMemcpy(self.ptr, other.ptr, self.nbytes, 3), "hipMemcpy D2D")
+
     def to_host(self):
         out = np.empty(self.shape)
         _ok(hip().hipMemcpy(out.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes, 2), "hipMemcpy D2H")
