@@ -57,6 +57,11 @@ thread_local std::string g_err;
 #endif
 constexpr int kSideStreams = tlno_amd_side_streams;
 
+// total angular momentum and Cartesian components of the six pair classes
+// ss, ps, pp, ds, dp, dd
+constexpr int kClassLAB[6] = {0, 1, 2, 2, 3, 4};
+constexpr int kClassNcart[6] = {1, 3, 9, 6, 18, 36};
+
 int side_streams_for_queues() {
     int q = 4;  // HIP's default
     if (const char* v = std::getenv("GPU_MAX_HW_QUEUES")) {
@@ -348,6 +353,7 @@ struct tlno_gpu_plan {
     long long last_route[4] = {0, 0, 0, 0};  // candidates, density-kept, kept, FP32 (last routed build)
     int team_min_out = TLNO_TEAM_MIN_OUT;    // 0: lane evaluator for every class
     signed char team_sel[6][6];              // autotuned choice per class pair: -1 none, 0 lane, 1 team
+    double cls_cost[6][6] = {};              // measured ms of the chosen evaluator per class pair (0: not measured)
     // optional per-class kernel timing with HIP events on the plan stream
     int timing = 0;
     hipEvent_t ev[6][6][2] = {};
@@ -1350,6 +1356,17 @@ int list_split(long long ntask, int nchunk) {
     return static_cast<int>(std::min<long long>(256, (kTargetWaves + waves - 1) / waves));
 }
 
+// Estimated cost (ms) of a class pair's share of one build, for ordering its
+// launches: the autotuner's measurement when there is one, else primitive
+// quartets times a per-primitive work model (roots, 2D tables, products).
+double class_cost(const tlno_gpu_plan* pl, const TaskSet& ts, int X, int Y) {
+    if (pl->cls_cost[X][Y] > 0.0) return pl->cls_cost[X][Y];
+    const int LAB = kClassLAB[X], LCD = kClassLAB[Y];
+    const int nroot = (LAB + LCD) / 2 + 1;
+    const double work = 100.0 + nroot * (15.0 * (LAB + 1) * (LCD + 1) + 3.0 * kClassNcart[X] * kClassNcart[Y]);
+    return static_cast<double>(ts.cls_pq[X][Y]) * work * 1e-10;
+}
+
 void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, double thr) {
     DeviceCtx& ctx = device_ctx(pl->device);
     HIP_OK(hipSetDevice(pl->device));
@@ -1460,15 +1477,26 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
             HIP_OK(hipEventRecord(ctx.ev_fork, st));
             for (int s = 0; s < ctx.nside; ++s) HIP_OK(hipStreamWaitEvent(ctx.side[s], ctx.ev_fork, 0));
         }
-        int rr = 0;
+        // Every class launch (FP64 share, FP32 share) becomes a job with an
+        // estimated cost; jobs are collected heaviest class first, then
+        // launched.
+        struct Job {
+            int X, Y, team_min;
+            bool fp32;
+            JKArgs args;
+            long long nwaves;
+            double cost;
+        };
+        std::vector<Job> jobs;
+        auto add_jobs = [&](int X, int Y, bool fp32, const JKArgs& args, long long nwaves, int team_min,
+                            double cost) { jobs.push_back({X, Y, team_min, fp32, args, nwaves, cost}); };
         for (int X = 5; X >= 0 && route; --X)
             for (int Y = X; Y >= 0; --Y) {
                 const int xy = 6 * X + Y;
                 const long long n64 = tcount[xy], n32 = tcount[36 + xy];
                 if (n64 + n32 == 0) continue;
                 const ClassKernels& k = g_kernels[X][Y];
-                const hipStream_t cs = concurrent ? ctx.side[rr++ % ctx.nside] : st;
-                if (pl->timing) HIP_OK(hipEventRecord(pl->ev[X][Y][0], cs));
+                const double c = class_cost(pl, ts, X, Y) / static_cast<double>(n64 + n32);
                 a.cut = nullptr;
                 a.qlist = pl->d_qlist;
                 if (n64 > 0) {
@@ -1476,18 +1504,14 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
                     a.ntasks = static_cast<int>(n64);
                     a.split = use_team(k, false, a, class_team_min(pl, X, Y)) ? 1 : list_split(n64, k.nchunk);
                     a.team_slices = team_slices_for(ts.cls_sq[X][Y]);
-                    launch_jk(k, false, a, n64 * a.split, cs, class_team_min(pl, X, Y));
+                    add_jobs(X, Y, false, a, n64 * a.split, class_team_min(pl, X, Y), c * n64);
                 }
                 if (n32 > 0) {
-                    // the FP32 share goes to a stream of its own so that it
-                    // overlaps the class's FP64 share
-                    const hipStream_t cs32 = (concurrent && n64 > 0) ? ctx.side[rr++ % ctx.nside] : cs;
                     a.tasks = pl->d_t32 + ts.toff[xy];
                     a.ntasks = static_cast<int>(n32);
                     a.split = list_split(n32, k.nchunk);
-                    launch_jk(k, true, a, n32 * a.split, cs32);
+                    add_jobs(X, Y, true, a, n32 * a.split, 0, c * n32);
                 }
-                if (pl->timing) HIP_OK(hipEventRecord(pl->ev[X][Y][1], cs));
             }
         a.qlist = nullptr;
         a.split = 1;
@@ -1503,23 +1527,44 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
                     n64 = std::lower_bound(bm.begin(), bm.end(), cut, [](double v, double c) { return v >= c; }) -
                           bm.begin();
                 }
-                const ClassKernels& k = g_kernels[X][Y];
-                const hipStream_t cs = concurrent ? ctx.side[rr++ % ctx.nside] : st;
-                if (pl->timing) HIP_OK(hipEventRecord(pl->ev[X][Y][0], cs));
+                const double c = class_cost(pl, ts, X, Y) / static_cast<double>(nt);
                 a.cut = nullptr;  // density screening always takes the routed path
                 if (n64 > 0) {
                     a.tasks = ts.d_tasks + ts.off[X][Y];
                     a.ntasks = static_cast<int>(n64);
                     a.team_slices = team_slices_for(ts.cls_sq[X][Y]);
-                    launch_jk(k, false, a, n64, cs, class_team_min(pl, X, Y));
+                    add_jobs(X, Y, false, a, n64, class_team_min(pl, X, Y), c * n64);
                 }
                 if (nt > n64) {
                     a.tasks = ts.d_tasks + ts.off[X][Y] + n64;
                     a.ntasks = static_cast<int>(nt - n64);
-                    launch_jk(k, true, a, nt - n64, cs);
+                    add_jobs(X, Y, true, a, nt - n64, 0, c * (nt - n64));
                 }
-                if (pl->timing) HIP_OK(hipEventRecord(pl->ev[X][Y][1], cs));
             }
+        if (concurrent) {
+            // each job onto the side stream with the least estimated work so
+            // far (list scheduling in class order; ordering by cost or
+            // splitting the chunk groups of a class over streams measured no
+            // better: concurrent kernels do not add up their serial times)
+            double load[kSideStreams] = {};
+            for (Job& j : jobs) {
+                int best = 0;
+                for (int q = 1; q < ctx.nside; ++q)
+                    if (load[q] < load[best]) best = q;
+                load[best] += j.cost;
+                launch_jk(g_kernels[j.X][j.Y], j.fp32, j.args, j.nwaves, ctx.side[best], j.team_min);
+            }
+        } else {
+            // timing mode: one stream, class order, events around each class's jobs
+            for (std::size_t i = 0; i < jobs.size(); ++i) {
+                Job& j = jobs[i];
+                const bool first = i == 0 || jobs[i - 1].X != j.X || jobs[i - 1].Y != j.Y;
+                const bool last = i + 1 == jobs.size() || jobs[i + 1].X != j.X || jobs[i + 1].Y != j.Y;
+                if (first) HIP_OK(hipEventRecord(pl->ev[j.X][j.Y][0], st));
+                launch_jk(g_kernels[j.X][j.Y], j.fp32, j.args, j.nwaves, st, j.team_min);
+                if (last) HIP_OK(hipEventRecord(pl->ev[j.X][j.Y][1], st));
+            }
+        }
         if (concurrent)
             for (int s = 0; s < ctx.nside; ++s) {
                 HIP_OK(hipEventRecord(ctx.ev_join[s], ctx.side[s]));
@@ -1747,6 +1792,7 @@ std::string plan_autotune(tlno_gpu_plan* pl, double thr, int repeats) {
             const bool has_team = g_kernels[X][Y].team != nullptr;
             const bool team = has_team && ms[1][X][Y] < ms[0][X][Y];
             pl->team_sel[X][Y] = static_cast<signed char>(team ? 1 : 0);
+            pl->cls_cost[X][Y] = team ? ms[1][X][Y] : ms[0][X][Y];
             if (!has_team || ms[0][X][Y] == 0.0) continue;
             char buf[200];
             std::snprintf(buf, sizeof(buf), "%s\"(%s|%s)\": {\"lane_ms\": %.4f, \"team_ms\": %.4f, \"choice\": \"%s\"}",
