@@ -2456,6 +2456,7 @@ struct tlno_gpu_batch {
     };
     std::map<double, Tasks> tasks;
     tlno_amd::LaunchShape shape[6][6];
+    double cost[6][6] = {};  // autotuned ms per class pair (0: not measured)
     int team_min_out = TLNO_TEAM_MIN_OUT;
     // captured build
     hipGraph_t graph = nullptr;
@@ -2680,8 +2681,11 @@ void batch_launch_class(tlno_gpu_batch* bt, const tlno_gpu_batch::Tasks& ts, int
                                0, st));
 }
 
-// the whole build on the batch stream (capturable: kernels only)
-void batch_enqueue(tlno_gpu_batch* bt, const double* dD, double* dJ, double* dK, double thr) {
+// The whole build: prep and finalize on the batch stream, the class pairs
+// list-scheduled onto the device's side streams between a fork and a join
+// (capturable: kernels and event records/waits only).  The caller holds
+// ctx.side_mu and has created the side streams.
+void batch_enqueue(tlno_gpu_batch* bt, DeviceCtx& ctx, const double* dD, double* dJ, double* dK, double thr) {
     const tlno_gpu_batch::Tasks& ts = batch_tasks(bt, thr);
     const hipStream_t st = bt->stream;
     const unsigned g = grid_for(bt->total, 256);
@@ -2690,8 +2694,26 @@ void batch_enqueue(tlno_gpu_batch* bt, const double* dD, double* dJ, double* dK,
     if (dK) k_fill<<<g, 256, 0, st>>>(bt->dKacc, bt->total, 0.0);
     k_fill<<<1, 64, 0, st>>>(reinterpret_cast<double*>(bt->dcounter), 4, 0.0);
     const JKArgs a = batch_args(bt, dJ != nullptr, dK != nullptr);
+    HIP_OK(hipEventRecord(ctx.ev_fork, st));
+    for (int s = 0; s < ctx.nside; ++s) HIP_OK(hipStreamWaitEvent(ctx.side[s], ctx.ev_fork, 0));
+    double load[kSideStreams] = {};
     for (int X = 5; X >= 0; --X)
-        for (int Y = X; Y >= 0; --Y) batch_launch_class(bt, ts, X, Y, bt->shape[X][Y], a, st);
+        for (int Y = X; Y >= 0; --Y) {
+            if (ts.cnt[X][Y] == 0) continue;
+            // the autotuned time, else tasks x outputs per quartet
+            const double c = bt->cost[X][Y] > 0.0
+                                 ? bt->cost[X][Y]
+                                 : 1e-6 * static_cast<double>(ts.cnt[X][Y]) * kClassNcart[X] * kClassNcart[Y];
+            int best = 0;
+            for (int q = 1; q < ctx.nside; ++q)
+                if (load[q] < load[best]) best = q;
+            load[best] += c;
+            batch_launch_class(bt, ts, X, Y, bt->shape[X][Y], a, ctx.side[best]);
+        }
+    for (int s = 0; s < ctx.nside; ++s) {
+        HIP_OK(hipEventRecord(ctx.ev_join[s], ctx.side[s]));
+        HIP_OK(hipStreamWaitEvent(st, ctx.ev_join[s], 0));
+    }
     if (dJ) k_fin_batch<<<g, 256, 0, st>>>(bt->dJacc, dJ, bt->dperm, bt->dpbase, bt->dmbase, bt->dmM, bt->nmol, bt->total);
     if (dK) k_fin_batch<<<g, 256, 0, st>>>(bt->dKacc, dK, bt->dperm, bt->dpbase, bt->dmbase, bt->dmM, bt->nmol, bt->total);
     HIP_OK(hipGetLastError());
@@ -2702,6 +2724,11 @@ void batch_build(tlno_gpu_batch* bt, const double* dD, double* dJ, double* dK, d
         throw std::runtime_error("eri_ao_tensor_hgp: schwarz_threshold must be finite and >= 0.");
     HIP_OK(hipSetDevice(bt->device));
     const tlno_gpu_batch::Tasks& ts = batch_tasks(bt, thr);  // host work stays outside any capture
+    DeviceCtx& ctx = device_ctx(bt->device);
+    // the side streams belong to this build until its capture ends or its
+    // launches are enqueued
+    std::lock_guard<std::mutex> side_lk(ctx.side_mu);
+    ensure_side_streams(ctx);
     HIP_OK(hipEventRecord(bt->ev0, bt->stream));
     if (use_graph) {
         const bool same = bt->graph_exec && bt->graph_key[0] == dD && bt->graph_key[1] == dJ &&
@@ -2710,7 +2737,7 @@ void batch_build(tlno_gpu_batch* bt, const double* dD, double* dJ, double* dK, d
             bt->drop_graph();
             HIP_OK(hipStreamBeginCapture(bt->stream, hipStreamCaptureModeThreadLocal));
             try {
-                batch_enqueue(bt, dD, dJ, dK, thr);
+                batch_enqueue(bt, ctx, dD, dJ, dK, thr);
             } catch (...) {
                 hipGraph_t g = nullptr;
                 (void)hipStreamEndCapture(bt->stream, &g);
@@ -2726,7 +2753,7 @@ void batch_build(tlno_gpu_batch* bt, const double* dD, double* dJ, double* dK, d
         }
         HIP_OK(hipGraphLaunch(bt->graph_exec, bt->stream));
     } else {
-        batch_enqueue(bt, dD, dJ, dK, thr);
+        batch_enqueue(bt, ctx, dD, dJ, dK, thr);
     }
     HIP_OK(hipEventRecord(bt->ev1, bt->stream));
     HIP_OK(hipStreamSynchronize(bt->stream));
@@ -2776,6 +2803,7 @@ std::string batch_autotune(tlno_gpu_batch* bt, double thr, int repeats) {
                     }
                 }
             bt->shape[X][Y] = best;
+            bt->cost[X][Y] = best_ms;
             static const char* names[6] = {"ss", "ps", "pp", "ds", "dp", "dd"};
             char buf[256];
             std::snprintf(buf, sizeof(buf), "%s\"(%s|%s)\": {\"block\": %d, \"waves_per_cu\": %d, \"ms\": %.4f, "
