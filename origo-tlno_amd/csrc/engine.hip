@@ -271,6 +271,15 @@ struct TaskSet {
     // running shell/primitive quartet counts (for the precision split)
     std::vector<double> bmax[6][6];
     std::vector<long long> cum_sq[6][6], cum_pq[6][6];
+    // Bucketed list tasks, for ket classes whose primitive-pair counts differ
+    // (a wave takes as long as its heaviest ket): d_bk holds, per ket class,
+    // its pairs grouped by bucket of primitive-pair count (each group in
+    // bound order), and list tasks {P, offset into d_bk, count} hold at most
+    // 64 kets of one group; lcnt = 0 where the range tasks are used.
+    int* d_bk = nullptr;
+    int4* d_ltasks = nullptr;
+    long long loff[6][6] = {}, lcnt[6][6] = {};
+    std::vector<double> lbmax[6][6];
     // per-quartet routing (density test, learned screen): rows of at most
     // kRouteSeg candidate kets {P, k_begin, k_end, 6 X + Y}, the qlist region
     // of each row, and the task capacity of each class pair; built on first use
@@ -1182,6 +1191,59 @@ int wave_split(int split, int nq) {
     return (split + lanes_per_ket - 1) / lanes_per_ket;
 }
 
+// bucket of a primitive-pair count: half octaves, so counts inside one
+// bucket differ by less than a factor sqrt(2)
+inline int prim_bucket(int pn) {
+    int b = 0;
+    while ((1LL << (b + 1)) <= 2LL * pn * pn) ++b;  // b = floor(log2(2 pn^2)) = floor(2 log2 pn) + 1
+    return b;
+}
+
+// Bucketed list tasks of class pair (X, Y) (see TaskSet).  bk: the ket
+// groups of class Y, gbeg/gend their ranges in bk.
+void bucket_list_tasks(const tlno_gpu_plan* pl, double thr, int X, int Y, const std::vector<int>& bk,
+                       const std::vector<std::pair<int, int>>& groups, std::vector<int4>& out,
+                       std::vector<double>& bound) {
+    std::vector<std::pair<double, int4>> tasks;
+    for (int P = pl->cls_begin[X]; P < pl->cls_begin[X + 1]; ++P)
+        for (const auto& g : groups) {
+            // group entries are pair indices in increasing order (bounds descending)
+            int lo = g.first, hi = g.second;
+            if (thr > 0.0) {
+                int a = lo, b = hi;
+                while (a < b) {
+                    const int mid = (a + b) / 2;
+                    if (pl->Q[P] * pl->Q[bk[mid]] >= thr) a = mid + 1;
+                    else b = mid;
+                }
+                hi = a;
+            }
+            if (X == Y) {  // unique quartets: kets Q <= P
+                int a = lo, b = hi;
+                while (a < b) {
+                    const int mid = (a + b) / 2;
+                    if (bk[mid] <= P) a = mid + 1;
+                    else b = mid;
+                }
+                hi = a;
+            }
+            for (int j0 = lo; j0 < hi; j0 += 64) {
+                const long long off = j0;
+                const int n = std::min(64, hi - j0);
+                tasks.push_back({pl->Q[P] * pl->Q[bk[j0]],
+                                 make_int4(P, static_cast<int>(off & 0xffffffffLL),
+                                           n | static_cast<int>((off >> 32) << 8), 0)});
+            }
+        }
+    // heaviest first; the tasks below a precision gate form a suffix
+    std::stable_sort(tasks.begin(), tasks.end(),
+                     [](const std::pair<double, int4>& u, const std::pair<double, int4>& v) { return u.first > v.first; });
+    for (const auto& t : tasks) {
+        out.push_back(t.second);
+        bound.push_back(t.first);
+    }
+}
+
 TaskSet& get_tasks(tlno_gpu_plan* pl, double thr) {
     auto it = pl->tasks.find(thr);
     if (it != pl->tasks.end()) return it->second;
@@ -1263,6 +1325,33 @@ TaskSet& get_tasks(tlno_gpu_plan* pl, double thr) {
     if (!all.empty()) {
         HIP_OK(hipMalloc(&ts.d_tasks, all.size() * sizeof(int4)));
         HIP_OK(hipMemcpy(ts.d_tasks, all.data(), all.size() * sizeof(int4), hipMemcpyHostToDevice));
+    }
+    // ket groups by primitive-pair bucket, for classes with more than one bucket
+    std::vector<int> bk;
+    std::vector<std::pair<int, int>> kgroups[6];
+    for (int c = 0; c < 6; ++c) {
+        std::map<int, std::vector<int>> by;
+        for (int Q = pl->cls_begin[c]; Q < pl->cls_begin[c + 1]; ++Q) by[prim_bucket(pl->pn_host[Q])].push_back(Q);
+        if (by.size() < 2) continue;
+        for (auto& kv : by) {
+            const int b = static_cast<int>(bk.size());
+            bk.insert(bk.end(), kv.second.begin(), kv.second.end());
+            kgroups[c].push_back({b, static_cast<int>(bk.size())});
+        }
+    }
+    std::vector<int4> lall;
+    for (int X = 0; X < 6; ++X)
+        for (int Y = 0; Y <= X; ++Y) {
+            ts.loff[X][Y] = static_cast<long long>(lall.size());
+            if (!kgroups[Y].empty() && ts.cls_sq[X][Y] > 0)
+                bucket_list_tasks(pl, thr, X, Y, bk, kgroups[Y], lall, ts.lbmax[X][Y]);
+            ts.lcnt[X][Y] = static_cast<long long>(lall.size()) - ts.loff[X][Y];
+        }
+    if (!lall.empty()) {
+        HIP_OK(hipMalloc(&ts.d_bk, bk.size() * sizeof(int)));
+        HIP_OK(hipMemcpy(ts.d_bk, bk.data(), bk.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIP_OK(hipMalloc(&ts.d_ltasks, lall.size() * sizeof(int4)));
+        HIP_OK(hipMemcpy(ts.d_ltasks, lall.data(), lall.size() * sizeof(int4), hipMemcpyHostToDevice));
     }
     return pl->tasks.emplace(thr, ts).first->second;
 }
@@ -1517,6 +1606,37 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
         a.split = 1;
         for (int X = 5; X >= 0 && !route; --X)
             for (int Y = X; Y >= 0; --Y) {
+                if (ts.lcnt[X][Y] > 0 && !use_team(g_kernels[X][Y], false, a, class_team_min(pl, X, Y))) {
+                    // lane evaluator: bucketed list tasks (kets of one task have
+                    // similar primitive counts); the team evaluator (one wave per
+                    // quartet) keeps the range tasks
+                    const long long nt = ts.lcnt[X][Y];
+                    long long n64 = nt;
+                    if (pl->fp32_gate > 0.0) {
+                        const auto& bm = ts.lbmax[X][Y];
+                        const double cut = pl->fp32_gate / std::max(maxD, 1e-300);
+                        n64 = std::lower_bound(bm.begin(), bm.end(), cut,
+                                               [](double v, double c) { return v >= c; }) - bm.begin();
+                    }
+                    const ClassKernels& k = g_kernels[X][Y];
+                    const double c = class_cost(pl, ts, X, Y) / static_cast<double>(nt);
+                    JKArgs al = a;
+                    al.cut = nullptr;
+                    al.qlist = ts.d_bk;
+                    if (n64 > 0) {
+                        al.tasks = ts.d_ltasks + ts.loff[X][Y];
+                        al.ntasks = static_cast<int>(n64);
+                        al.split = list_split(n64, k.nchunk);
+                        add_jobs(X, Y, false, al, n64 * al.split, class_team_min(pl, X, Y), c * n64);
+                    }
+                    if (nt > n64) {
+                        al.tasks = ts.d_ltasks + ts.loff[X][Y] + n64;
+                        al.ntasks = static_cast<int>(nt - n64);
+                        al.split = list_split(nt - n64, k.nchunk);
+                        add_jobs(X, Y, true, al, (nt - n64) * al.split, 0, c * (nt - n64));
+                    }
+                    continue;
+                }
                 const long long nt = ts.cnt[X][Y];
                 if (nt == 0) continue;
                 // FP64 prefix / FP32 suffix of the bound-ordered task list
@@ -1841,6 +1961,8 @@ tlno_gpu_plan::~tlno_gpu_plan() {
     f(d_qlist); f(d_t64); f(d_t32); f(d_tcount); f(d_rstats);
     for (auto& kv : tasks) {
         f(kv.second.d_tasks);
+        f(kv.second.d_bk);
+        f(kv.second.d_ltasks);
         f(kv.second.d_rows);
         f(kv.second.d_rbase);
     }

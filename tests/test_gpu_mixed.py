@@ -49,3 +49,24 @@ def test_mixed_gate_zero_and_huge():
     assert np.abs(J1 - J0).max() < 1e-4 * np.abs(J0).max()
     with pytest.raises(Exception):
         plan.set_mixed_precision(-1.0)
+
+
+def test_mixed_precision_mixed_contractions():
+    """organic30 (1-6 primitives per shell): the lane evaluator takes
+    bucketed list tasks there; the FP64 prefix / FP32 suffix split of those
+    lists must cover every quartet once (shell-quartet count equal to the
+    FP64 build's) and stay within the mixed-route bound."""
+    b, D = configs.organic30()
+    plan = DirectPlan(b, ScreeningOptions(integral_threshold=1e-12), cartesian=True)
+    J0, K0 = plan.jk(D)
+    sq0, pq0 = plan.last_build_stats()
+    plan.set_mixed_precision(1e-6)
+    J1, K1 = plan.jk(D)
+    st = plan.last_mixed_stats()
+    sq1, pq1 = plan.last_build_stats()
+    plan.set_mixed_precision(0.0)
+    ej, ek = np.abs(J1 - J0).max(), np.abs(K1 - K0).max()
+    print(f"organic30: FP32 share {st['fp32_shell_quartets'] / sq1:.3f}; max|dJ| {ej:.2e} max|dK| {ek:.2e}")
+    assert (sq1, pq1) == (sq0, pq0)
+    assert 0 < st["fp32_shell_quartets"] < sq1
+    assert ej <= 1e-7 and ek <= 1e-7
