@@ -303,7 +303,12 @@ constexpr double kTwoPi52 = 34.986836655249725693;  // 2 pi^{5/2}
 
 // Classes whose per-primitive work is large enough to pay for per-lane
 // primitive indexing walk the flattened (ket primitive, bra primitive) grid.
-__host__ __device__ constexpr bool flat_prims(int LA, int LB, int LC, int LD) { return LA + LB + LC + LD >= 4; }
+#ifndef TLNO_FLAT_MIN_L
+#define TLNO_FLAT_MIN_L 4
+#endif
+__host__ __device__ constexpr bool flat_prims(int LA, int LB, int LC, int LD) {
+    return LA + LB + LC + LD >= TLNO_FLAT_MIN_L;
+}
 
 // Accumulates the lane's block slice (unit-diagonal components) into acc over
 // its share of the primitive quartets: bra primitive pairs [ib0, ib1) of P
