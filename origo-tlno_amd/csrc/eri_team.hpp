@@ -52,6 +52,10 @@ struct TeamShape {
     static constexpr int GS = (LAB + 1) * (LCD + 1);       // one axis table
     static constexpr int CS = 3 * GS + 1;                  // one combination (padded)
     static constexpr int CB = TLNO_TEAM_CB;                // combinations per batch
+    // phase B unrolled over combinations where the wave is at one per SIMD
+    // anyway (the LDS addresses then take immediate offsets); elsewhere the
+    // extra registers would cost occupancy
+    static constexpr int UB = IPL >= 16 ? 8 : 1;
     static constexpr int LDS_A = CB * CS;
     static constexpr int LDS_C = NI + NOUT;
     static constexpr int LDS_D = NI + NOUT + NA * NB + NC * ND + NA * NC + NA * ND + NB * NC + NB * ND;
@@ -170,6 +174,7 @@ __device__ __forceinline__ void eri_team_quartet(const JKArgs& a, double* __rest
         __syncthreads();
         // ---- phase B: accumulate the owned intermediates over the batch
         const int nb = min(T::CB, ncomb - c0);
+#pragma unroll T::UB
         for (int j = 0; j < nb; ++j) {
             const double* g = lds + j * T::CS;
 #pragma unroll
