@@ -285,17 +285,17 @@ __device__ __forceinline__ void axis_table(R C00, R D00, R B00, R B10, R B01, R 
         }
 }
 
-// Output chunk of a class: all bra components a (AC < 0) or just a = AC; all
-// ket components c (CC < 0) or c in [CC, CC + CN).
-template <int LA, int LB, int LC, int LD, int AC, int CC, int CN>
+// Output chunk of a class: all bra components a (AC < 0) or a in
+// [AC, AC + AN); all ket components c (CC < 0) or c in [CC, CC + CN).
+template <int LA, int LB, int LC, int LD, int AC, int AN, int CC, int CN>
 struct Shape {
     static constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
     static constexpr int LAB = LA + LB, LCD = LC + LD;
     static constexpr int NR = (LAB + LCD) / 2 + 1;
-    static constexpr int NAL = AC < 0 ? NA : 1;
+    static constexpr int NAL = AC < 0 ? NA : AN;
     static constexpr int NCL = CC < 0 ? NC : CN;
     static constexpr int NOUT = NAL * NB * NCL * ND;
-    __device__ static constexpr int acomp(int ai) { return AC < 0 ? ai : AC; }
+    __device__ static constexpr int acomp(int ai) { return AC < 0 ? ai : AC + ai; }
     __device__ static constexpr int ccomp(int ci) { return CC < 0 ? ci : CC + ci; }
 };
 
@@ -315,10 +315,10 @@ __host__ __device__ constexpr bool flat_prims(int LA, int LB, int LC, int LD) {
 // (a task's slice) times all ket primitive pairs of Q, of which the lane takes
 // every js-th starting at j0 -- of the flattened grid (bra index fastest) for
 // flat_prims classes, else of the bra pairs (with every ket pair).
-template <typename R, int LA, int LB, int LC, int LD, int AC, int CC, int CN>
+template <typename R, int LA, int LB, int LC, int LD, int AC, int AN, int CC, int CN>
 __device__ __forceinline__ void quartet(const DevTables& tb, const PairData& pd, int P, int Q, int ib0, int ib1,
-                                        int j0, int js, R (&acc)[Shape<LA, LB, LC, LD, AC, CC, CN>::NOUT]) {
-    using S = Shape<LA, LB, LC, LD, AC, CC, CN>;
+                                        int j0, int js, R (&acc)[Shape<LA, LB, LC, LD, AC, AN, CC, CN>::NOUT]) {
+    using S = Shape<LA, LB, LC, LD, AC, AN, CC, CN>;
     constexpr CartT<LA> ca = make_cart<LA>();
     constexpr CartT<LB> cb = make_cart<LB>();
     constexpr CartT<LC> cc = make_cart<LC>();
@@ -483,9 +483,9 @@ __device__ __forceinline__ int ket_stride(int nq) {
 // ib0 | ib1 << 16} (range mode) or a list-mode task (see JKArgs).  Lane l
 // takes ket l % ks of the task (ks = ket_stride(kets)) and share l / ks of
 // the 64 / ks shares of its primitive quartets (see quartet).
-template <typename R, int LA, int LB, int LC, int LD, int AC, int CC, int CN>
+template <typename R, int LA, int LB, int LC, int LD, int AC, int AN, int CC, int CN>
 __device__ __forceinline__ void eri_jk_task(const JKArgs& a, const int wave) {
-    using S = Shape<LA, LB, LC, LD, AC, CC, CN>;
+    using S = Shape<LA, LB, LC, LD, AC, AN, CC, CN>;
     const int lane = threadIdx.x & 63;
     const int task = a.qlist ? wave / a.split : wave;
     if (task >= a.ntasks) return;  // whole wave leaves together
@@ -547,7 +547,7 @@ __device__ __forceinline__ void eri_jk_task(const JKArgs& a, const int wave) {
     }
 
     R acc[S::NOUT];
-    quartet<R, LA, LB, LC, LD, AC, CC, CN>(a.tb, a.pd, P, Q, ib0, active ? ib1 : ib0, sub, ibs, acc);
+    quartet<R, LA, LB, LC, LD, AC, AN, CC, CN>(a.tb, a.pd, P, Q, ib0, active ? ib1 : ib0, sub, ibs, acc);
     if (ibs > 1) {
         // the ibs lanes of a ket (lane bits above ks) hold parts of one block:
         // sum them into the first, which alone digests
@@ -706,24 +706,24 @@ __device__ __forceinline__ void eri_jk_task(const JKArgs& a, const int wave) {
 }
 
 // Grid entry: one work item per wave, or a persistent loop over work items.
-template <typename R, int LA, int LB, int LC, int LD, int AC, int CC, int CN>
+template <typename R, int LA, int LB, int LC, int LD, int AC, int AN, int CC, int CN>
 __device__ __forceinline__ void eri_jk_body(const JKArgs& a) {
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int end = a.stride > 0 ? a.nwork : wave + 1;
     const int step = a.stride > 0 ? a.stride : 1;
-    for (int w = wave; w < end; w += step) eri_jk_task<R, LA, LB, LC, LD, AC, CC, CN>(a, w);
+    for (int w = wave; w < end; w += step) eri_jk_task<R, LA, LB, LC, LD, AC, AN, CC, CN>(a, w);
 }
 
 // Writes the lane's slice of the unit-diagonal Cartesian block of (P|Q) into
 // out[off + ((a*NB + b)*NC + c)*ND + d] (bra/ket shells in pair orientation).
-template <int LA, int LB, int LC, int LD, int AC, int CC, int CN>
+template <int LA, int LB, int LC, int LD, int AC, int AN, int CC, int CN>
 __device__ __forceinline__ void eri_store_body(const StoreArgs& a) {
-    using S = Shape<LA, LB, LC, LD, AC, CC, CN>;
+    using S = Shape<LA, LB, LC, LD, AC, AN, CC, CN>;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     const int2 pq = a.quartets[i];
     double acc[S::NOUT];
-    quartet<double, LA, LB, LC, LD, AC, CC, CN>(a.tb, a.pd, pq.x, pq.y, 0, a.pd.pn[pq.x], 0, 1, acc);
+    quartet<double, LA, LB, LC, LD, AC, AN, CC, CN>(a.tb, a.pd, pq.x, pq.y, 0, a.pd.pn[pq.x], 0, 1, acc);
     double* out = a.out + a.off[i];
     int o = 0;
 #pragma unroll

@@ -2,7 +2,8 @@
 // (bra X >= ket Y) is split into output chunks of at most TLNO_CHUNK_MAXOUT
 // values:
 //   mode 0: the full block (one chunk);
-//   mode 1: one chunk per bra component a;
+//   mode 1: one chunk per run of AN consecutive bra components a (as many
+//           as fit TLNO_CHUNK_RUN_MAXOUT; the last run may be shorter);
 //   mode 2: one chunk per bra component a and group of CN ket components c
 //           (the largest CN dividing ncart(LC) that fits the limit).
 // A chunk holds its outputs in registers for the whole primitive loop, so
@@ -18,6 +19,9 @@
 
 #ifndef TLNO_CHUNK_MAXOUT
 #define TLNO_CHUNK_MAXOUT 108
+#endif
+#ifndef TLNO_CHUNK_RUN_MAXOUT
+#define TLNO_CHUNK_RUN_MAXOUT 81
 #endif
 #ifndef TLNO_CHUNK_GROUP
 #define TLNO_CHUNK_GROUP 2
@@ -40,8 +44,14 @@ struct ChunkPolicy {
     }
     static constexpr int cn = mode == 2 ? pick_cn() : 1;
     static constexpr int ncg = NC / cn;  // ket component groups (mode 2)
-    static constexpr int nchunk = mode == 0 ? 1 : (mode == 1 ? NA : NA * ncg);
-    static constexpr int ac(int i) { return mode == 0 ? -1 : (mode == 1 ? i : i / ncg); }
+    static constexpr int per_a = nc / NA;  // outputs per bra component
+    static constexpr int an_max = TLNO_CHUNK_RUN_MAXOUT / per_a > 1 ? TLNO_CHUNK_RUN_MAXOUT / per_a : 1;
+    static constexpr int an_run = mode == 1 ? (an_max < NA ? an_max : NA) : 1;
+    static constexpr int nchunk = mode == 0 ? 1 : (mode == 1 ? (NA + an_run - 1) / an_run : NA * ncg);
+    static constexpr int ac(int i) { return mode == 0 ? -1 : (mode == 1 ? i * an_run : i / ncg); }
+    static constexpr int an(int i) {
+        return mode == 1 ? (NA - i * an_run < an_run ? NA - i * an_run : an_run) : 1;
+    }
     static constexpr int cc(int i) { return mode == 2 ? (i % ncg) * cn : -1; }
 };
 
@@ -49,14 +59,14 @@ template <typename R, int LA, int LB, int LC, int LD, int C0, int... J>
 __global__ __launch_bounds__(256) void k_eri_jk(JKArgs a) {
     using CP = ChunkPolicy<LA, LB, LC, LD>;
     const int ch = C0 + static_cast<int>(blockIdx.y);
-    ((ch == C0 + J ? (eri_jk_body<R, LA, LB, LC, LD, CP::ac(C0 + J), CP::cc(C0 + J), CP::cn>(a), 0) : 0), ...);
+    ((ch == C0 + J ? (eri_jk_body<R, LA, LB, LC, LD, CP::ac(C0 + J), CP::an(C0 + J), CP::cc(C0 + J), CP::cn>(a), 0) : 0), ...);
 }
 
 template <int LA, int LB, int LC, int LD, int... I>
 __global__ __launch_bounds__(256) void k_eri_store(StoreArgs a) {
     using CP = ChunkPolicy<LA, LB, LC, LD>;
     const int ch = blockIdx.y;
-    ((ch == I ? (eri_store_body<LA, LB, LC, LD, CP::ac(I), CP::cc(I), CP::cn>(a), 0) : 0), ...);
+    ((ch == I ? (eri_store_body<LA, LB, LC, LD, CP::ac(I), CP::an(I), CP::cc(I), CP::cn>(a), 0) : 0), ...);
 }
 
 template <int LA, int LB, int LC, int LD, int... I>
