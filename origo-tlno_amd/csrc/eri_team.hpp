@@ -10,8 +10,10 @@
 //   phase A  each lane takes one (ket primitive, bra primitive, root) combination
 //            of a batch, evaluates its root and the three G tables and writes
 //            them to LDS;
-//   phase B  each lane owns a slice of the [e|f] intermediates and accumulates
-//            Gx Gy Gz over the batch from LDS;
+//   phase B  each lane owns one bra intermediate e and all [e|f] (the three
+//            table rows of e are one read each per combination) and takes
+//            every NP-th combination of the batch; the NP parts are summed
+//            with cross-lane shuffles after the contraction;
 //   phase C  [e|f] goes to LDS, each lane expands its slice of the outputs by
 //            the binomial form of the HRR, (a, b| = sum_{t <= b} C(b, t) AB^(b-t) [a+t|
 //            (per axis), the same on the ket;
@@ -45,21 +47,39 @@ struct TeamShape {
     static constexpr int NE = ncart_cum(LAB) - (LA > 0 ? ncart_cum(LA - 1) : 0);
     static constexpr int NF = ncart_cum(LCD) - (LC > 0 ? ncart_cum(LC - 1) : 0);
     static constexpr int NI = NE * NF;                     // [e|f] intermediates
-    static constexpr int IPL = (NI + 63) / 64;             // per lane
+    // phase B ownership: lane = (bra intermediate e < EP, part); a lane holds
+    // the NF intermediates [e|f] of its e and takes every NP-th combination
+    static constexpr int EP = NE <= 8 ? 8 : (NE <= 16 ? 16 : 32);
+    static constexpr int NP = 64 / EP;
     static constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
     static constexpr int NOUT = NA * NB * NC * ND;
     static constexpr int OPL = (NOUT + 63) / 64;
     static constexpr int GS = (LAB + 1) * (LCD + 1);       // one axis table
     static constexpr int CS = 3 * GS + 1;                  // one combination (padded)
     static constexpr int CB = TLNO_TEAM_CB;                // combinations per batch
-    // phase B unrolled over combinations where the wave is at one per SIMD
-    // anyway (the LDS addresses then take immediate offsets); elsewhere the
-    // extra registers would cost occupancy
-    static constexpr int UB = IPL >= 16 ? 8 : 1;
     static constexpr int LDS_A = CB * CS;
     static constexpr int LDS_C = NI + NOUT;
     static constexpr int LDS_D = NI + NOUT + NA * NB + NC * ND + NA * NC + NA * ND + NB * NC + NB * ND;
     static constexpr int LDS = LDS_A > (LDS_C > LDS_D ? LDS_C : LDS_D) ? LDS_A : (LDS_C > LDS_D ? LDS_C : LDS_D);
+};
+
+// the triples of the list (|f| from LMIN to LMAX, by |f| then descending
+// lex), as compile-time tables for unrolled loops
+template <int LMIN, int LMAX>
+struct TriTab {
+    static constexpr int N = ncart_cum(LMAX) - (LMIN > 0 ? ncart_cum(LMIN - 1) : 0);
+    int x[N], y[N], z[N];
+    constexpr TriTab() : x(), y(), z() {
+        int p = 0;
+        for (int L = LMIN; L <= LMAX; ++L)
+            for (int dx = 0; dx <= L; ++dx)
+                for (int k = 0; k <= dx; ++k) {
+                    x[p] = L - dx;
+                    y[p] = dx - k;
+                    z[p] = L - x[p] - y[p];
+                    ++p;
+                }
+    }
 };
 
 // triple of list position p (|e| from LMIN), decoded arithmetically
@@ -91,22 +111,16 @@ __device__ __forceinline__ void eri_team_quartet(const JKArgs& a, double* __rest
     const int lane = threadIdx.x & 63;
     const PairData& pd = a.pd;
 
-    // owned intermediates: LDS offsets of their G factors inside a combination
-    int ox[T::IPL], oy[T::IPL], oz[T::IPL];
+    // owned intermediates [e|f] for all f: their factors are the rows
+    // Gx[ex][*], Gy[ey][*], Gz[ez][*] of a combination's tables
+    constexpr TriTab<LC, LC + LD> ft{};
+    const int e_own = lane % T::EP, part = lane / T::EP;
+    int ex, ey, ez;
+    tri_decode(LA, e_own < T::NE ? e_own : 0, ex, ey, ez);
+    const int rx0 = ex * (T::LCD + 1), ry0 = T::GS + ey * (T::LCD + 1), rz0 = 2 * T::GS + ez * (T::LCD + 1);
+    double acc[T::NF];
 #pragma unroll
-    for (int k = 0; k < T::IPL; ++k) {
-        int i = lane + 64 * k;
-        if (i >= T::NI) i = T::NI - 1;
-        int ex, ey, ez, fx, fy, fz;
-        tri_decode(LA, i / T::NF, ex, ey, ez);
-        tri_decode(LC, i % T::NF, fx, fy, fz);
-        ox[k] = ex * (T::LCD + 1) + fx;
-        oy[k] = T::GS + ey * (T::LCD + 1) + fy;
-        oz[k] = 2 * T::GS + ez * (T::LCD + 1) + fz;
-    }
-    double acc[T::IPL];
-#pragma unroll
-    for (int k = 0; k < T::IPL; ++k) acc[k] = 0.0;
+    for (int f = 0; f < T::NF; ++f) acc[f] = 0.0;
 
     const double Ax = pd.A[3 * P], Ay = pd.A[3 * P + 1], Az = pd.A[3 * P + 2];
     const double Cx = pd.A[3 * Q], Cy = pd.A[3 * Q + 1], Cz = pd.A[3 * Q + 2];
@@ -172,24 +186,38 @@ __device__ __forceinline__ void eri_team_quartet(const JKArgs& a, double* __rest
             }
         }
         __syncthreads();
-        // ---- phase B: accumulate the owned intermediates over the batch
+        // ---- phase B: accumulate the owned intermediates over this part's
+        // combinations of the batch (three LDS rows per combination)
         const int nb = min(T::CB, ncomb - c0);
-#pragma unroll T::UB
-        for (int j = 0; j < nb; ++j) {
+        for (int j = part; j < nb; j += T::NP) {
             const double* g = lds + j * T::CS;
+            double rx[T::LCD + 1], ry[T::LCD + 1], rz[T::LCD + 1];
 #pragma unroll
-            for (int k = 0; k < T::IPL; ++k) acc[k] = fma(g[ox[k]] * g[oy[k]], g[oz[k]], acc[k]);
+            for (int t = 0; t <= T::LCD; ++t) {
+                rx[t] = g[rx0 + t];
+                ry[t] = g[ry0 + t];
+                rz[t] = g[rz0 + t];
+            }
+#pragma unroll
+            for (int f = 0; f < T::NF; ++f) acc[f] = fma(rx[ft.x[f]] * ry[ft.y[f]], rz[ft.z[f]], acc[f]);
         }
         __syncthreads();
     }
 
     // ---- phase C: HRR from [e|f] to the unit-diagonal Cartesian block
-    double* ef = lds;             // NI
+    double* ef = lds;             // NI, [e][f]
     double* blk = lds + T::NI;    // NOUT, [a][b][c][d]
+    // sum the parts (lane bits above EP) into part 0, which stores its row
 #pragma unroll
-    for (int k = 0; k < T::IPL; ++k) {
-        const int i = lane + 64 * k;
-        if (i < T::NI) ef[i] = acc[k];
+    for (int f = 0; f < T::NF; ++f) {
+        double v = acc[f];
+#pragma unroll
+        for (int m = T::EP; m < 64; m <<= 1) v += __shfl_xor(v, m, 64);
+        acc[f] = v;
+    }
+    if (part == 0 && e_own < T::NE) {
+#pragma unroll
+        for (int f = 0; f < T::NF; ++f) ef[e_own * T::NF + f] = acc[f];
     }
     __syncthreads();
     const double AB[3] = {pd.AB[3 * P], pd.AB[3 * P + 1], pd.AB[3 * P + 2]};
