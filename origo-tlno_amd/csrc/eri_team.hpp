@@ -24,10 +24,6 @@
 
 #include "eri_device.hpp"
 
-#ifndef TLNO_TEAM_CB
-#define TLNO_TEAM_CB 64
-#endif
-
 namespace tlno_amd {
 
 constexpr int ncart_cum(int L) { return (L + 1) * (L + 2) * (L + 3) / 6; }  // sum_{l <= L} ncart(l)
@@ -56,7 +52,9 @@ struct TeamShape {
     static constexpr int OPL = (NOUT + 63) / 64;
     static constexpr int GS = (LAB + 1) * (LCD + 1);       // one axis table
     static constexpr int CS = 3 * GS + 1;                  // one combination (padded)
-    static constexpr int CB = TLNO_TEAM_CB;                // combinations per batch
+    // combinations per batch: 64 (one per lane) while their tables fit in
+    // 20 KB of LDS, else 32, so that two waves per SIMD keep fitting in LDS
+    static constexpr int CB = 64 * CS * 8 <= 20 * 1024 ? 64 : 32;
     static constexpr int LDS_A = CB * CS;
     static constexpr int LDS_C = NI + NOUT;
     static constexpr int LDS_D = NI + NOUT + NA * NB + NC * ND + NA * NC + NA * ND + NB * NC + NB * ND;
@@ -278,11 +276,13 @@ __device__ __forceinline__ void eri_team_quartet(const JKArgs& a, double* __rest
             double v = 0.0;
             if (t < NA * NB) {
                 const int ai = t / NB, bi = t % NB;
+                #pragma unroll 1
                 for (int ci = 0; ci < NC; ++ci)
                     for (int di = 0; di < ND; ++di) v = fma(B(ai, bi, ci, di), dcd[ci * ND + di], v);
                 unsafeAtomicAdd(Jm + (A0 + ai) * M + B0 + bi, 2.0 * s * v);
             } else {
                 const int u = t - NA * NB, ci = u / ND, di = u % ND;
+                #pragma unroll 1
                 for (int ai = 0; ai < NA; ++ai)
                     for (int bi = 0; bi < NB; ++bi) v = fma(B(ai, bi, ci, di), dab[ai * NB + bi], v);
                 unsafeAtomicAdd(Jm + (C0 + ci) * M + D0 + di, 2.0 * s * v);
@@ -297,21 +297,25 @@ __device__ __forceinline__ void eri_team_quartet(const JKArgs& a, double* __rest
             double v = 0.0;
             if (t < n0) {
                 const int ai = t / NC, ci = t % NC;
+                #pragma unroll 1
                 for (int bi = 0; bi < NB; ++bi)
                     for (int di = 0; di < ND; ++di) v = fma(B(ai, bi, ci, di), dbd[bi * ND + di], v);
                 unsafeAtomicAdd(Km + (A0 + ai) * M + C0 + ci, s * v);
             } else if (t < n0 + n1) {
                 const int u = t - n0, ai = u / ND, di = u % ND;
+                #pragma unroll 1
                 for (int bi = 0; bi < NB; ++bi)
                     for (int ci = 0; ci < NC; ++ci) v = fma(B(ai, bi, ci, di), dbc[bi * NC + ci], v);
                 unsafeAtomicAdd(Km + (A0 + ai) * M + D0 + di, s * v);
             } else if (t < n0 + n1 + n2) {
                 const int u = t - n0 - n1, bi = u / NC, ci = u % NC;
+                #pragma unroll 1
                 for (int ai = 0; ai < NA; ++ai)
                     for (int di = 0; di < ND; ++di) v = fma(B(ai, bi, ci, di), dad[ai * ND + di], v);
                 unsafeAtomicAdd(Km + (B0 + bi) * M + C0 + ci, s * v);
             } else {
                 const int u = t - n0 - n1 - n2, bi = u / ND, di = u % ND;
+                #pragma unroll 1
                 for (int ai = 0; ai < NA; ++ai)
                     for (int ci = 0; ci < NC; ++ci) v = fma(B(ai, bi, ci, di), dac[ai * NC + ci], v);
                 unsafeAtomicAdd(Km + (B0 + bi) * M + D0 + di, s * v);
