@@ -1191,6 +1191,15 @@ int wave_split(int split, int nq) {
     return (split + lanes_per_ket - 1) / lanes_per_ket;
 }
 
+// Classes whose tasks give fewer waves than this split each task's bra
+// primitive pairs over several waves.  Every slice digests its own partial
+// block (one FP64 atomic per J/K element), so the split trades parallelism
+// for memory-side atomics: 4096 measured best (organic30 18.2 ms at 16384,
+// 15.3 ms at 4096, 16.1 ms without splitting).
+#ifndef TLNO_TARGET_WAVES
+#define TLNO_TARGET_WAVES 4096
+#endif
+
 // bucket of a primitive-pair count: half octaves, so counts inside one
 // bucket differ by less than a factor sqrt(2)
 inline int prim_bucket(int pn) {
@@ -1284,7 +1293,7 @@ TaskSet& get_tasks(tlno_gpu_plan* pl, double thr) {
                 // Too few waves to fill 256 CUs (rare, heavy classes): split the
                 // bra primitive pairs of each task into slices of their own.
                 const long long waves = static_cast<long long>(base.size()) * g_kernels[X][Y].nchunk;
-                constexpr long long kTargetWaves = 16384;
+                constexpr long long kTargetWaves = TLNO_TARGET_WAVES;
                 int split = 1;
                 if (waves > 0 && waves < kTargetWaves)
                     split = static_cast<int>(std::min<long long>(256, (kTargetWaves + waves - 1) / waves));
@@ -1440,7 +1449,7 @@ void ensure_route(tlno_gpu_plan* pl, TaskSet& ts, double thr) {
 // spreads over the machine (same policy as the range-mode task split)
 int list_split(long long ntask, int nchunk) {
     const long long waves = ntask * nchunk;
-    constexpr long long kTargetWaves = 16384;
+    constexpr long long kTargetWaves = TLNO_TARGET_WAVES;
     if (waves <= 0 || waves >= kTargetWaves) return 1;
     return static_cast<int>(std::min<long long>(256, (kTargetWaves + waves - 1) / waves));
 }
@@ -2725,7 +2734,7 @@ tlno_gpu_batch::Tasks& batch_tasks(tlno_gpu_batch* bt, double thr) {
                 }
             }
             const long long waves = static_cast<long long>(base.size()) * g_kernels[X][Y].nchunk;
-            constexpr long long kTargetWaves = 16384;
+            constexpr long long kTargetWaves = TLNO_TARGET_WAVES;
             int split = 1;
             if (waves > 0 && waves < kTargetWaves)
                 split = static_cast<int>(std::min<long long>(256, (kTargetWaves + waves - 1) / waves));
