@@ -1371,10 +1371,12 @@ inline int class_team_min(const tlno_gpu_plan* pl, int X, int Y) {
 }
 
 // Team evaluator waves per shell quartet for a class with nq quartets: a class
-// too small to give every SIMD a few waves splits each quartet's primitive
-// combinations over several waves (each repeats the transfer and digestion).
+// too small to give every SIMD a wave splits each quartet's primitive
+// combinations over several waves (each repeats the transfer and digestion,
+// atomics included, so the target stays low: 1024 measured best of 1024 to
+// 4096).
 #ifndef TLNO_TEAM_TARGET_WAVES
-#define TLNO_TEAM_TARGET_WAVES 4096
+#define TLNO_TEAM_TARGET_WAVES 1024
 #endif
 #ifndef TLNO_TEAM_MAX_SLICES
 #define TLNO_TEAM_MAX_SLICES 8
