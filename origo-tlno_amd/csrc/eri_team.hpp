@@ -102,10 +102,6 @@ __device__ __forceinline__ void eri_team_quartet(const JKArgs& a, double* __rest
                                                  double* __restrict__ Jm, double* __restrict__ Km,
                                                  const double* __restrict__ Dm, long M, int slice, int ns) {
     using T = TeamShape<LA, LB, LC, LD>;
-    constexpr CartT<LA> ca = make_cart<LA>();
-    constexpr CartT<LB> cb = make_cart<LB>();
-    constexpr CartT<LC> cc = make_cart<LC>();
-    constexpr CartT<LD> cd = make_cart<LD>();
     const int lane = threadIdx.x & 63;
     const PairData& pd = a.pd;
 
@@ -225,10 +221,13 @@ __device__ __forceinline__ void eri_team_quartet(const JKArgs& a, double* __rest
     for (int o = lane; o < T::NOUT; o += 64) {
         const int d = o % T::ND, c = (o / T::ND) % T::NC, b = (o / (T::ND * T::NC)) % T::NB,
                   a0 = o / (T::ND * T::NC * T::NB);
-        const int axc = ca.e[0][a0], ayc = ca.e[1][a0], azc = ca.e[2][a0];
-        const int bx = cb.e[0][b], by = cb.e[1][b], bz = cb.e[2][b];
-        const int cxc = cc.e[0][c], cyc = cc.e[1][c], czc = cc.e[2][c];
-        const int dx = cd.e[0][d], dy = cd.e[1][d], dz = cd.e[2][d];
+        // exponents decoded arithmetically: a lane-dependent index into a
+        // constexpr table would put the table in per-lane scratch memory
+        int axc, ayc, azc, bx, by, bz, cxc, cyc, czc, dx, dy, dz;
+        tri_decode(LA, a0, axc, ayc, azc);
+        tri_decode(LB, b, bx, by, bz);
+        tri_decode(LC, c, cxc, cyc, czc);
+        tri_decode(LD, d, dx, dy, dz);
         double v = 0.0;
         for (int tx = 0; tx <= bx; ++tx)
             for (int ty = 0; ty <= by; ++ty)
@@ -243,7 +242,8 @@ __device__ __forceinline__ void eri_team_quartet(const JKArgs& a, double* __rest
                                         row[tri_index(LC, cxc + ux, cyc + uy, czc + uz)], u);
                     v = fma(wb, u, v);
                 }
-        blk[o] = v * (ca.nang[a0] * cb.nang[b] * cc.nang[c] * cd.nang[d]);
+        auto nang = [](int x, int y, int z) { return inv_sqrt_small(dfact2m1(x) * dfact2m1(y) * dfact2m1(z)); };
+        blk[o] = v * (nang(axc, ayc, azc) * nang(bx, by, bz) * nang(cxc, cyc, czc) * nang(dx, dy, dz));
     }
     __syncthreads();
 
