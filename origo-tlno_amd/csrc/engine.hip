@@ -1253,6 +1253,81 @@ void bucket_list_tasks(const tlno_gpu_plan* pl, double thr, int X, int Y, const 
     }
 }
 
+// Bra primitive slices of one class's list-mode tasks, balanced by cost.
+// The lane evaluator launches fewer waves than the machine holds, so a
+// launch lasts as long as its slowest wave, and a wave's time is that of its
+// busiest lane: ceil(bra pairs of the slice / lanes per ket) x ket pairs (the
+// flattened grid for flat_prims classes: ceil(bra x ket pairs / lanes per
+// ket)).  The smallest per-wave cost C is chosen for which the slices
+// (each task cut into the fewest equal slices costing at most C) number at
+// most max(target, 5/4 of the tasks) waves over the class's chunks; each
+// slice digests its own partial block, which bounds the split.  Slices stay
+// adjacent, so the bound order of the list (the FP32 suffix) is kept.
+void balance_list_slices(const tlno_gpu_plan* pl, const std::vector<int>& bk, int X, int Y,
+                         const std::vector<int4>& in, const std::vector<double>& in_bound, std::vector<int4>& out,
+                         std::vector<double>& out_bound) {
+    const bool flat = flat_prims(kClassLAB[X], 0, kClassLAB[Y], 0);
+    const long long nchunk = std::max(1, g_kernels[X][Y].nchunk);
+    const std::size_t n = in.size();
+    std::vector<int> pbn(n), pkn(n), ibs(n);
+    for (std::size_t i = 0; i < n; ++i) {
+        const int4& t = in[i];
+        const long long off = static_cast<long long>(static_cast<unsigned>(t.y)) | (static_cast<long long>(t.z >> 8) << 32);
+        const int nq = t.z & 255;
+        int mk = 1;
+        for (int j = 0; j < nq; ++j) mk = std::max(mk, pl->pn_host[bk[off + j]]);
+        int ks = 1;
+        while (ks < nq && ks < 64) ks <<= 1;
+        pbn[i] = std::max(1, pl->pn_host[t.x]);
+        pkn[i] = mk;
+        ibs[i] = 64 / ks;
+    }
+    // fewest slices of task i whose busiest lane costs at most C
+    auto slices = [&](std::size_t i, double C) {
+        long long nbmax;  // bra pairs per slice within the cost
+        if (flat) nbmax = static_cast<long long>(std::floor(C)) * ibs[i] / pkn[i];
+        else nbmax = static_cast<long long>(std::floor(C / pkn[i])) * ibs[i];
+        if (nbmax < 1) return pbn[i];
+        return static_cast<int>(std::min<long long>(pbn[i], (pbn[i] + nbmax - 1) / nbmax));
+    };
+    auto cost1 = [&](std::size_t i) {
+        return flat ? std::ceil(static_cast<double>(pbn[i]) * pkn[i] / ibs[i])
+                    : std::ceil(static_cast<double>(pbn[i]) / ibs[i]) * pkn[i];
+    };
+    static const long long target = [] {
+        const char* e = std::getenv("TLNO_LIST_TARGET_WAVES");
+        return e ? std::max(1LL, std::atoll(e)) : static_cast<long long>(TLNO_TARGET_WAVES);
+    }();
+    const long long cap = std::max(target, static_cast<long long>(n) * nchunk * 5 / 4);
+    double lo = 1.0, hi = 1.0;
+    for (std::size_t i = 0; i < n; ++i) hi = std::max(hi, cost1(i));
+    auto count = [&](double C) {
+        long long w = 0;
+        for (std::size_t i = 0; i < n; ++i) w += slices(i, C);
+        return w * nchunk;
+    };
+    if (count(lo) > cap) {
+        for (int it = 0; it < 48 && hi - lo > 0.5; ++it) {
+            const double mid = 0.5 * (lo + hi);
+            if (count(mid) > cap) lo = mid;
+            else hi = mid;
+        }
+    } else {
+        hi = lo;
+    }
+    for (std::size_t i = 0; i < n; ++i) {
+        const int s = slices(i, hi);
+        const int step = (pbn[i] + s - 1) / s;
+        for (int i0 = 0; i0 < pbn[i]; i0 += step) {
+            const int i1 = std::min(pbn[i], i0 + step);
+            int4 t = in[i];
+            t.w = i0 | (i1 << 16);
+            out.push_back(t);
+            out_bound.push_back(in_bound[i]);
+        }
+    }
+}
+
 TaskSet& get_tasks(tlno_gpu_plan* pl, double thr) {
     auto it = pl->tasks.find(thr);
     if (it != pl->tasks.end()) return it->second;
@@ -1352,8 +1427,12 @@ TaskSet& get_tasks(tlno_gpu_plan* pl, double thr) {
     for (int X = 0; X < 6; ++X)
         for (int Y = 0; Y <= X; ++Y) {
             ts.loff[X][Y] = static_cast<long long>(lall.size());
-            if (!kgroups[Y].empty() && ts.cls_sq[X][Y] > 0)
-                bucket_list_tasks(pl, thr, X, Y, bk, kgroups[Y], lall, ts.lbmax[X][Y]);
+            if (!kgroups[Y].empty() && ts.cls_sq[X][Y] > 0) {
+                std::vector<int4> cls;
+                std::vector<double> bound;
+                bucket_list_tasks(pl, thr, X, Y, bk, kgroups[Y], cls, bound);
+                balance_list_slices(pl, bk, X, Y, cls, bound, lall, ts.lbmax[X][Y]);
+            }
             ts.lcnt[X][Y] = static_cast<long long>(lall.size()) - ts.loff[X][Y];
         }
     if (!lall.empty()) {
@@ -1637,14 +1716,14 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
                     if (n64 > 0) {
                         al.tasks = ts.d_ltasks + ts.loff[X][Y];
                         al.ntasks = static_cast<int>(n64);
-                        al.split = list_split(n64, k.nchunk);
-                        add_jobs(X, Y, false, al, n64 * al.split, class_team_min(pl, X, Y), c * n64);
+                        al.split = 0;  // the tasks carry their cost-balanced slices
+                        add_jobs(X, Y, false, al, n64, class_team_min(pl, X, Y), c * n64);
                     }
                     if (nt > n64) {
                         al.tasks = ts.d_ltasks + ts.loff[X][Y] + n64;
                         al.ntasks = static_cast<int>(nt - n64);
-                        al.split = list_split(nt - n64, k.nchunk);
-                        add_jobs(X, Y, true, al, (nt - n64) * al.split, 0, c * (nt - n64));
+                        al.split = 0;
+                        add_jobs(X, Y, true, al, nt - n64, 0, c * (nt - n64));
                     }
                     continue;
                 }

@@ -425,9 +425,11 @@ struct JKArgs {
     PairData pd;
     DevTables tb;
     const int4* tasks;  // range mode {P, q_begin, q_end, slice};
-                        // list mode {P, offset bits 0-31, count | offset bits 32-55 << 8, unused}
+                        // list mode {P, offset bits 0-31, count | offset bits 32-55 << 8,
+                        //            bra primitive slice ib0 | ib1 << 16 (split == 0)}
     int ntasks;
-    int split;          // list mode: waves per task (bra primitive slices)
+    int split;          // list mode: waves per task (bra primitive slices); 0: one wave per
+                        // task, on the slice the task names
     const double* D;    // internal Cartesian density, M x M
     double* J;          // Jacc
     double* K;          // Kacc
@@ -487,7 +489,7 @@ template <typename R, int LA, int LB, int LC, int LD, int AC, int AN, int CC, in
 __device__ __forceinline__ void eri_jk_task(const JKArgs& a, const int wave) {
     using S = Shape<LA, LB, LC, LD, AC, AN, CC, CN>;
     const int lane = threadIdx.x & 63;
-    const int task = a.qlist ? wave / a.split : wave;
+    const int task = a.qlist && a.split > 0 ? wave / a.split : wave;
     if (task >= a.ntasks) return;  // whole wave leaves together
     const int4 t = a.tasks[task];
     // the task is wave-uniform: keep it in scalar registers
@@ -495,18 +497,26 @@ __device__ __forceinline__ void eri_jk_task(const JKArgs& a, const int wave) {
     bool active;
     int Q, ib0, ib1, nq, ks;
     if (a.qlist) {
-        // list mode: up to 64 kets at qlist[offset ..], bra primitive slice
-        // (wave % split) of split near-equal slices
+        // list mode: up to 64 kets at qlist[offset ..]; bra primitive slice
+        // (wave % split) of split near-equal slices, or (split == 0) the
+        // task's own slice [ib0, ib1) = (w & 0xffff, w >> 16)
         const long long off = static_cast<long long>(static_cast<unsigned>(__builtin_amdgcn_readfirstlane(t.y))) |
                               (static_cast<long long>(__builtin_amdgcn_readfirstlane(t.z) >> 8) << 32);
         nq = __builtin_amdgcn_readfirstlane(t.z) & 255;
+        if (nq <= 0) return;
         const int pbn = a.pd.pn[P];
-        const int ns = min(a.split, pbn);
-        const int slice = wave - task * a.split;
-        if (nq <= 0 || slice >= ns) return;
-        const int step = (pbn + ns - 1) / ns;
-        ib0 = slice * step;
-        ib1 = min(pbn, ib0 + step);
+        if (a.split > 0) {
+            const int ns = min(a.split, pbn);
+            const int slice = wave - task * a.split;
+            if (slice >= ns) return;
+            const int step = (pbn + ns - 1) / ns;
+            ib0 = slice * step;
+            ib1 = min(pbn, ib0 + step);
+        } else {
+            const int tw = __builtin_amdgcn_readfirstlane(t.w);
+            ib0 = tw & 0xffff;
+            ib1 = min(pbn, (tw >> 16) & 0xffff);
+        }
         if (ib0 >= ib1) return;
         ks = ket_stride(nq);
         const int kq = lane & (ks - 1);
