@@ -1263,6 +1263,8 @@ void bucket_list_tasks(const tlno_gpu_plan* pl, double thr, int X, int Y, const 
 // most max(target, 5/4 of the tasks) waves over the class's chunks; each
 // slice digests its own partial block, which bounds the split.  Slices stay
 // adjacent, so the bound order of the list (the FP32 suffix) is kept.
+// Measured on organic30 (autotuned build): 15.4 -> 14.65 ms; the classes'
+// serialized launches 15-40 % shorter.
 void balance_list_slices(const tlno_gpu_plan* pl, const std::vector<int>& bk, int X, int Y,
                          const std::vector<int4>& in, const std::vector<double>& in_bound, std::vector<int4>& out,
                          std::vector<double>& out_bound) {
@@ -1298,7 +1300,10 @@ void balance_list_slices(const tlno_gpu_plan* pl, const std::vector<int>& bk, in
         const char* e = std::getenv("TLNO_LIST_TARGET_WAVES");
         return e ? std::max(1LL, std::atoll(e)) : static_cast<long long>(TLNO_TARGET_WAVES);
     }();
-    const long long cap = std::max(target, static_cast<long long>(n) * nchunk * 5 / 4);
+    // a launch of many times the target is balanced by the hardware's
+    // dynamic dispatch: no split (water40 FP64 measured 0.6 % slower split)
+    const long long nw = static_cast<long long>(n) * nchunk;
+    const long long cap = nw >= 8 * target ? nw : std::max(target, nw * 5 / 4);
     double lo = 1.0, hi = 1.0;
     for (std::size_t i = 0; i < n; ++i) hi = std::max(hi, cost1(i));
     auto count = [&](double C) {

@@ -56,8 +56,12 @@ struct TeamShape {
     // 20 KB of LDS, else 32, so that two waves per SIMD keep fitting in LDS
     static constexpr int CB = 64 * CS * 8 <= 20 * 1024 ? 64 : 32;
     static constexpr int LDS_A = CB * CS;
-    static constexpr int LDS_C = NI + NOUT;
-    static constexpr int LDS_D = NI + NOUT + NA * NB + NC * ND + NA * NC + NA * ND + NB * NC + NB * ND;
+    // phase C: [e|f] then the block share [0, R0); the ket-transformed
+    // [e|cd] sits after them; phase D: the block, then six density blocks
+    static constexpr int NX = NE * NC * ND;
+    static constexpr int R0 = NI > NOUT ? NI : NOUT;
+    static constexpr int LDS_C = R0 + NX;
+    static constexpr int LDS_D = NOUT + NA * NB + NC * ND + NA * NC + NA * ND + NB * NC + NB * ND;
     static constexpr int LDS = LDS_A > (LDS_C > LDS_D ? LDS_C : LDS_D) ? LDS_A : (LDS_C > LDS_D ? LDS_C : LDS_D);
 };
 
@@ -198,9 +202,11 @@ __device__ __forceinline__ void eri_team_quartet(const JKArgs& a, double* __rest
         __syncthreads();
     }
 
-    // ---- phase C: HRR from [e|f] to the unit-diagonal Cartesian block
+    // ---- phase C: HRR from [e|f] to the unit-diagonal Cartesian block, ket
+    // side first ([e|f] -> [e|cd] for every e), then bra side
     double* ef = lds;             // NI, [e][f]
-    double* blk = lds + T::NI;    // NOUT, [a][b][c][d]
+    double* xk = lds + T::R0;     // NX, [e][c][d]
+    double* blk = lds;            // NOUT, [a][b][c][d] (over [e|f], read before)
     // sum the parts (lane bits above EP) into part 0, which stores its row
 #pragma unroll
     for (int f = 0; f < T::NF; ++f) {
@@ -216,33 +222,41 @@ __device__ __forceinline__ void eri_team_quartet(const JKArgs& a, double* __rest
     __syncthreads();
     const double AB[3] = {pd.AB[3 * P], pd.AB[3 * P + 1], pd.AB[3 * P + 2]};
     const double CD[3] = {pd.AB[3 * Q], pd.AB[3 * Q + 1], pd.AB[3 * Q + 2]};
-    // binomial HRR weight C(n, t) X^(n - t) for n <= 2
+    // binomial HRR weight C(n, t) X^(n - t) for n <= 2; exponents are decoded
+    // arithmetically (a lane-dependent index into a constexpr table would put
+    // the table in per-lane scratch memory)
     auto hw = [](int n, int t, double X) { return t == n ? 1.0 : (n - t == 1 ? n * X : X * X); };
+    // [e|c d) = sum_u C(d, u) CD^(d - u) [e|c + u]
+    for (int i = lane; i < T::NX; i += 64) {
+        const int d = i % T::ND, c = (i / T::ND) % T::NC, e = i / (T::ND * T::NC);
+        int cxc, cyc, czc, dx, dy, dz;
+        tri_decode(LC, c, cxc, cyc, czc);
+        tri_decode(LD, d, dx, dy, dz);
+        const double* row = ef + e * T::NF;
+        double u = 0.0;
+        for (int ux = 0; ux <= dx; ++ux)
+            for (int uy = 0; uy <= dy; ++uy)
+                for (int uz = 0; uz <= dz; ++uz)
+                    u = fma(hw(dx, ux, CD[0]) * hw(dy, uy, CD[1]) * hw(dz, uz, CD[2]),
+                            row[tri_index(LC, cxc + ux, cyc + uy, czc + uz)], u);
+        xk[i] = u;
+    }
+    __syncthreads();
+    // (a b| = sum_t C(b, t) AB^(b - t) (a + t|, then the unit-diagonal norms
+    auto nang = [](int x, int y, int z) { return inv_sqrt_small(dfact2m1(x) * dfact2m1(y) * dfact2m1(z)); };
     for (int o = lane; o < T::NOUT; o += 64) {
-        const int d = o % T::ND, c = (o / T::ND) % T::NC, b = (o / (T::ND * T::NC)) % T::NB,
-                  a0 = o / (T::ND * T::NC * T::NB);
-        // exponents decoded arithmetically: a lane-dependent index into a
-        // constexpr table would put the table in per-lane scratch memory
+        const int cd_ = o % (T::NC * T::ND), b = (o / (T::NC * T::ND)) % T::NB, a0 = o / (T::NC * T::ND * T::NB);
         int axc, ayc, azc, bx, by, bz, cxc, cyc, czc, dx, dy, dz;
         tri_decode(LA, a0, axc, ayc, azc);
         tri_decode(LB, b, bx, by, bz);
-        tri_decode(LC, c, cxc, cyc, czc);
-        tri_decode(LD, d, dx, dy, dz);
+        tri_decode(LC, cd_ / T::ND, cxc, cyc, czc);
+        tri_decode(LD, cd_ % T::ND, dx, dy, dz);
         double v = 0.0;
         for (int tx = 0; tx <= bx; ++tx)
             for (int ty = 0; ty <= by; ++ty)
-                for (int tz = 0; tz <= bz; ++tz) {
-                    const double wb = hw(bx, tx, AB[0]) * hw(by, ty, AB[1]) * hw(bz, tz, AB[2]);
-                    const double* row = ef + tri_index(LA, axc + tx, ayc + ty, azc + tz) * T::NF;
-                    double u = 0.0;
-                    for (int ux = 0; ux <= dx; ++ux)
-                        for (int uy = 0; uy <= dy; ++uy)
-                            for (int uz = 0; uz <= dz; ++uz)
-                                u = fma(hw(dx, ux, CD[0]) * hw(dy, uy, CD[1]) * hw(dz, uz, CD[2]),
-                                        row[tri_index(LC, cxc + ux, cyc + uy, czc + uz)], u);
-                    v = fma(wb, u, v);
-                }
-        auto nang = [](int x, int y, int z) { return inv_sqrt_small(dfact2m1(x) * dfact2m1(y) * dfact2m1(z)); };
+                for (int tz = 0; tz <= bz; ++tz)
+                    v = fma(hw(bx, tx, AB[0]) * hw(by, ty, AB[1]) * hw(bz, tz, AB[2]),
+                            xk[tri_index(LA, axc + tx, ayc + ty, azc + tz) * (T::NC * T::ND) + cd_], v);
         blk[o] = v * (nang(axc, ayc, azc) * nang(bx, by, bz) * nang(cxc, cyc, czc) * nang(dx, dy, dz));
     }
     __syncthreads();
