@@ -289,6 +289,10 @@ struct TaskSet {
     long long qtotal = 0;
     long long toff[36] = {}, tcap[36] = {};
     long long tcap_total = 0;
+    // team evaluator work lists: per class pair, the shell quartets {P, Q}
+    // of its range tasks in task order (cum_sq gives the prefix of the first
+    // n tasks); built on the first team launch of the class
+    int2* d_tq[6][6] = {};
 };
 
 }  // namespace tlno_amd
@@ -540,6 +544,19 @@ __global__ void k_fill(double* p, long n, double v) {
     if (i < n) p[i] = v;
 }
 
+// Shell quartets of range tasks, one block of 64 threads per task: lane k
+// of task i writes {P, q_begin + k} at cum[i] + k for its kets (tasks of a
+// later bra primitive slice list none: slice 0 holds the task's quartets)
+__global__ void k_team_quartets(const int4* __restrict__ tasks, int ntasks, const long long* __restrict__ cum,
+                                int2* __restrict__ out) {
+    const int i = blockIdx.x, k = threadIdx.x;
+    if (i >= ntasks) return;
+    const int4 t = tasks[i];
+    if ((t.w & 0xffff) != 0) return;
+    const int nq = min(t.z - t.y, 64);
+    if (k < nq) out[cum[i] + k] = make_int2(t.x, t.y + k);
+}
+
 template <int MMAX>
 __global__ void k_boys(DevTables tb, const double* T, double* out, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -579,9 +596,10 @@ void launch_jk(const ClassKernels& k, bool fp32, JKArgs& args, long long nwaves,
     if (use_team(k, fp32, args, team_min_out)) {  // one block per shell quartet and slice (64 per task)
         args.stride = 0;
         const long long ns = std::max(1, args.team_slices);
-        HIP_OK(hipLaunchKernel(reinterpret_cast<const void*>(k.team),
-                               dim3(static_cast<unsigned>(static_cast<long long>(args.ntasks) * 64 * ns)), dim3(64),
-                               params, 0, st));
+        const long long items = args.tquart ? args.ntquart : static_cast<long long>(args.ntasks) * 64;
+        if (items * ns > 0x7fffffffLL) throw std::runtime_error("tlno_gpu: team launch too large for one grid");
+        HIP_OK(hipLaunchKernel(reinterpret_cast<const void*>(k.team), dim3(static_cast<unsigned>(items * ns)),
+                               dim3(64), params, 0, st));
         return;
     }
     const unsigned grid = grid_for(nwaves * 64, 256);
@@ -1551,6 +1569,26 @@ double class_cost(const tlno_gpu_plan* pl, const TaskSet& ts, int X, int Y) {
     return static_cast<double>(ts.cls_pq[X][Y]) * work * 1e-10;
 }
 
+// The team evaluator's quartet list of class pair (X, Y), expanded on the
+// device from the range tasks on first use (one-off: allocates, then waits
+// for the plan's stream so the staging buffer can go).
+const int2* ensure_team_quartets(tlno_gpu_plan* pl, TaskSet& ts, int X, int Y) {
+    if (ts.d_tq[X][Y]) return ts.d_tq[X][Y];
+    const std::vector<long long>& cum = ts.cum_sq[X][Y];
+    const long long nt = ts.cnt[X][Y];
+    if (nt <= 0 || static_cast<long long>(cum.size()) != nt + 1 || cum.back() <= 0) return nullptr;
+    long long* d_cum = nullptr;
+    HIP_OK(hipMalloc(&ts.d_tq[X][Y], cum.back() * sizeof(int2)));
+    HIP_OK(hipMalloc(&d_cum, cum.size() * sizeof(long long)));
+    HIP_OK(hipMemcpyAsync(d_cum, cum.data(), cum.size() * sizeof(long long), hipMemcpyHostToDevice, pl->stream));
+    k_team_quartets<<<static_cast<unsigned>(nt), 64, 0, pl->stream>>>(ts.d_tasks + ts.off[X][Y],
+                                                                       static_cast<int>(nt), d_cum, ts.d_tq[X][Y]);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(pl->stream));
+    HIP_OK(hipFree(d_cum));
+    return ts.d_tq[X][Y];
+}
+
 void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, double thr) {
     DeviceCtx& ctx = device_ctx(pl->device);
     HIP_OK(hipSetDevice(pl->device));
@@ -1748,7 +1786,16 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
                     a.tasks = ts.d_tasks + ts.off[X][Y];
                     a.ntasks = static_cast<int>(n64);
                     a.team_slices = team_slices_for(ts.cls_sq[X][Y]);
+                    // the team evaluator takes one wave per listed quartet
+                    // (the range tasks hold empty ket slots and the lane
+                    // evaluator's bra primitive slices)
+                    if (use_team(g_kernels[X][Y], false, a, class_team_min(pl, X, Y))) {
+                        a.tquart = ensure_team_quartets(pl, ts, X, Y);
+                        a.ntquart = a.tquart ? ts.cum_sq[X][Y][n64] : 0;
+                    }
                     add_jobs(X, Y, false, a, n64, class_team_min(pl, X, Y), c * n64);
+                    a.tquart = nullptr;
+                    a.ntquart = 0;
                 }
                 if (nt > n64) {
                     a.tasks = ts.d_tasks + ts.off[X][Y] + n64;
@@ -2060,6 +2107,8 @@ tlno_gpu_plan::~tlno_gpu_plan() {
         f(kv.second.d_ltasks);
         f(kv.second.d_rows);
         f(kv.second.d_rbase);
+        for (int X = 0; X < 6; ++X)
+            for (int Y = 0; Y < 6; ++Y) f(kv.second.d_tq[X][Y]);
     }
     if (timing)
         for (int X = 0; X < 6; ++X)

@@ -455,6 +455,11 @@ struct JKArgs {
     // team evaluator: waves per shell quartet, each taking a slice of its
     // primitive combinations (0 or 1: one wave)
     int team_slices;
+    // team evaluator, quartet mode (tquart != nullptr): work item i is the
+    // shell quartet {P, Q} = tquart[i], i < ntquart (no task decoding, no
+    // empty waves)
+    const int2* tquart;
+    long long ntquart;
 };
 
 struct StoreArgs {

@@ -340,9 +340,10 @@ __device__ __forceinline__ void eri_team_quartet(const JKArgs& a, double* __rest
 }
 
 // One wave (= one block of 64 threads) per shell quartet and combination
-// slice: work item w is slice w % S of ket (w / S) % 64 of task w / (64 S)
-// (S = team_slices).  Same task formats, counters, molecule offsets and
-// persistent stride (over work items) as the lane evaluator; FP64 only.
+// slice: work item w is slice w % S (S = team_slices) of quartet
+// tquart[w / S] (quartet mode), else of ket (w / S) % 64 of task w / (64 S).
+// Same task formats, counters, molecule offsets and persistent stride (over
+// work items) as the lane evaluator; FP64 only.
 template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(64) void k_eri_team(JKArgs a) {
     using T = TeamShape<LA, LB, LC, LD>;
@@ -350,31 +351,39 @@ __global__ __launch_bounds__(64) void k_eri_team(JKArgs a) {
     const int lane = threadIdx.x & 63;
     const long long w0 = blockIdx.x;
     const int ns = a.team_slices > 1 ? a.team_slices : 1;
-    const long long nwork = static_cast<long long>(a.ntasks) * 64 * ns;
+    const long long nwork = (a.tquart ? a.ntquart : static_cast<long long>(a.ntasks) * 64) * ns;
     const long long end = a.stride > 0 ? nwork : w0 + 1;
     const long long step = a.stride > 0 ? a.stride : 1;
     for (long long w = w0; w < end; w += step) {
         const int slice = static_cast<int>(w % ns);
         const long long item = w / ns;
-        const int task = static_cast<int>(item >> 6), k = static_cast<int>(item & 63);
-        if (task >= a.ntasks) return;
-        const int4 t = a.tasks[task];
-        const int P = __builtin_amdgcn_readfirstlane(t.x);
-        int Q, nq;
-        if (a.qlist) {
-            const long long off = static_cast<long long>(static_cast<unsigned>(__builtin_amdgcn_readfirstlane(t.y))) |
-                                  (static_cast<long long>(__builtin_amdgcn_readfirstlane(t.z) >> 8) << 32);
-            nq = __builtin_amdgcn_readfirstlane(t.z) & 255;
-            if (k >= nq) continue;
-            Q = __builtin_amdgcn_readfirstlane(a.qlist[off + k]);
+        int P, Q;
+        if (a.tquart) {
+            if (item >= a.ntquart) return;
+            const int2 pq = a.tquart[item];
+            P = __builtin_amdgcn_readfirstlane(pq.x);
+            Q = __builtin_amdgcn_readfirstlane(pq.y);
         } else {
-            if ((__builtin_amdgcn_readfirstlane(t.w) & 0xffff) != 0) continue;  // slice 0 takes all primitives
-            const int qb = __builtin_amdgcn_readfirstlane(t.y);
-            int qe = __builtin_amdgcn_readfirstlane(t.z);
-            if (a.cut) qe = min(qe, a.cut[6 * P]);
-            nq = min(qe - qb, 64);
-            if (k >= nq) continue;
-            Q = qb + k;
+            const int task = static_cast<int>(item >> 6), k = static_cast<int>(item & 63);
+            if (task >= a.ntasks) return;
+            const int4 t = a.tasks[task];
+            P = __builtin_amdgcn_readfirstlane(t.x);
+            int nq;
+            if (a.qlist) {
+                const long long off = static_cast<long long>(static_cast<unsigned>(__builtin_amdgcn_readfirstlane(t.y))) |
+                                      (static_cast<long long>(__builtin_amdgcn_readfirstlane(t.z) >> 8) << 32);
+                nq = __builtin_amdgcn_readfirstlane(t.z) & 255;
+                if (k >= nq) continue;
+                Q = __builtin_amdgcn_readfirstlane(a.qlist[off + k]);
+            } else {
+                if ((__builtin_amdgcn_readfirstlane(t.w) & 0xffff) != 0) continue;  // slice 0 takes all primitives
+                const int qb = __builtin_amdgcn_readfirstlane(t.y);
+                int qe = __builtin_amdgcn_readfirstlane(t.z);
+                if (a.cut) qe = min(qe, a.cut[6 * P]);
+                nq = min(qe - qb, 64);
+                if (k >= nq) continue;
+                Q = qb + k;
+            }
         }
         long M = a.M;
         const double* __restrict__ Dm = a.D;
