@@ -144,6 +144,35 @@ __device__ __forceinline__ void rys_roots(const DevTables& tb, double T, double*
         boys_all<1>(tb, T, F);
         r[0] = F[1] * rcp_nr(F[0]);
         w[0] = F[0];
+    } else if constexpr (N == 2) {
+        if (T < 49.0) {
+            // two-point Gauss rule in x = t^2 from the moments F_0..F_3 of its
+            // weight: x^2 + c1 x + c0 is orthogonal to 1 and x, its roots are
+            // the nodes, and the weights match F_0 and F_1 (12 Boys table
+            // values per T instead of 48 interpolation coefficients; the
+            // gathers, not the arithmetic, bound the classes that use it)
+            double F[4];
+            boys_all<3>(tb, T, F);
+            const double id = rcp_nr(F[0] * F[2] - F[1] * F[1]);
+            const double c1 = (F[1] * F[2] - F[0] * F[3]) * id;
+            const double c0 = (F[1] * F[3] - F[2] * F[2]) * id;
+            const double disc = c1 * c1 - 4.0 * c0;
+            const double xb = 0.5 * (disc * rsqrt_nr(disc) - c1);  // c1 < 0: no cancellation
+            const double xs = c0 * rcp_nr(xb);
+            const double iw = rcp_nr(xb - xs);
+            r[0] = xs;
+            r[1] = xb;
+            w[0] = (xb * F[0] - F[1]) * iw;
+            w[1] = (F[1] - xs * F[0]) * iw;
+        } else {
+            const double isT = rsqrt_nr(T);
+            const double iT = isT * isT;
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                r[i] = tb.herm_r2[N][i] * iT;
+                w[i] = tb.herm_w[N][i] * isT;
+            }
+        }
     } else {
         constexpr int ttab[6] = {0, 41, 49, 55, 61, 67};
         if (T < ttab[N]) {
