@@ -344,9 +344,12 @@ __host__ __device__ constexpr bool flat_prims(int LA, int LB, int LC, int LD) {
 // (a task's slice) times all ket primitive pairs of Q, of which the lane takes
 // every js-th starting at j0 -- of the flattened grid (bra index fastest) for
 // flat_prims classes, else of the bra pairs (with every ket pair).
+// uniform: P, ib0 and ib1 are the same in every lane of the wave (lane 0
+// holds the range when some lanes pass an empty one).
 template <typename R, int LA, int LB, int LC, int LD, int AC, int AN, int CC, int CN>
 __device__ __forceinline__ void quartet(const DevTables& tb, const PairData& pd, int P, int Q, int ib0, int ib1,
-                                        int j0, int js, R (&acc)[Shape<LA, LB, LC, LD, AC, AN, CC, CN>::NOUT]) {
+                                        int j0, int js, bool uniform,
+                                        R (&acc)[Shape<LA, LB, LC, LD, AC, AN, CC, CN>::NOUT]) {
     using S = Shape<LA, LB, LC, LD, AC, AN, CC, CN>;
     constexpr CartT<LA> ca = make_cart<LA>();
     constexpr CartT<LB> cb = make_cart<LB>();
@@ -424,6 +427,25 @@ __device__ __forceinline__ void quartet(const DevTables& tb, const PairData& pd,
                     ib -= nb;
                     ++ik;
                 }
+            }
+        }
+    } else if (uniform && js == 1) {
+        // one lane per ket: every lane walks the same bra primitive pairs
+        // (lane 0's range; lanes without a ket of their own run it on a
+        // stand-in ket and are not digested), so their data are wave-uniform
+        // and read through the scalar cache (the pair data are read-only for
+        // the kernel's lifetime: the constant address space is exact)
+        using CDouble = const __attribute__((address_space(4))) double;
+        CDouble* cpP = (CDouble*)(pd.pP);
+        CDouble* cpw = (CDouble*)(pd.pw);
+        const int ub = __builtin_amdgcn_readfirstlane(pb0);
+        const int u0 = __builtin_amdgcn_readfirstlane(ib0), u1 = __builtin_amdgcn_readfirstlane(ib1);
+        for (int ik = 0; ik < pkn; ++ik) {
+            const double4 Qp = pd.pP[pk0 + ik];
+            const double wcd = pd.pw[pk0 + ik];
+            for (int ib = u0; ib < u1; ++ib) {
+                const long i4 = 4L * (ub + ib);
+                prim(make_double4(cpP[i4], cpP[i4 + 1], cpP[i4 + 2], cpP[i4 + 3]), cpw[ub + ib], Qp, wcd);
             }
         }
     } else {
@@ -591,7 +613,7 @@ __device__ __forceinline__ void eri_jk_task(const JKArgs& a, const int wave) {
     }
 
     R acc[S::NOUT];
-    quartet<R, LA, LB, LC, LD, AC, AN, CC, CN>(a.tb, a.pd, P, Q, ib0, active ? ib1 : ib0, sub, ibs, acc);
+    quartet<R, LA, LB, LC, LD, AC, AN, CC, CN>(a.tb, a.pd, P, Q, ib0, active ? ib1 : ib0, sub, ibs, true, acc);
     if (ibs > 1) {
         // the ibs lanes of a ket (lane bits above ks) hold parts of one block:
         // sum them into the first, which alone digests
@@ -767,7 +789,7 @@ __device__ __forceinline__ void eri_store_body(const StoreArgs& a) {
     if (i >= a.n) return;
     const int2 pq = a.quartets[i];
     double acc[S::NOUT];
-    quartet<double, LA, LB, LC, LD, AC, AN, CC, CN>(a.tb, a.pd, pq.x, pq.y, 0, a.pd.pn[pq.x], 0, 1, acc);
+    quartet<double, LA, LB, LC, LD, AC, AN, CC, CN>(a.tb, a.pd, pq.x, pq.y, 0, a.pd.pn[pq.x], 0, 1, false, acc);
     double* out = a.out + a.off[i];
     int o = 0;
 #pragma unroll
