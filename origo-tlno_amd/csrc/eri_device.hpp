@@ -413,23 +413,13 @@ __device__ __forceinline__ void quartet(const DevTables& tb, const PairData& pd,
                         }
         }
     };
-    if constexpr (flat_prims(LA, LB, LC, LD)) {
-        const int nb = ib1 - ib0;
-        if (nb > 0) {
-            const int ntot = pkn * nb;
-            int ik = j0 / nb, ib = j0 - ik * nb;
-            const int dk = js / nb, db = js - dk * nb;
-            for (int j = j0; j < ntot; j += js) {
-                prim(pd.pP[pb0 + ib0 + ib], pd.pw[pb0 + ib0 + ib], pd.pP[pk0 + ik], pd.pw[pk0 + ik]);
-                ik += dk;
-                ib += db;
-                if (ib >= nb) {
-                    ib -= nb;
-                    ++ik;
-                }
-            }
-        }
-    } else if (uniform && js == 1) {
+    // the uniform walk serves the smaller flattened-grid classes too (same
+    // visiting order with one lane per ket); for the larger ones the second
+    // copy of the primitive code costs more than the scalar loads gain
+    // (organic30: (dp|ps) 1.09 -> 1.01 ms, but (ds|pp) 0.85 -> 1.12 ms)
+    constexpr bool kUniformWalk =
+        !flat_prims(LA, LB, LC, LD) || (LA + LB + LC + LD <= 4 && S::NOUT * (LC + LD + 1) <= 108);
+    if (kUniformWalk && uniform && js == 1) {
         // one lane per ket: every lane walks the same bra primitive pairs
         // (lane 0's range; lanes without a ket of their own run it on a
         // stand-in ket and are not digested), so their data are wave-uniform
@@ -446,6 +436,22 @@ __device__ __forceinline__ void quartet(const DevTables& tb, const PairData& pd,
             for (int ib = u0; ib < u1; ++ib) {
                 const long i4 = 4L * (ub + ib);
                 prim(make_double4(cpP[i4], cpP[i4 + 1], cpP[i4 + 2], cpP[i4 + 3]), cpw[ub + ib], Qp, wcd);
+            }
+        }
+    } else if constexpr (flat_prims(LA, LB, LC, LD)) {
+        const int nb = ib1 - ib0;
+        if (nb > 0) {
+            const int ntot = pkn * nb;
+            int ik = j0 / nb, ib = j0 - ik * nb;
+            const int dk = js / nb, db = js - dk * nb;
+            for (int j = j0; j < ntot; j += js) {
+                prim(pd.pP[pb0 + ib0 + ib], pd.pw[pb0 + ib0 + ib], pd.pP[pk0 + ik], pd.pw[pk0 + ik]);
+                ik += dk;
+                ib += db;
+                if (ib >= nb) {
+                    ib -= nb;
+                    ++ik;
+                }
             }
         }
     } else {
