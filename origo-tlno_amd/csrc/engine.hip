@@ -285,6 +285,7 @@ struct TaskSet {
     // of each row, and the task capacity of each class pair; built on first use
     int4* d_rows = nullptr;
     long long* d_rbase = nullptr;
+    long long* d_rtask = nullptr;  // first task slot of each row
     long long nrows = 0;
     long long qtotal = 0;
     long long toff[36] = {}, tcap[36] = {};
@@ -361,8 +362,7 @@ struct tlno_gpu_plan {
     long long qlist_cap = 0;
     int4 *d_t64 = nullptr, *d_t32 = nullptr;
     long long tlist_cap = 0;
-    unsigned* d_tcount = nullptr;            // [2][36] tasks emitted per route and class pair
-    unsigned long long* d_rstats = nullptr;  // [4]
+    unsigned long long* d_rstats = nullptr;  // [kCounterSlots][4]
     long long last_route[4] = {0, 0, 0, 0};  // candidates, density-kept, kept, FP32 (last routed build)
     int team_min_out = TLNO_TEAM_MIN_OUT;    // 0: lane evaluator for every class
     signed char team_sel[6][6];              // autotuned choice per class pair: -1 none, 0 lane, 1 team
@@ -710,7 +710,11 @@ __global__ void k_block_max(const double* __restrict__ D, const int* __restrict_
 //     one integral, a block of them adds up); without it est = Q_ab Q_cd;
 //   * precision: FP32 iff est * max|D| < gate.
 // Survivors are compacted into the row's qlist region (FP64 from the front,
-// FP32 from the back) and emitted as list-mode tasks of up to 64 kets.
+// FP32 from the back) and emitted as list-mode tasks of up to 64 kets into
+// the row's own task slots (ceil(row length / 64) in t64 and in t32, fixed
+// at plan time; unused slots get empty tasks), so no task counter is shared
+// between rows and the host launches the class kernels without reading
+// anything back.
 constexpr int kRouteSeg = 512;
 struct RouteArgs {
     const int4* rows;
@@ -734,9 +738,8 @@ struct RouteArgs {
     int* qlist;
     int4* t64;
     int4* t32;
-    long long toff[36];
-    unsigned* tcount;             // [2][36]
-    unsigned long long* stats;    // candidates, density-kept, kept, FP32
+    const long long* rtask;       // first task slot of each row (in t64 and in t32)
+    unsigned long long* stats;    // [kCounterSlots][4]: candidates, density-kept, kept, FP32
 };
 
 __device__ inline int4 list_task(int P, long long off, int cnt) {
@@ -809,22 +812,14 @@ __global__ __launch_bounds__(256) void k_route(RouteArgs a) {
         n64 += __popcll(m64);
         n32 += __popcll(m32);
     }
-    if (n64 > 0) {
-        const int nt = (n64 + 63) >> 6;
-        unsigned b = 0;
-        if (lane == 0) b = atomicAdd(a.tcount + xy, static_cast<unsigned>(nt));
-        b = __shfl(b, 0, 64);
-        for (int i = lane; i < nt; i += 64)
-            a.t64[a.toff[xy] + b + i] = list_task(P, base + 64ll * i, min(64, n64 - 64 * i));
-    }
-    if (n32 > 0) {
-        const int nt = (n32 + 63) >> 6;
-        unsigned b = 0;
-        if (lane == 0) b = atomicAdd(a.tcount + 36 + xy, static_cast<unsigned>(nt));
-        b = __shfl(b, 0, 64);
-        for (int i = lane; i < nt; i += 64) {
-            const int c = min(64, n32 - 64 * i);
-            a.t32[a.toff[xy] + b + i] = list_task(P, base + len - 64ll * i - c, c);
+    const long long slot = a.rtask[wave];
+    const int ncap = static_cast<int>((len + 63) >> 6);
+    for (int i = lane; i < ncap; i += 64) {
+        const int c64 = max(0, min(64, n64 - 64 * i));
+        a.t64[slot + i] = list_task(P, base + 64ll * i, c64);
+        if (a.gate > 0.0) {
+            const int c32 = max(0, min(64, n32 - 64 * i));
+            a.t32[slot + i] = list_task(P, base + len - 64ll * i - c32, c32);
         }
     }
 #pragma unroll
@@ -833,10 +828,11 @@ __global__ __launch_bounds__(256) void k_route(RouteArgs a) {
         dkept += __shfl_xor(dkept, m, 64);
     }
     if (lane == 0) {
-        atomicAdd(a.stats + 0, cand);
-        atomicAdd(a.stats + 1, dkept);
-        atomicAdd(a.stats + 2, static_cast<unsigned long long>(n64 + n32));
-        atomicAdd(a.stats + 3, static_cast<unsigned long long>(n32));
+        unsigned long long* st = a.stats + 4 * (wave % kCounterSlots);
+        atomicAdd(st + 0, cand);
+        atomicAdd(st + 1, dkept);
+        atomicAdd(st + 2, static_cast<unsigned long long>(n64 + n32));
+        atomicAdd(st + 3, static_cast<unsigned long long>(n32));
     }
 }
 
@@ -1183,8 +1179,7 @@ void build_plan(tlno_gpu_plan* pl, int n, const double* centers, const int* ang,
         dev_upload(pl->d_grows, grows);
         HIP_OK(hipMalloc(&pl->d_blk, std::max<std::size_t>(1, static_cast<std::size_t>(ng) * ng) * sizeof(double)));
         HIP_OK(hipMalloc(&pl->d_bsum, std::max<std::size_t>(1, static_cast<std::size_t>(ng) * ng) * sizeof(double)));
-        HIP_OK(hipMalloc(&pl->d_tcount, 72 * sizeof(unsigned)));
-        HIP_OK(hipMalloc(&pl->d_rstats, 4 * sizeof(unsigned long long)));
+        HIP_OK(hipMalloc(&pl->d_rstats, 4 * kCounterSlots * sizeof(unsigned long long)));
         dev_upload(pl->d_pgA, pl->pgA);
         dev_upload(pl->d_pgB, pl->pgB);
         dev_upload(pl->d_pcls, pcls);
@@ -1194,7 +1189,7 @@ void build_plan(tlno_gpu_plan* pl, int n, const double* centers, const int* ang,
         HIP_OK(hipMalloc(&pl->d_clsmax, 8 * sizeof(unsigned long long)));
         HIP_OK(hipMalloc(&pl->d_pairfac, std::max(1, np) * sizeof(double)));
         HIP_OK(hipMalloc(&pl->d_cut, 6 * static_cast<std::size_t>(std::max(1, np)) * sizeof(int)));
-        HIP_OK(hipMalloc(&pl->d_counter, 4 * sizeof(unsigned long long)));
+        HIP_OK(hipMalloc(&pl->d_counter, 4 * kCounterSlots * sizeof(unsigned long long)));
     }
 }
 
@@ -1496,7 +1491,7 @@ int team_slices_for(long long nq) {
 void ensure_route(tlno_gpu_plan* pl, TaskSet& ts, double thr) {
     if (ts.d_rbase) return;
     std::vector<int4> rows;
-    std::vector<long long> rbase;
+    std::vector<long long> rbase, rtask;
     long long q = 0, cap = 0;
     for (int X = 5; X >= 0; --X)
         for (int Y = X; Y >= 0; --Y) {
@@ -1520,6 +1515,7 @@ void ensure_route(tlno_gpu_plan* pl, TaskSet& ts, double thr) {
                     const int s1 = std::min(end, s0 + kRouteSeg);
                     rows.push_back(make_int4(P, s0, s1, xy));
                     rbase.push_back(q);
+                    rtask.push_back(cap + ts.tcap[xy]);
                     q += s1 - s0;
                     ts.tcap[xy] += (s1 - s0 + 63) / 64;
                 }
@@ -1531,6 +1527,7 @@ void ensure_route(tlno_gpu_plan* pl, TaskSet& ts, double thr) {
     ts.qtotal = q;
     ts.tcap_total = cap;
     dev_upload(ts.d_rows, rows);
+    dev_upload(ts.d_rtask, rtask);
     rbase.push_back(q);  // never empty: marks the list as built
     dev_upload(ts.d_rbase, rbase);
     if (q > pl->qlist_cap) {
@@ -1630,7 +1627,7 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
     a.cut = nullptr;
     a.qlist = nullptr;
     a.counter = pl->d_counter;
-    k_fill<<<1, 64, 0, st>>>(reinterpret_cast<double*>(pl->d_counter), 4, 0.0);
+    k_fill<<<grid_for(4 * kCounterSlots, 256), 256, 0, st>>>(reinterpret_cast<double*>(pl->d_counter), 4 * kCounterSlots, 0.0);
     if (pl->density_thr > 0.0 && (a.do_j || a.do_k)) {
         const int ng = static_cast<int>(pl->groups.size());
         k_fill<<<grid_for(ng, 256), 256, 0, st>>>(reinterpret_cast<double*>(pl->d_gmax), ng, 0.0);
@@ -1651,8 +1648,8 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
     unsigned tcount[72] = {};
     if (route) {
         ensure_route(pl, ts, thr);
-        k_fill<<<1, 64, 0, st>>>(reinterpret_cast<double*>(pl->d_tcount), 36, 0.0);
-        k_fill<<<1, 64, 0, st>>>(reinterpret_cast<double*>(pl->d_rstats), 4, 0.0);
+        k_fill<<<grid_for(4 * kCounterSlots, 256), 256, 0, st>>>(reinterpret_cast<double*>(pl->d_rstats),
+                                                                4 * kCounterSlots, 0.0);
         RouteArgs ra;
         ra.rows = ts.d_rows;
         ra.rbase = ts.d_rbase;
@@ -1675,16 +1672,19 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
         ra.qlist = pl->d_qlist;
         ra.t64 = pl->d_t64;
         ra.t32 = pl->d_t32;
-        for (int i = 0; i < 36; ++i) ra.toff[i] = ts.toff[i];
-        ra.tcount = pl->d_tcount;
+        ra.rtask = ts.d_rtask;
         ra.stats = pl->d_rstats;
         if (ts.nrows > 0) {
             if (pl->learned) k_route<true><<<grid_for(ts.nrows * 64, 256), 256, 0, st>>>(ra);
             else k_route<false><<<grid_for(ts.nrows * 64, 256), 256, 0, st>>>(ra);
         }
         HIP_OK(hipGetLastError());
-        HIP_OK(hipMemcpyAsync(tcount, pl->d_tcount, sizeof(tcount), hipMemcpyDeviceToHost, st));
-        HIP_OK(hipStreamSynchronize(st));
+        // every row's tasks sit in its fixed slots: each class launch covers
+        // the class's slot range (empty tasks leave at once)
+        for (int xy = 0; xy < 36; ++xy) {
+            tcount[xy] = static_cast<unsigned>(ts.tcap[xy]);
+            tcount[36 + xy] = pl->fp32_gate > 0.0 ? static_cast<unsigned>(ts.tcap[xy]) : 0u;
+        }
     }
     if (a.do_j || a.do_k) {
         // Class-pair kernels are independent (their J/K contributions are
@@ -1731,8 +1731,16 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
                 if (n32 > 0) {
                     a.tasks = pl->d_t32 + ts.toff[xy];
                     a.ntasks = static_cast<int>(n32);
-                    a.split = list_split(n32, k.nchunk);
-                    add_jobs(X, Y, true, a, n32 * a.split, 0, c * n32);
+                    if (use_team(k, false, a, class_team_min(pl, X, Y))) {
+                        // the team evaluator (FP64) outruns the FP32 lane
+                        // evaluator on its classes: the FP32 share stays on it
+                        a.split = 1;
+                        a.team_slices = team_slices_for(ts.cls_sq[X][Y]);
+                        add_jobs(X, Y, false, a, n32, class_team_min(pl, X, Y), c * n32);
+                    } else {
+                        a.split = list_split(n32, k.nchunk);
+                        add_jobs(X, Y, true, a, n32 * a.split, 0, c * n32);
+                    }
                 }
             }
         a.qlist = nullptr;
@@ -1772,9 +1780,11 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
                 }
                 const long long nt = ts.cnt[X][Y];
                 if (nt == 0) continue;
-                // FP64 prefix / FP32 suffix of the bound-ordered task list
+                // FP64 prefix / FP32 suffix of the bound-ordered task list (a
+                // class on the team evaluator, FP64 and faster there than the
+                // FP32 lane evaluator, keeps all its quartets)
                 long long n64 = nt;
-                if (pl->fp32_gate > 0.0) {
+                if (pl->fp32_gate > 0.0 && !use_team(g_kernels[X][Y], false, a, class_team_min(pl, X, Y))) {
                     const auto& bm = ts.bmax[X][Y];
                     const double cut = pl->fp32_gate / std::max(maxD, 1e-300);
                     n64 = std::lower_bound(bm.begin(), bm.end(), cut, [](double v, double c) { return v >= c; }) -
@@ -1851,15 +1861,18 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
     HIP_OK(hipStreamSynchronize(st));
     {
         unsigned long long evaluated[4] = {0, 0, 0, 0};
-        HIP_OK(hipMemcpy(evaluated, pl->d_counter, sizeof(evaluated), hipMemcpyDeviceToHost));
+        std::vector<unsigned long long> slots(4 * kCounterSlots);
+        HIP_OK(hipMemcpy(slots.data(), pl->d_counter, slots.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        for (int i = 0; i < 4 * kCounterSlots; ++i) evaluated[i % 4] += slots[i];
         pl->last_evaluated = static_cast<long long>(evaluated[0]);
         pl->last_evaluated_prim = static_cast<long long>(evaluated[1]);
         pl->last_fp32_sq = static_cast<long long>(evaluated[2]);
         pl->last_fp32_pq = static_cast<long long>(evaluated[3]);
         if (route) {
-            unsigned long long rs[4] = {0, 0, 0, 0};
-            HIP_OK(hipMemcpy(rs, pl->d_rstats, sizeof(rs), hipMemcpyDeviceToHost));
-            for (int i = 0; i < 4; ++i) pl->last_route[i] = static_cast<long long>(rs[i]);
+            std::vector<unsigned long long> rs(4 * kCounterSlots);
+            HIP_OK(hipMemcpy(rs.data(), pl->d_rstats, rs.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+            for (int i = 0; i < 4; ++i) pl->last_route[i] = 0;
+            for (int i = 0; i < 4 * kCounterSlots; ++i) pl->last_route[i % 4] += static_cast<long long>(rs[i]);
         } else {
             for (int i = 0; i < 4; ++i) pl->last_route[i] = 0;
         }
@@ -2100,13 +2113,14 @@ tlno_gpu_plan::~tlno_gpu_plan() {
     f(d_row_group); f(d_pgA); f(d_pgB); f(d_pcls); f(d_clsb); f(d_Q); f(d_gmax); f(d_clsmax); f(d_pairfac);
     f(d_cut); f(d_counter);
     f(d_pf0); f(d_pf1); f(d_pcen); f(d_grows); f(d_blk); f(d_bsum); f(d_frag); f(d_mvec); f(d_mnorm);
-    f(d_qlist); f(d_t64); f(d_t32); f(d_tcount); f(d_rstats);
+    f(d_qlist); f(d_t64); f(d_t32); f(d_rstats);
     for (auto& kv : tasks) {
         f(kv.second.d_tasks);
         f(kv.second.d_bk);
         f(kv.second.d_ltasks);
         f(kv.second.d_rows);
         f(kv.second.d_rbase);
+        f(kv.second.d_rtask);
         for (int X = 0; X < 6; ++X)
             for (int Y = 0; Y < 6; ++Y) f(kv.second.d_tq[X][Y]);
     }
@@ -2832,7 +2846,7 @@ void batch_build_tables(tlno_gpu_batch* bt) {
     HIP_OK(hipMalloc(&bt->dDint, tot * sizeof(double)));
     HIP_OK(hipMalloc(&bt->dJacc, tot * sizeof(double)));
     HIP_OK(hipMalloc(&bt->dKacc, tot * sizeof(double)));
-    HIP_OK(hipMalloc(&bt->dcounter, 4 * sizeof(unsigned long long)));
+    HIP_OK(hipMalloc(&bt->dcounter, 4 * kCounterSlots * sizeof(unsigned long long)));
 }
 
 tlno_gpu_batch::Tasks& batch_tasks(tlno_gpu_batch* bt, double thr) {
@@ -2958,7 +2972,7 @@ void batch_enqueue(tlno_gpu_batch* bt, DeviceCtx& ctx, const double* dD, double*
     k_prep_batch<<<g, 256, 0, st>>>(dD, bt->dDint, bt->dperm, bt->dpbase, bt->dmbase, bt->dmM, bt->nmol, bt->total);
     if (dJ) k_fill<<<g, 256, 0, st>>>(bt->dJacc, bt->total, 0.0);
     if (dK) k_fill<<<g, 256, 0, st>>>(bt->dKacc, bt->total, 0.0);
-    k_fill<<<1, 64, 0, st>>>(reinterpret_cast<double*>(bt->dcounter), 4, 0.0);
+    k_fill<<<grid_for(4 * kCounterSlots, 256), 256, 0, st>>>(reinterpret_cast<double*>(bt->dcounter), 4 * kCounterSlots, 0.0);
     const JKArgs a = batch_args(bt, dJ != nullptr, dK != nullptr);
     HIP_OK(hipEventRecord(ctx.ev_fork, st));
     for (int s = 0; s < ctx.nside; ++s) HIP_OK(hipStreamWaitEvent(ctx.side[s], ctx.ev_fork, 0));
@@ -3027,7 +3041,9 @@ void batch_build(tlno_gpu_batch* bt, const double* dD, double* dJ, double* dK, d
     HIP_OK(hipEventElapsedTime(&ms, bt->ev0, bt->ev1));
     bt->last_ms = ms;
     unsigned long long c[4] = {0, 0, 0, 0};
-    HIP_OK(hipMemcpy(c, bt->dcounter, sizeof(c), hipMemcpyDeviceToHost));
+    std::vector<unsigned long long> slots(4 * kCounterSlots);
+    HIP_OK(hipMemcpy(slots.data(), bt->dcounter, slots.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    for (int k = 0; k < 4 * kCounterSlots; ++k) c[k % 4] += slots[k];
     bt->last_sq = static_cast<long long>(c[0]);
     bt->last_pq = static_cast<long long>(c[1]);
     (void)ts;

@@ -478,6 +478,11 @@ __device__ __forceinline__ void quartet(const DevTables& tb, const PairData& pd,
 }
 
 // ------------------------------------------------------------ kernels ----
+// Counters are spread over kCounterSlots slots of four (work item w adds to
+// slot w % kCounterSlots), so the waves of a launch do not all serialize on
+// the same four addresses; readers sum the slots.
+constexpr int kCounterSlots = 64;
+
 struct JKArgs {
     PairData pd;
     DevTables tb;
@@ -494,7 +499,8 @@ struct JKArgs {
     int do_j, do_k;
     const int* cut;     // density screen: ket range of P ends at cut[6 P] (pointer offset by ket class; nullptr: off)
     const int* qlist;   // list mode: kets of a task are qlist[offset .. offset + count)
-    unsigned long long* counter;  // [0] shell, [1] primitive quartets evaluated, [2], [3] the FP32 share
+    unsigned long long* counter;  // [kCounterSlots][4]: [0] shell, [1] primitive quartets evaluated,
+                                  // [2], [3] the FP32 share
     // checksum mode (cw != nullptr, no J/K): chk += sum over the block of
     // cw[pair kind of P][a][b] * cw[pair kind of Q][c][d] * (ab|cd), for
     // P != Q; pair kind = 2 * class + (1 if a diagonal pair)
@@ -609,12 +615,13 @@ __device__ __forceinline__ void eri_jk_task(const JKArgs& a, const int wave) {
         for (int m = 32; m >= 1; m >>= 1) npq += __shfl_xor(npq, m, 64);
         if (lane == 0) {
             constexpr int base = std::is_same<R, float>::value ? 2 : 0;
+            unsigned long long* cnt = a.counter + 4 * (wave % kCounterSlots);
             if (ib0 == 0) {
-                atomicAdd(a.counter, static_cast<unsigned long long>(nq));
-                if (base) atomicAdd(a.counter + base, static_cast<unsigned long long>(nq));
+                atomicAdd(cnt, static_cast<unsigned long long>(nq));
+                if (base) atomicAdd(cnt + base, static_cast<unsigned long long>(nq));
             }
-            atomicAdd(a.counter + 1, static_cast<unsigned long long>(npq));
-            if (base) atomicAdd(a.counter + base + 1, static_cast<unsigned long long>(npq));
+            atomicAdd(cnt + 1, static_cast<unsigned long long>(npq));
+            if (base) atomicAdd(cnt + base + 1, static_cast<unsigned long long>(npq));
         }
     }
 

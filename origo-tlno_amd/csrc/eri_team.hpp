@@ -398,8 +398,9 @@ __global__ __launch_bounds__(64) void k_eri_team(JKArgs a) {
             Km += base;
         }
         if (a.counter && lane == 0 && slice == 0) {
-            atomicAdd(a.counter, 1ull);
-            atomicAdd(a.counter + 1, static_cast<unsigned long long>(a.pd.pn[Q]) * a.pd.pn[P]);
+            unsigned long long* cnt = a.counter + 4 * (item % kCounterSlots);
+            atomicAdd(cnt, 1ull);
+            atomicAdd(cnt + 1, static_cast<unsigned long long>(a.pd.pn[Q]) * a.pd.pn[P]);
         }
         eri_team_quartet<LA, LB, LC, LD>(a, lds, P, Q, Jm, Km, Dm, M, slice, ns);
     }
