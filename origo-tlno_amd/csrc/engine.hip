@@ -367,6 +367,10 @@ struct tlno_gpu_plan {
     int team_min_out = TLNO_TEAM_MIN_OUT;    // 0: lane evaluator for every class
     signed char team_sel[6][6];              // autotuned choice per class pair: -1 none, 0 lane, 1 team
     double cls_cost[6][6] = {};              // measured ms of the chosen evaluator per class pair (0: not measured)
+    // autotuned: the FP32 share of a lane class runs the FP32 kernels only
+    // where they measured faster than the FP64 ones (an FP32 FMA issues at
+    // the FP64 rate without packed math, so the gain is occupancy, if any)
+    bool fp32_slower[6][6] = {};
     // optional per-class kernel timing with HIP events on the plan stream
     int timing = 0;
     hipEvent_t ev[6][6][2] = {};
@@ -627,10 +631,22 @@ __global__ void k_group_rowmax(const double* __restrict__ D, const int* row_grou
 __global__ void k_pair_factor(const unsigned long long* gmax, const int* gA, const int* gB, const int* pcls,
                               int npair, double* pairfac, unsigned long long* clsmax) {
     const int P = blockIdx.x * blockDim.x + threadIdx.x;
-    if (P >= npair) return;
-    const double g = fmax(__longlong_as_double(gmax[gA[P]]), __longlong_as_double(gmax[gB[P]]));
-    pairfac[P] = g;
-    atomicMax(clsmax + pcls[P], static_cast<unsigned long long>(__double_as_longlong(g)));
+    const bool in = P < npair;
+    const double g = in ? fmax(__longlong_as_double(gmax[gA[P]]), __longlong_as_double(gmax[gB[P]])) : 0.0;
+    const int c = in ? pcls[P] : -1;
+    if (in) pairfac[P] = g;
+    // pairs are in class order: a wave whose pairs share one class takes
+    // their maximum first (one atomic per wave, not per pair, on six slots)
+    const int c0 = __shfl(c, 0, 64);
+    if (__all(c == c0 || c < 0)) {
+        double m = g;
+#pragma unroll
+        for (int sh = 32; sh >= 1; sh >>= 1) m = fmax(m, __shfl_xor(m, sh, 64));
+        if ((threadIdx.x & 63) == 0 && c0 >= 0)
+            atomicMax(clsmax + c0, static_cast<unsigned long long>(__double_as_longlong(m)));
+    } else if (in) {
+        atomicMax(clsmax + c, static_cast<unsigned long long>(__double_as_longlong(g)));
+    }
 }
 
 // cut[6 P + Y]: end of the ket prefix of class Y kept for bra pair P, from
@@ -1711,7 +1727,12 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
         };
         std::vector<Job> jobs;
         auto add_jobs = [&](int X, int Y, bool fp32, const JKArgs& args, long long nwaves, int team_min,
-                            double cost) { jobs.push_back({X, Y, team_min, fp32, args, nwaves, cost}); };
+                            double cost) {
+            // the FP32 share of a class whose FP32 kernels measured slower
+            // runs the FP64 kernels on the same tasks
+            if (fp32 && pl->fp32_slower[X][Y]) fp32 = false;
+            jobs.push_back({X, Y, team_min, fp32, args, nwaves, cost});
+        };
         for (int X = 5; X >= 0 && route; --X)
             for (int Y = X; Y >= 0; --Y) {
                 const int xy = 6 * X + Y;
@@ -2048,10 +2069,13 @@ std::string plan_autotune(tlno_gpu_plan* pl, double thr, int repeats) {
                 for (int e = 0; e < 2; ++e)
                     if (!pl->ev[X][Y][e]) HIP_OK(hipEventCreate(&pl->ev[X][Y][e]));
     pl->timing = 1;
-    double ms[2][6][6] = {};
-    for (int mode = 0; mode < 2; ++mode) {
+    double ms[3][6][6] = {};
+    std::memset(pl->fp32_slower, 0, sizeof(pl->fp32_slower));
+    // modes: 0 lane (FP64), 1 team, 2 lane with every quartet on the FP32 kernels
+    for (int mode = 0; mode < 3; ++mode) {
         for (int X = 0; X < 6; ++X)
-            for (int Y = 0; Y < 6; ++Y) pl->team_sel[X][Y] = static_cast<signed char>(mode);
+            for (int Y = 0; Y < 6; ++Y) pl->team_sel[X][Y] = static_cast<signed char>(mode == 1 ? 1 : 0);
+        pl->fp32_gate = mode == 2 ? 1e300 : 0.0;
         jk_device(pl, dD, dJ, dK, thr);  // warm-up
         std::memset(pl->cls_ms, 0, sizeof(pl->cls_ms));
         pl->timed_builds = 0;
@@ -2068,10 +2092,19 @@ std::string plan_autotune(tlno_gpu_plan* pl, double thr, int repeats) {
             const bool team = has_team && ms[1][X][Y] < ms[0][X][Y];
             pl->team_sel[X][Y] = static_cast<signed char>(team ? 1 : 0);
             pl->cls_cost[X][Y] = team ? ms[1][X][Y] : ms[0][X][Y];
-            if (!has_team || ms[0][X][Y] == 0.0) continue;
-            char buf[200];
-            std::snprintf(buf, sizeof(buf), "%s\"(%s|%s)\": {\"lane_ms\": %.4f, \"team_ms\": %.4f, \"choice\": \"%s\"}",
-                          first ? "" : ", ", names[X], names[Y], ms[0][X][Y], ms[1][X][Y], team ? "team" : "lane");
+            pl->fp32_slower[X][Y] = ms[2][X][Y] > 0.0 && ms[2][X][Y] >= ms[0][X][Y];
+            if (ms[0][X][Y] == 0.0) continue;
+            char buf[256];
+            if (has_team)
+                std::snprintf(buf, sizeof(buf),
+                              "%s\"(%s|%s)\": {\"lane_ms\": %.4f, \"team_ms\": %.4f, \"lane_fp32_ms\": %.4f, "
+                              "\"choice\": \"%s\"}",
+                              first ? "" : ", ", names[X], names[Y], ms[0][X][Y], ms[1][X][Y], ms[2][X][Y],
+                              team ? "team" : "lane");
+            else
+                std::snprintf(buf, sizeof(buf),
+                              "%s\"(%s|%s)\": {\"lane_ms\": %.4f, \"lane_fp32_ms\": %.4f, \"choice\": \"lane\"}",
+                              first ? "" : ", ", names[X], names[Y], ms[0][X][Y], ms[2][X][Y]);
             rep += buf;
             first = false;
         }
@@ -2284,6 +2317,7 @@ int tlno_gpu_set_team_threshold(tlno_gpu_plan* pl, int min_outputs) {
         if (min_outputs < 0) throw std::runtime_error("tlno_gpu: team threshold must be >= 0");
         pl->team_min_out = min_outputs;
         std::memset(pl->team_sel, -1, sizeof(pl->team_sel));  // an explicit policy replaces autotuned choices
+        std::memset(pl->fp32_slower, 0, sizeof(pl->fp32_slower));
     });
 }
 
