@@ -138,9 +138,12 @@ int tlno_gpu_set_team_threshold(tlno_gpu_plan* plan, int min_outputs);
 
 /* Autotuner of the plan's kernels (no reference counterpart): per class pair,
  * times the lane and the team evaluator on this plan (repeats serialized FP64
- * builds each) and keeps the faster for following FP64 builds; report
- * receives a JSON summary (may be NULL).  tlno_gpu_set_team_threshold
- * discards the choices. */
+ * builds each) and keeps the faster for following builds, and times the lane
+ * evaluator once more with every quartet on its FP32 kernels: a class whose
+ * FP32 kernels are not faster runs its FP32 share (tlno_gpu_set_mixed_precision)
+ * on the FP64 kernels.  report receives a JSON summary per class pair
+ * (lane_ms, team_ms, lane_fp32_ms, choice; may be NULL).
+ * tlno_gpu_set_team_threshold discards the choices. */
 int tlno_gpu_plan_autotune(tlno_gpu_plan* plan, double schwarz, int repeats, char* report,
                            int report_len);
 

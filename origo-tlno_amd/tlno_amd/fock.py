@@ -157,7 +157,8 @@ class DirectPlan:
 
     def autotune(self, schwarz=None, repeats=2):
         """Pick the faster of the lane and team evaluators per class pair on
-        this plan; returns the timings and choices."""
+        this plan, and whether the FP32 kernels pay for a class's FP32 share;
+        returns the timings and choices."""
         import json
         thr = self.options.integral_threshold if schwarz is None else schwarz
         buf = C.create_string_buffer(8192)

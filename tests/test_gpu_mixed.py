@@ -70,3 +70,27 @@ def test_mixed_precision_mixed_contractions():
     assert (sq1, pq1) == (sq0, pq0)
     assert 0 < st["fp32_shell_quartets"] < sq1
     assert ej <= 1e-7 and ek <= 1e-7
+
+
+def test_mixed_precision_after_autotune():
+    """The autotuner times every lane class with its FP32 kernels as well; a
+    class whose FP32 kernels are not faster runs its FP32 share on the FP64
+    kernels.  The mixed build after autotuning must evaluate every quartet
+    once (same counts as FP64), keep the mixed-route bound, and the report
+    must carry the FP32 timing of every lane class."""
+    b, D = configs.water_cluster(10)
+    plan = DirectPlan(b, ScreeningOptions(integral_threshold=1e-12), cartesian=True)
+    rep = plan.autotune(schwarz=1e-12, repeats=1)
+    assert rep and all("lane_fp32_ms" in v and v["choice"] in ("lane", "team") for v in rep.values())
+    J0, K0 = plan.jk(D)
+    sq0, pq0 = plan.last_build_stats()
+    plan.set_mixed_precision(1e-6)
+    J1, K1 = plan.jk(D)
+    st = plan.last_mixed_stats()
+    sq1, pq1 = plan.last_build_stats()
+    plan.set_mixed_precision(0.0)
+    ej, ek = np.abs(J1 - J0).max(), np.abs(K1 - K0).max()
+    print(f"water10 autotuned: FP32 share {st['fp32_shell_quartets'] / sq1:.3f}; max|dJ| {ej:.2e} max|dK| {ek:.2e}")
+    assert (sq1, pq1) == (sq0, pq0)
+    assert 0 <= st["fp32_shell_quartets"] <= sq1
+    assert ej <= 1e-7 and ek <= 1e-7
