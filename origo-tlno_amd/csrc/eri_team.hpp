@@ -134,9 +134,14 @@ __device__ __forceinline__ void eri_team_quartet(const JKArgs& a, double* __rest
     if (cbeg >= ncomb) return;  // the whole block leaves together
 
     for (int c0 = cbeg; c0 < ncomb; c0 += T::CB) {
-        // ---- phase A: one combination per lane (lanes >= CB idle)
-        const int c = c0 + lane;
-        if (lane < T::CB && c < ncomb) {
+        // ---- phase A: one combination per lane; with 32-combination
+        // batches the upper half of the wave evaluates the z tables of the
+        // lower half's combinations (each half repeats the roots)
+        const int cl = T::CB == 64 ? lane : (lane & 31);
+        const int ax0 = T::CB == 64 ? 0 : (lane < 32 ? 0 : 2);
+        const int ax1 = T::CB == 64 ? 3 : (lane < 32 ? 2 : 3);
+        const int c = c0 + cl;
+        if (c < ncomb) {
             const int r = c % T::NR;
             const int pq = c / T::NR;
             const int ib = pq % pbn, ik = pq / pbn;
@@ -156,10 +161,10 @@ __device__ __forceinline__ void eri_team_quartet(const JKArgs& a, double* __rest
             const double B10 = (0.5 * rcp_nr(p)) * (1.0 - qo * t2);
             const double B01 = (0.5 * rcp_nr(q)) * (1.0 - po * t2);
             const double qt = qo * t2, pt = po * t2;
-            double* g = lds + lane * T::CS;
+            double* g = lds + cl * T::CS;
             // one axis at a time (x, y, z), so a single 2D table is live
 #pragma unroll 1
-            for (int ax = 0; ax < 3; ++ax) {
+            for (int ax = ax0; ax < ax1; ++ax) {
                 const double c00 = ax == 0 ? Pp.x - Ax - qt * PQx : (ax == 1 ? Pp.y - Ay - qt * PQy : Pp.z - Az - qt * PQz);
                 const double d00 = ax == 0 ? Qp.x - Cx + pt * PQx : (ax == 1 ? Qp.y - Cy + pt * PQy : Qp.z - Cz + pt * PQz);
                 double G[T::LAB + 1][T::LCD + 1];
