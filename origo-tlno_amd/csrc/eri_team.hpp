@@ -36,6 +36,10 @@ __host__ __device__ constexpr int tri_index(int LMIN, int x, int y, int z) {
     return (ncart_cum(L - 1) - (LMIN > 0 ? ncart_cum(LMIN - 1) : 0)) + dx * (dx + 1) / 2 + (dx - y);
 }
 
+#ifndef TLNO_TEAM_LDS_KB
+#define TLNO_TEAM_LDS_KB 20
+#endif
+
 template <int LA, int LB, int LC, int LD>
 struct TeamShape {
     static constexpr int LAB = LA + LB, LCD = LC + LD;
@@ -53,8 +57,10 @@ struct TeamShape {
     static constexpr int GS = (LAB + 1) * (LCD + 1);       // one axis table
     static constexpr int CS = 3 * GS + 1;                  // one combination (padded)
     // combinations per batch: 64 (one per lane) while their tables fit in
-    // 20 KB of LDS, else 32, so that two waves per SIMD keep fitting in LDS
-    static constexpr int CB = 64 * CS * 8 <= 20 * 1024 ? 64 : 32;
+    // TLNO_TEAM_LDS_KB of LDS, else 32 (the upper half-wave then evaluates
+    // the z tables of the lower half's combinations), so that two waves per
+    // SIMD keep fitting in LDS
+    static constexpr int CB = 64 * CS * 8 <= TLNO_TEAM_LDS_KB * 1024 ? 64 : 32;
     static constexpr int LDS_A = CB * CS;
     // phase C: [e|f] then the block share [0, R0); the ket-transformed
     // [e|cd] sits after them; phase D: the block, then six density blocks
