@@ -109,9 +109,10 @@ __device__ __forceinline__ double rsqrt_nr(double x) {
 // F_0..F_MMAX at T
 template <int MMAX>
 __device__ __forceinline__ void boys_all(const DevTables& tb, double T, double* F) {
+    static_assert(MMAX + BOYS_TAYLOR - 1 <= BOYS_MMAX, "Boys order beyond the tabulated orders");
     if (T < BOYS_TCUT) {
-        const int k = static_cast<int>(T * 8.0 + 0.5);
-        const double dd = k * 0.125 - T;  // F_m(T) = sum_j F_{m+j}(T_k) dd^j / j!
+        const int k = static_cast<int>(T * BOYS_STEPS + 0.5);
+        const double dd = k * (1.0 / BOYS_STEPS) - T;  // F_m(T) = sum_j F_{m+j}(T_k) dd^j / j!
         const double* row = tb.boys + k * (BOYS_MMAX + 1);
 #pragma unroll
         for (int m = 0; m <= MMAX; ++m) {

@@ -186,7 +186,7 @@ HostTables* build_tables() {
     t->boys.assign(static_cast<std::size_t>(BOYS_NGRID) * (BOYS_MMAX + 1), 0.0);
     for (int k = 0; k < BOYS_NGRID; ++k)
         for (int m = 0; m <= BOYS_MMAX; ++m)
-            t->boys[static_cast<std::size_t>(k) * (BOYS_MMAX + 1) + m] = double(boys_ld(m, ld(k) / 8.0L));
+            t->boys[static_cast<std::size_t>(k) * (BOYS_MMAX + 1) + m] = double(boys_ld(m, ld(k) / BOYS_STEPS));
     // Rys: Chebyshev interpolation at D+1 nodes per unit interval, converted
     // to monomials in u = T - (k + 1/2), all in long double.
     int off = 0;

@@ -2,11 +2,12 @@
 // device code.  Everything is computed at library initialisation from the
 // defining integrals; nothing is tabulated in the source.
 //
-//   Boys:  F_m(T) = int_0^1 t^{2m} exp(-T t^2) dt, sampled on T_k = k/8,
+//   Boys:  F_m(T) = int_0^1 t^{2m} exp(-T t^2) dt, sampled on T_k = k/32,
 //          k = 0..BOYS_NGRID-1, m = 0..BOYS_MMAX, by the positive series
 //          e^{-T} sum_k (2T)^k / ((2m+1)(2m+3)...(2m+2k+1)) in long double.
-//          Device evaluation: 9-term Taylor expansion about the nearest grid
-//          point using dF_m/dT = -F_{m+1} (remainder < 5e-17 relative).
+//          Device evaluation: 7-term Taylor expansion about the nearest grid
+//          point using dF_m/dT = -F_{m+1} (|T - T_k| <= 1/64, remainder
+//          <= 64^-7 / 7! < 5e-17 relative); orders up to BOYS_MMAX - 6.
 //
 //   Rys:   roots t_i^2 and weights w_i of the rule exact for
 //          int_0^1 f(t^2) exp(-T t^2) dt, deg f <= 2N-1 (the reference's
@@ -25,9 +26,10 @@
 namespace tlno_amd {
 
 constexpr int BOYS_MMAX = 24;          // orders stored per grid point
-constexpr int BOYS_NGRID = 41 * 8 + 1; // T in [0, 41], step 1/8
+constexpr int BOYS_STEPS = 32;         // grid points per unit of T
+constexpr int BOYS_NGRID = 41 * BOYS_STEPS + 1; // T in [0, 41], step 1/32
 constexpr double BOYS_TCUT = 40.0;     // asymptotic + upward recursion beyond
-constexpr int BOYS_TAYLOR = 9;
+constexpr int BOYS_TAYLOR = 7;
 
 constexpr int RYS_NMAX = 5;   // shells up to d: at most 5 roots
 constexpr int RYS_DEG = 11;             // polynomial degree per interval
