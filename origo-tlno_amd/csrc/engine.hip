@@ -601,11 +601,13 @@ void launch_jk(const ClassKernels& k, bool fp32, JKArgs& args, long long nwaves,
         args.stride = 0;
         const long long ns = std::max(1, args.team_slices);
         const long long items = args.tquart ? args.ntquart : static_cast<long long>(args.ntasks) * 64;
+        if (items <= 0) return;
         if (items * ns > 0x7fffffffLL) throw std::runtime_error("tlno_gpu: team launch too large for one grid");
         HIP_OK(hipLaunchKernel(reinterpret_cast<const void*>(k.team), dim3(static_cast<unsigned>(items * ns)),
                                dim3(64), params, 0, st));
         return;
     }
+    if (nwaves <= 0) return;
     const unsigned grid = grid_for(nwaves * 64, 256);
     for (int g = 0; g < k.ngroup; ++g) {
         const JKKernel kern = fp32 ? k.grp[g].jk32 : k.grp[g].jk;
