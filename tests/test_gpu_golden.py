@@ -101,6 +101,22 @@ def test_organic30_benchmark_molecule_vs_reference():
     assert np.linalg.norm(K) == pytest.approx(float(g["K_fro"]), rel=1e-12)
 
 
+def test_organic30_autotuned_vs_reference():
+    """The benchmark molecule with the autotuned plan (team evaluator on most
+    d class pairs, quartet lists, cost-balanced lane slices) against the
+    reference's G at Schwarz 1e-12."""
+    g = load("organic30.npz")
+    b, _ = configs.organic30()
+    plan = DirectPlan(b, ScreeningOptions(integral_threshold=1e-12))
+    rep = plan.autotune(schwarz=1e-12, repeats=1)
+    assert any(v["choice"] == "team" for v in rep.values())
+    D = configs.spherical_density(plan.M, 5, 60)
+    J, K = plan.jk(D)
+    assert np.abs(J - 0.5 * K - g["G_12"]).max() < 1e-10
+    assert np.linalg.norm(J) == pytest.approx(float(g["J_fro"]), rel=1e-12)
+    assert np.linalg.norm(K) == pytest.approx(float(g["K_fro"]), rel=1e-12)
+
+
 @pytest.mark.parametrize("tau,tag", [(1e-9, "9"), (1e-7, "7"), (1e-5, "5")])
 def test_density_screen_vs_reference(tau, tag):
     """The reference's density-screened J (J-only test), K (K-only test) and
