@@ -165,6 +165,86 @@ def mixed_run(steps=3, gate=1e-6, tau=SCREEN_TAU):
     return out
 
 
+def water100_run(gate=1e-6, tau=SCREEN_TAU):
+    """water100 (100 waters at water40's number density, 2500 Cartesian
+    functions), Schwarz 1e-12, densities resident in HBM: the Schwarz-screened
+    FP64 build judges the network-routed build (learned+mixed, shipped
+    weights) and the same build without the network (density+mixed); one
+    build each after a warm-up build of the FP64 plan's kernels."""
+    from tlno_amd import learned
+    lib = _lib.load()
+    b, D = configs.water100()
+    thr = 1e-12
+    t0 = time.perf_counter()
+    plan = DirectPlan(b, ScreeningOptions(integral_threshold=thr), cartesian=True)
+    plan_s = time.perf_counter() - t0
+    M = plan.M
+    dD = hipmem.DeviceArray.from_host(D)
+    dJ = hipmem.DeviceArray((M, M))
+    dK = hipmem.DeviceArray((M, M))
+    params, margin = learned.load_weights()
+    runs = {}
+    for name, g, t, use_net in (("fp64", 0.0, 0.0, False), ("density+mixed", gate, tau, False),
+                                ("learned+mixed", gate, tau, True)):
+        _lib.check(lib.tlno_gpu_set_mixed_precision(plan.handle, g))
+        _lib.check(lib.tlno_gpu_set_density_threshold(plan.handle, t))
+        plan.set_learned_screen(params if use_net else None, margin)
+        hipmem.synchronize()
+        t0 = time.perf_counter()
+        _lib.check(lib.tlno_gpu_build_jk_device(plan.handle, dD.ptr, dJ.ptr, dK.ptr, thr))
+        hipmem.synchronize()
+        sec = time.perf_counter() - t0
+        sq, pq = plan.last_build_stats()
+        runs[name] = {"ms": sec * 1e3, "J": dJ.to_host(), "K": dK.to_host(), "shell_quartets": sq,
+                      "prim_quartets": pq, "fp32": plan.last_mixed_stats(), "route": plan.last_route_stats()}
+        print(f"bench: water100 {name} {sec * 1e3:.0f} ms", file=sys.stderr, flush=True)
+    plan.set_learned_screen(None)
+    base = runs["fp64"]
+    out = {"workload": "water100 (100 waters, 2500 Cartesian functions, Schwarz 1e-12)", "fp32_gate": gate,
+           "density_threshold": tau, "learned_margin_decades": margin, "plan_seconds": plan_s,
+           "fp64_ms": base["ms"], "fp64_shell_quartets": base["shell_quartets"],
+           "fp64_prim_quartets": base["prim_quartets"]}
+    for name in ("density+mixed", "learned+mixed"):
+        r = runs[name]
+        out[name] = {"ms": r["ms"], "speedup_vs_fp64": base["ms"] / r["ms"],
+                     "shell_quartets": r["shell_quartets"], "prim_quartets": r["prim_quartets"],
+                     "fp32_share_shell_quartets": r["fp32"]["fp32_shell_quartets"] / max(1, r["shell_quartets"]),
+                     "max_abs_err_J": float(np.abs(r["J"] - base["J"]).max()),
+                     "max_abs_err_K": float(np.abs(r["K"] - base["K"]).max()),
+                     "rms_err_J": float(np.sqrt(np.mean((r["J"] - base["J"]) ** 2))),
+                     "rms_err_K": float(np.sqrt(np.mean((r["K"] - base["K"]) ** 2))), "route": r["route"]}
+    out["learned_gain_over_density_only"] = runs["density+mixed"]["ms"] / runs["learned+mixed"]["ms"]
+    out["learned_quartet_reduction_over_density_only"] = (runs["learned+mixed"]["shell_quartets"] /
+                                                          max(1, runs["density+mixed"]["shell_quartets"]))
+    return out
+
+
+def routing_profile_run(builds=3, gate=1e-6, tau=SCREEN_TAU):
+    """The learned+mixed water40 build alone (shipped weights, FP32 gate 1e-6,
+    density threshold 2e-9, Schwarz 1e-12, no autotuning), `builds` times:
+    the workload of the routing-kernel profiles."""
+    from tlno_amd import learned
+    lib = _lib.load()
+    b, D = configs.water40()
+    plan = DirectPlan(b, ScreeningOptions(integral_threshold=1e-12), cartesian=True)
+    dD = hipmem.DeviceArray.from_host(D)
+    dJ = hipmem.DeviceArray((plan.M, plan.M))
+    dK = hipmem.DeviceArray((plan.M, plan.M))
+    params, margin = learned.load_weights()
+    plan.set_learned_screen(params, margin)
+    _lib.check(lib.tlno_gpu_set_mixed_precision(plan.handle, gate))
+    _lib.check(lib.tlno_gpu_set_density_threshold(plan.handle, tau))
+    ms = []
+    for _ in range(builds):
+        hipmem.synchronize()
+        t0 = time.perf_counter()
+        _lib.check(lib.tlno_gpu_build_jk_device(plan.handle, dD.ptr, dJ.ptr, dK.ptr, 1e-12))
+        hipmem.synchronize()
+        ms.append((time.perf_counter() - t0) * 1e3)
+    return {"workload": "water40 learned+mixed (routing profile)", "builds": builds, "ms": ms,
+            "shell_quartets": plan.last_build_stats()[0], "route": plan.last_route_stats()}
+
+
 def batched_run(steps=5):
     """batch256: 256 molecules of 3-40 atoms (25-460 Cartesian functions,
     every class up to (dd|dd)), FP64, Schwarz 1e-12: one batched build = J and
@@ -216,17 +296,7 @@ def batched_run(steps=5):
             "graph_vs_default_max_abs_diff_J": err, "autotuner": tune}
 
 
-def cpu_baseline(b, seconds_cap=60.0):
-    try:
-        from oracle import oracle as O
-        ref = O.Reference()
-    except Exception as e:  # reference not built on this host
-        return {"value": None, "unit": "Fock builds/s", "cores": 0, "kind": "reference",
-                "sample": f"unavailable: {e}"}
-    M = ref.plan_M(b)
-    D = configs.spherical_density(M, 5, 60)
-    _, sec = ref.direct(b, D, "G", schwarz=1e-12, threads=1, repeats=1)
-    ncpu = os.cpu_count()
+def host_cpu():
     model = ""
     try:
         with open("/proc/cpuinfo") as f:
@@ -236,9 +306,41 @@ def cpu_baseline(b, seconds_cap=60.0):
                     break
     except OSError:
         pass
-    return {"value": 1.0 / sec, "unit": "Fock builds/s", "cores": 1, "kind": "reference",
-            "sample": (f"one full G = J - K/2 build of the same molecule and basis in the reference's spherical "
-                       f"basis (M={M}), Schwarz 1e-12, 1 thread ({sec:.2f} s); host {model}, {ncpu} cpus")}
+    return model
+
+
+def cpu_baseline(b, rounds=3):
+    """The reference engine (oracle/_ref, built from its own sources) on this
+    host: after one warm-up build, `rounds` interleaved rounds of a full
+    G = J - K/2 build on one thread (its K and G endpoints refuse more,
+    rys/native_quartets.ipp:666-670) and a J-only build on all the host's
+    cores (its OpenMP J, rys/native_parallel_j.ipp:12-114); medians reported
+    (benchmarks/two_electron_engine/scripts/benchmark_statistics.py:99-131)."""
+    try:
+        from oracle import oracle as O
+        ref = O.Reference()
+    except Exception as e:  # reference not built on this host
+        return {"value": None, "unit": "Fock builds/s", "cores": 0, "kind": "reference",
+                "sample": f"unavailable: {e}"}
+    M = ref.plan_M(b)
+    D = configs.spherical_density(M, 5, 60)
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    cores = max(1, min(cores, os.cpu_count() or 1))
+    ref.direct(b, D, "J", schwarz=1e-12, threads=cores)  # warm-up (page-in, thread pool)
+    g_s, j_s = [], []
+    for _ in range(rounds):
+        g_s.append(ref.direct(b, D, "G", schwarz=1e-12, threads=1)[1])
+        j_s.append(ref.direct(b, D, "J", schwarz=1e-12, threads=cores)[1])
+    g = float(np.median(g_s))
+    j = float(np.median(j_s))
+    ncpu = os.cpu_count()
+    return {"value": 1.0 / g, "unit": "Fock builds/s", "cores": 1, "kind": "reference",
+            "sample": (f"full G = J - K/2 builds of the same molecule and basis in the reference's spherical basis "
+                       f"(M={M}; its public 2e API has no Cartesian d), Schwarz 1e-12, 1 thread: median of "
+                       f"{rounds} interleaved builds after a warm-up, {g:.2f} s (all: "
+                       f"{', '.join(f'{x:.2f}' for x in g_s)}); host {host_cpu()}, {ncpu} cpus"),
+            "j_only_all_cores": {"value": 1.0 / j, "unit": "J builds/s", "cores": cores,
+                                 "median_s": j, "all_s": j_s}}
 
 
 def main():
@@ -250,6 +352,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-water100", action="store_true")
+    ap.add_argument("--no-queue-probe", action="store_true",
+                    help="skip the organic30 rerun with HIP's default 4 hardware queues")
+    ap.add_argument("--routing-profile", type=int, default=0, metavar="BUILDS",
+                    help="only run BUILDS learned+mixed water40 builds (the routing-kernel profile workload)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first copy a buffer of this many MiB device to device (a copy kernel of known traffic "
                          "for calibrating hardware byte counters)")
@@ -266,6 +373,9 @@ def main():
         hipmem.synchronize()
         src.free()
         dst.free()
+    if args.routing_profile > 0:
+        print(json.dumps(routing_profile_run(args.routing_profile)), flush=True)
+        return
     b, D = configs.organic30() if args.workload == "organic30" else configs.water40()
     print(f"bench: {args.workload} J/K builds ({args.warmup} warm-up, {args.steps} timed)", file=sys.stderr,
           flush=True)
@@ -299,7 +409,7 @@ def main():
                            "primitive_quartets_per_s": pq_tot / (wall / args.steps),
                            "cartesian_integrals_per_s": ints_tot / (wall / args.steps),
                            "shell_quartets_per_build": sq_tot, "primitive_quartets_per_build": pq_tot},
-        "roofline": {"bound": "mfma", "pipe": "fp64 VALU (fp64 vector peak = fp64 matrix peak)",
+        "roofline": {"bound": "fp64_valu", "pipe": "fp64 VALU (MI355X fp64 vector peak = fp64 matrix peak)",
                      "kernel": dom_kernel, "achieved": dom["tflops"],
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": dom["tflops"] / FP64_PEAK_TFLOPS,
                      "traffic": measured_traffic(dom_kernel), "algorithmic_gflop_per_launch": dom["gflop"],
@@ -311,9 +421,15 @@ def main():
     if not args.no_mixed:
         print("bench: water40 mixed-precision and screened builds", file=sys.stderr, flush=True)
         result["mixed_precision"] = mixed_run()
+    if not args.no_water100:
+        print("bench: water100 learned-routed builds", file=sys.stderr, flush=True)
+        result["learned_water100"] = water100_run()
     if not args.no_batch:
         print("bench: 256-molecule batch builds", file=sys.stderr, flush=True)
         result["batched"] = batched_run()
+    if not args.no_queue_probe:
+        print("bench: organic30 with 4 hardware queues (HIP's default)", file=sys.stderr, flush=True)
+        result["default_hw_queues"] = queue_probe(args.steps, args.warmup)
     if not args.no_cpu:
         print("bench: reference CPU build", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(b)
@@ -325,6 +441,20 @@ def main():
 HW_QUEUES = 8
 
 
+def queue_probe(steps, warmup, queues=4):
+    """The headline build in a child process with `queues` hardware queues
+    (HIP's default is 4; the headline runs with 8): builds/s and ms."""
+    import subprocess
+    env = dict(os.environ, GPU_MAX_HW_QUEUES=str(queues), TLNO_BENCH_QUEUE_PROBE="1")
+    cmd = [sys.executable, os.path.abspath(__file__), "--no-cpu", "--no-mixed", "--no-batch", "--no-water100",
+           "--no-queue-probe", "--steps", str(steps), "--warmup", str(warmup)]
+    r = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        return {"hw_queues": queues, "error": r.stderr[-400:]}
+    line = json.loads(r.stdout.strip().splitlines()[-1])
+    return {"hw_queues": queues, "value": line["value"], "ms_per_step": line["ms_per_step"]}
+
+
 def relaunch_with_hw_queues():
     """The plan runs the class kernels of a build on up to 8 concurrent
     streams, which HIP spreads over GPU_MAX_HW_QUEUES hardware queues (4 by
@@ -332,7 +462,7 @@ def relaunch_with_hw_queues():
     queues: if the environment says otherwise it reruns itself as a child
     process with GPU_MAX_HW_QUEUES=8 (nothing has touched the GPU yet) and
     exits with the child's status."""
-    if os.environ.get("GPU_MAX_HW_QUEUES") == str(HW_QUEUES):
+    if os.environ.get("GPU_MAX_HW_QUEUES") == str(HW_QUEUES) or os.environ.get("TLNO_BENCH_QUEUE_PROBE") == "1":
         return
     import subprocess
     env = dict(os.environ, GPU_MAX_HW_QUEUES=str(HW_QUEUES))

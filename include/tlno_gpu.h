@@ -183,12 +183,16 @@ int tlno_gpu_class_stats(const tlno_gpu_plan* plan, long long* shell_quartets,
 int tlno_gpu_eri_cart_blocks(tlno_gpu_plan* plan, int n, const int* groups4,
                              const long long* offsets, double* out);
 
-/* Boys F_0..F_mmax at n arguments: out[i*(mmax+1)+m] (mmax <= 16).
- * Counterpart of boys_single / boys_r000_all (core/math/general_math.hpp:6-8)
- * evaluated by the device evaluator used in the ERI kernels. */
+/* Boys F_0..F_mmax at n arguments: out[i*(mmax+1)+m], 0 <= mmax <=
+ * TLNO_GPU_BOYS_MMAX.  Counterpart of boys_single / boys_r000_all
+ * (core/math/general_math.hpp:6-8): orders up to 16 use the device evaluator
+ * of the ERI kernels, higher orders a series plus downward recursion.  Every
+ * T must be finite and >= 0 (checked; an error otherwise). */
+#define TLNO_GPU_BOYS_MMAX 128
 int tlno_gpu_boys(int n, const double* T, int mmax, double* out);
 
-/* Rys roots t^2 and weights for N = 1..5 at n arguments: roots[i*N+k].
+/* Rys roots t^2 and weights for N = 1..5 at n arguments: roots[i*N+k]; every
+ * T finite and >= 0 (checked).
  * Counterpart of rys_roots_weights (rys_engine.ipp:3820-3837). */
 int tlno_gpu_rys(int N, int n, const double* T, double* roots, double* weights);
 

@@ -102,15 +102,23 @@ tlno::integrals::twoe::EriPackedPlanProfile profile_of(const GpuPlan& g) {
 // ------------------------------------------------------------------ Boys --
 namespace tlno::core::math {
 
+// Orders 0..TLNO_GPU_BOYS_MMAX (the reference accepts any order; 128 is far
+// beyond what an integral of g shells needs).
+void require_order(const char* who, int n) {
+    if (n < 0 || n > TLNO_GPU_BOYS_MMAX)
+        throw std::runtime_error(std::string(who) + ": order must be in [0, " + std::to_string(TLNO_GPU_BOYS_MMAX) +
+                                 "].");
+}
+
 double boys_single(int order, double x) {
-    if (order < 0 || order > 16) throw std::runtime_error("boys_single: order must be in [0, 16].");
+    require_order("boys_single", order);
     std::vector<double> F(static_cast<std::size_t>(order) + 1);
     check(tlno_gpu_boys(1, &x, order, F.data()));
     return F[order];
 }
 
 void boys_r000_all(int max_n, double x, double p, double* out) {
-    if (max_n < 0 || max_n > 16) throw std::runtime_error("boys_r000_all: max_n must be in [0, 16].");
+    require_order("boys_r000_all", max_n);
     std::vector<double> F(static_cast<std::size_t>(max_n) + 1);
     check(tlno_gpu_boys(1, &x, max_n, F.data()));
     double s = 1.0;
@@ -122,16 +130,25 @@ void boys_r000_all_batch(int max_n, const double* xs, double p, double* out) {
     boys_r000_all_batch_masked<W>(max_n, xs, p, out, (1u << W) - 1u);
 }
 
+// Inactive lanes may hold anything (NaN) and xs may be null when no lane is
+// active (benchmarks/one_electron_engine/src/boys_mask_contract.cpp:17-47):
+// only active lanes are read and sent to the device, inactive outputs are 0.
 template <int W>
 void boys_r000_all_batch_masked(int max_n, const double* xs, double p, double* out, unsigned active_mask) {
     static_assert(W == 1 || W == 2 || W == 4 || W == 8, "W must be 1, 2, 4 or 8");
-    if (max_n < 0 || max_n > 16) throw std::runtime_error("boys_r000_all: max_n must be in [0, 16].");
+    require_order("boys_r000_all", max_n);
+    active_mask &= (1u << W) - 1u;
+    double live[W];
+    int nlive = 0;
+    for (int lane = 0; lane < W; ++lane)
+        if ((active_mask >> lane) & 1u) live[nlive++] = xs[lane];
     std::vector<double> F(static_cast<std::size_t>(W) * (max_n + 1));
-    check(tlno_gpu_boys(W, xs, max_n, F.data()));
-    for (int lane = 0; lane < W; ++lane) {
+    if (nlive > 0) check(tlno_gpu_boys(nlive, live, max_n, F.data()));
+    for (int lane = 0, k = 0; lane < W; ++lane) {
         const bool on = (active_mask >> lane) & 1u;
         double s = 1.0;
-        for (int n = 0; n <= max_n; ++n, s *= -2.0 * p) out[n * W + lane] = on ? s * F[lane * (max_n + 1) + n] : 0.0;
+        for (int n = 0; n <= max_n; ++n, s *= -2.0 * p) out[n * W + lane] = on ? s * F[k * (max_n + 1) + n] : 0.0;
+        if (on) ++k;
     }
 }
 

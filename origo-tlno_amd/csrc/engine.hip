@@ -82,6 +82,21 @@ struct DeviceCtx {
     int nside = 0;  // 0: not created yet
     hipStream_t side[kSideStreams] = {};
     hipEvent_t ev_fork = nullptr, ev_join[kSideStreams] = {};
+    // staging buffer of the small utility entry points (Boys, Rys), grown on
+    // demand and kept for the process; used under util_mu
+    std::mutex util_mu;
+    double* util = nullptr;
+    std::size_t util_n = 0;
+    double* util_buffer(std::size_t n) {
+        if (n > util_n) {
+            if (util) HIP_OK(hipFree(util));
+            util = nullptr;
+            util_n = 0;
+            HIP_OK(hipMalloc(&util, n * sizeof(double)));
+            util_n = n;
+        }
+        return util;
+    }
 };
 
 // creates the device's side streams and events (caller holds side_mu)
@@ -570,6 +585,41 @@ __global__ void k_boys(DevTables tb, const double* T, double* out, int n) {
     for (int m = 0; m <= MMAX; ++m) out[(long)i * (MMAX + 1) + m] = F[m];
 }
 
+// F_0..F_mmax at any order (mmax above the table's orders): F_mmax from the
+// positive series e^{-T} sum_k (2T)^k / ((2m+1)(2m+3)...(2m+2k+1)) and the
+// downward recursion F_{m-1} = (2T F_m + e^{-T}) / (2m - 1), both stable;
+// where T is large against mmax, the upward recursion
+// F_{m+1} = ((2m+1) F_m - e^{-T}) / (2T) from F_0 = sqrt(pi/T)/2 (erf(sqrt T)
+// = 1 in binary64 for T >= 40), stable while 2T > 2m+1.
+__global__ void k_boys_any(const double* T, double* out, int n, int mmax) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double t = T[i];
+    double* o = out + static_cast<long>(i) * (mmax + 1);
+    const double ex = exp(-t);
+    if (t < 40.0 || t < 1.5 * mmax + 10.0) {
+        double term = 1.0 / (2 * mmax + 1), sum = term;
+        for (int k = 1; k < 4000; ++k) {
+            term *= 2.0 * t / (2 * mmax + 2 * k + 1);
+            sum += term;
+            if (term < 1e-17 * sum) break;
+        }
+        double f = ex * sum;
+        o[mmax] = f;
+        for (int m = mmax; m >= 1; --m) {
+            f = (2.0 * t * f + ex) / (2 * m - 1);
+            o[m - 1] = f;
+        }
+    } else {
+        double f = 0.88622692545275801365 / sqrt(t);
+        o[0] = f;
+        for (int m = 0; m < mmax; ++m) {
+            f = ((2 * m + 1) * f - ex) / (2.0 * t);
+            o[m + 1] = f;
+        }
+    }
+}
+
 template <int N>
 __global__ void k_rys(DevTables tb, const double* T, double* r, double* w, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -592,28 +642,54 @@ bool use_team(const ClassKernels& k, bool fp32, const JKArgs& args, int min_out)
 // there is one (1 = team, expressed as threshold 1; 0 = lane, threshold 0)
 inline int class_team_min(const tlno_gpu_plan* pl, int X, int Y);
 
+// Largest grid (in waves) of one class launch.  HIP limits a launch to
+// 2^32 - 1 threads; larger work lists run persistent, each wave of a capped
+// grid striding over the work items (the kernels' persistent mode).  The
+// environment variable TLNO_MAX_GRID_WAVES lowers the cap (read at every
+// launch; used by the tests to force the persistent path).
+long long max_grid_waves() {
+    long long cap = 1LL << 24;
+    if (const char* v = std::getenv("TLNO_MAX_GRID_WAVES")) {
+        const long long n = std::atoll(v);
+        if (n > 0) cap = std::min(cap, n);
+    }
+    return cap;
+}
+
 // nwaves waves per chunk, every chunk group of the class (FP32 or FP64
 // kernels), or the team kernel with one 64-thread block per work item
 void launch_jk(const ClassKernels& k, bool fp32, JKArgs& args, long long nwaves, hipStream_t st,
                int team_min_out = 0) {
     void* params[] = {&args};
+    const long long cap = max_grid_waves();
     if (use_team(k, fp32, args, team_min_out)) {  // one block per shell quartet and slice (64 per task)
-        args.stride = 0;
         const long long ns = std::max(1, args.team_slices);
         const long long items = args.tquart ? args.ntquart : static_cast<long long>(args.ntasks) * 64;
         if (items <= 0) return;
-        if (items * ns > 0x7fffffffLL) throw std::runtime_error("tlno_gpu: team launch too large for one grid");
-        HIP_OK(hipLaunchKernel(reinterpret_cast<const void*>(k.team), dim3(static_cast<unsigned>(items * ns)),
-                               dim3(64), params, 0, st));
+        const long long work = items * ns;
+        args.stride = work > cap ? static_cast<int>(cap) : 0;
+        HIP_OK(hipLaunchKernel(reinterpret_cast<const void*>(k.team),
+                               dim3(static_cast<unsigned>(std::min(work, cap))), dim3(64), params, 0, st));
+        args.stride = 0;
         return;
     }
     if (nwaves <= 0) return;
-    const unsigned grid = grid_for(nwaves * 64, 256);
+    if (nwaves > 0x7fffffffLL) throw std::runtime_error("tlno_gpu: class work list too large (more than 2^31 waves)");
+    const int saved_stride = args.stride, saved_nwork = args.nwork;
+    long long gw = nwaves;
+    if (args.stride == 0 && nwaves > cap) {  // persistent: capped grid striding over the work items
+        args.stride = static_cast<int>(cap);
+        args.nwork = static_cast<int>(nwaves);
+        gw = cap;
+    }
+    const unsigned grid = grid_for(gw * 64, 256);
     for (int g = 0; g < k.ngroup; ++g) {
         const JKKernel kern = fp32 ? k.grp[g].jk32 : k.grp[g].jk;
         HIP_OK(hipLaunchKernel(reinterpret_cast<const void*>(kern), dim3(grid, k.grp[g].n), dim3(256), params, 0,
                                st));
     }
+    args.stride = saved_stride;
+    args.nwork = saved_nwork;
 }
 
 // ---- density screen (per build) ----
@@ -2051,17 +2127,49 @@ double eri_checksum(tlno_gpu_plan* pl, double thr) {
 // Per class pair, time the lane and the team evaluator on this plan (FP64,
 // J and K, a constant density, serialized launches bracketed by events) and
 // keep the faster one; returns a JSON summary.
+// Restores the plan settings the autotuner changes, and frees its scratch
+// buffers, however the autotuner ends: on an error the plan keeps its former
+// kernel choices and screening settings.
+struct AutotuneScope {
+    tlno_gpu_plan* pl;
+    int timing, learned;
+    double gate, dthr;
+    signed char team_sel[6][6];
+    bool fp32_slower[6][6];
+    double *dD = nullptr, *dJ = nullptr, *dK = nullptr;
+    bool done = false;
+    explicit AutotuneScope(tlno_gpu_plan* p)
+        : pl(p), timing(p->timing), learned(p->learned), gate(p->fp32_gate), dthr(p->density_thr) {
+        std::memcpy(team_sel, p->team_sel, sizeof(team_sel));
+        std::memcpy(fp32_slower, p->fp32_slower, sizeof(fp32_slower));
+    }
+    ~AutotuneScope() {
+        if (!done) {  // error path: the former choices stand
+            std::memcpy(pl->team_sel, team_sel, sizeof(team_sel));
+            std::memcpy(pl->fp32_slower, fp32_slower, sizeof(fp32_slower));
+            (void)hipStreamSynchronize(pl->stream);
+        }
+        pl->timing = timing;
+        std::memset(pl->cls_ms, 0, sizeof(pl->cls_ms));
+        pl->timed_builds = 0;
+        pl->fp32_gate = gate;
+        pl->density_thr = dthr;
+        pl->learned = learned;
+        (void)hipFree(dD);
+        (void)hipFree(dJ);
+        (void)hipFree(dK);
+    }
+};
+
 std::string plan_autotune(tlno_gpu_plan* pl, double thr, int repeats) {
     HIP_OK(hipSetDevice(pl->device));
     const long mo2 = static_cast<long>(pl->Mout) * pl->Mout;
-    double *dD = nullptr, *dJ = nullptr, *dK = nullptr;
-    HIP_OK(hipMalloc(&dD, mo2 * sizeof(double)));
-    HIP_OK(hipMalloc(&dJ, mo2 * sizeof(double)));
-    HIP_OK(hipMalloc(&dK, mo2 * sizeof(double)));
+    AutotuneScope scope(pl);
+    HIP_OK(hipMalloc(&scope.dD, mo2 * sizeof(double)));
+    HIP_OK(hipMalloc(&scope.dJ, mo2 * sizeof(double)));
+    HIP_OK(hipMalloc(&scope.dK, mo2 * sizeof(double)));
+    double *dD = scope.dD, *dJ = scope.dJ, *dK = scope.dK;
     k_fill<<<grid_for(mo2, 256), 256, 0, pl->stream>>>(dD, mo2, 1e-3);
-    const int saved_timing = pl->timing;
-    const double saved_gate = pl->fp32_gate, saved_dthr = pl->density_thr;
-    const int saved_learned = pl->learned;
     pl->fp32_gate = 0.0;
     pl->density_thr = 0.0;
     pl->learned = 0;
@@ -2084,6 +2192,10 @@ std::string plan_autotune(tlno_gpu_plan* pl, double thr, int repeats) {
         for (int r = 0; r < repeats; ++r) jk_device(pl, dD, dJ, dK, thr);
         for (int X = 0; X < 6; ++X)
             for (int Y = 0; Y < 6; ++Y) ms[mode][X][Y] = pl->cls_ms[X][Y] / std::max<long long>(1, pl->timed_builds);
+        // test hook: an error in the middle of the autotuner (after the team
+        // mode changed every class choice), for the restore guarantee
+        if (mode == 1 && std::getenv("TLNO_TEST_AUTOTUNE_FAIL"))
+            throw std::runtime_error("tlno_gpu: autotune interrupted (TLNO_TEST_AUTOTUNE_FAIL)");
     }
     std::string rep = "{";
     bool first = true;
@@ -2111,15 +2223,8 @@ std::string plan_autotune(tlno_gpu_plan* pl, double thr, int repeats) {
             first = false;
         }
     rep += "}";
-    pl->timing = saved_timing;
-    std::memset(pl->cls_ms, 0, sizeof(pl->cls_ms));
-    pl->timed_builds = 0;
-    pl->fp32_gate = saved_gate;
-    pl->density_thr = saved_dthr;
-    pl->learned = saved_learned;
-    HIP_OK(hipFree(dD));
-    HIP_OK(hipFree(dJ));
-    HIP_OK(hipFree(dK));
+    HIP_OK(hipStreamSynchronize(pl->stream));
+    scope.done = true;
     return rep;
 }
 
@@ -2629,12 +2734,18 @@ int tlno_gpu_onee(tlno_gpu_plan* pl, int n_nuc, const double* Z, const double* R
 int tlno_gpu_boys(int n, const double* T, int mmax, double* out) {
     return guarded([&] {
         if (n <= 0) return;
-        if (mmax < 0 || mmax > 16) throw std::runtime_error("tlno_gpu: boys order must be 0..16");
+        if (mmax < 0 || mmax > TLNO_GPU_BOYS_MMAX)
+            throw std::runtime_error("tlno_gpu: boys order must be 0.." + std::to_string(TLNO_GPU_BOYS_MMAX));
+        if (!T || !out) throw std::runtime_error("tlno_gpu: boys arguments and output must not be null");
+        for (int i = 0; i < n; ++i)
+            if (!(std::isfinite(T[i]) && T[i] >= 0.0))
+                throw std::runtime_error("tlno_gpu: boys argument must be finite and >= 0");
         DeviceCtx& ctx = device_ctx(g_device);
         HIP_OK(hipSetDevice(g_device));
-        double *dT = nullptr, *dO = nullptr;
-        HIP_OK(hipMalloc(&dT, n * sizeof(double)));
-        HIP_OK(hipMalloc(&dO, static_cast<std::size_t>(n) * (mmax + 1) * sizeof(double)));
+        const std::size_t nout = static_cast<std::size_t>(n) * (mmax + 1);
+        std::lock_guard<std::mutex> lk(ctx.util_mu);
+        double* dT = ctx.util_buffer(n + nout);
+        double* dO = dT + n;
         HIP_OK(hipMemcpy(dT, T, n * sizeof(double), hipMemcpyHostToDevice));
         const unsigned g = grid_for(n, 128);
         switch (mmax) {
@@ -2645,24 +2756,29 @@ int tlno_gpu_boys(int n, const double* T, int mmax, double* out) {
             TLNO_BOYS_CASE(10) TLNO_BOYS_CASE(11) TLNO_BOYS_CASE(12) TLNO_BOYS_CASE(13) TLNO_BOYS_CASE(14)
             TLNO_BOYS_CASE(15) TLNO_BOYS_CASE(16)
 #undef TLNO_BOYS_CASE
+            default: k_boys_any<<<g, 128>>>(dT, dO, n, mmax); break;
         }
         HIP_OK(hipGetLastError());
-        HIP_OK(hipMemcpy(out, dO, static_cast<std::size_t>(n) * (mmax + 1) * sizeof(double), hipMemcpyDeviceToHost));
-        HIP_OK(hipFree(dT));
-        HIP_OK(hipFree(dO));
+        HIP_OK(hipMemcpy(out, dO, nout * sizeof(double), hipMemcpyDeviceToHost));
     });
 }
 
 int tlno_gpu_rys(int N, int n, const double* T, double* roots, double* weights) {
     return guarded([&] {
         if (n <= 0) return;
-        if (N < 1 || N > RYS_NMAX) throw std::runtime_error("tlno_gpu: Rys order must be 1..5");
+        if (N < 1 || N > RYS_NMAX)
+            throw std::runtime_error("tlno_gpu: Rys order must be 1.." + std::to_string(RYS_NMAX));
+        if (!T || !roots || !weights) throw std::runtime_error("tlno_gpu: Rys arguments and outputs must not be null");
+        for (int i = 0; i < n; ++i)
+            if (!(std::isfinite(T[i]) && T[i] >= 0.0))
+                throw std::runtime_error("tlno_gpu: Rys argument must be finite and >= 0");
         DeviceCtx& ctx = device_ctx(g_device);
         HIP_OK(hipSetDevice(g_device));
-        double *dT = nullptr, *dR = nullptr, *dW = nullptr;
-        HIP_OK(hipMalloc(&dT, n * sizeof(double)));
-        HIP_OK(hipMalloc(&dR, static_cast<std::size_t>(n) * N * sizeof(double)));
-        HIP_OK(hipMalloc(&dW, static_cast<std::size_t>(n) * N * sizeof(double)));
+        const std::size_t nr = static_cast<std::size_t>(n) * N;
+        std::lock_guard<std::mutex> lk(ctx.util_mu);
+        double* dT = ctx.util_buffer(n + 2 * nr);
+        double* dR = dT + n;
+        double* dW = dR + nr;
         HIP_OK(hipMemcpy(dT, T, n * sizeof(double), hipMemcpyHostToDevice));
         const unsigned g = grid_for(n, 128);
         switch (N) {
@@ -2673,11 +2789,8 @@ int tlno_gpu_rys(int N, int n, const double* T, double* roots, double* weights) 
             case 5: k_rys<5><<<g, 128>>>(ctx.tb, dT, dR, dW, n); break;
         }
         HIP_OK(hipGetLastError());
-        HIP_OK(hipMemcpy(roots, dR, static_cast<std::size_t>(n) * N * sizeof(double), hipMemcpyDeviceToHost));
-        HIP_OK(hipMemcpy(weights, dW, static_cast<std::size_t>(n) * N * sizeof(double), hipMemcpyDeviceToHost));
-        HIP_OK(hipFree(dT));
-        HIP_OK(hipFree(dR));
-        HIP_OK(hipFree(dW));
+        HIP_OK(hipMemcpy(roots, dR, nr * sizeof(double), hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(weights, dW, nr * sizeof(double), hipMemcpyDeviceToHost));
     });
 }
 

@@ -111,7 +111,9 @@ template <int MMAX>
 __device__ __forceinline__ void boys_all(const DevTables& tb, double T, double* F) {
     static_assert(MMAX + BOYS_TAYLOR - 1 <= BOYS_MMAX, "Boys order beyond the tabulated orders");
     if (T < BOYS_TCUT) {
-        const int k = static_cast<int>(T * BOYS_STEPS + 0.5);
+        // T >= 0 for every integral argument; the clamp keeps a negative
+        // argument from a caller inside the table
+        const int k = max(static_cast<int>(T * BOYS_STEPS + 0.5), 0);
         const double dd = k * (1.0 / BOYS_STEPS) - T;  // F_m(T) = sum_j F_{m+j}(T_k) dd^j / j!
         const double* row = tb.boys + k * (BOYS_MMAX + 1);
 #pragma unroll
@@ -177,7 +179,7 @@ __device__ __forceinline__ void rys_roots(const DevTables& tb, double T, double*
     } else {
         constexpr int ttab[6] = {0, 41, 49, 55, 61, 67};
         if (T < ttab[N]) {
-            const int k = static_cast<int>(T);
+            const int k = max(static_cast<int>(T), 0);
             const double u = T - k - 0.5;
             const double* c = tb.rys + tb.rys_off[N] + k * (2 * N * RYS_NCOEF);
 #pragma unroll
@@ -217,7 +219,7 @@ struct RootEval {
         } else {
             constexpr int ttab[6] = {0, 41, 49, 55, 61, 67};
             if (T < ttab[N]) {
-                const int k = static_cast<int>(T);
+                const int k = max(static_cast<int>(T), 0);
                 u = T - k - 0.5;
                 c = tb.rys + tb.rys_off[N] + k * (2 * N * RYS_NCOEF);
             } else {

@@ -77,3 +77,30 @@ def test_facade_matches_reference_and_mirror(name, tmp_path):
     F = [O.boys(m, 2.5) for m in range(5)]
     want = [(-1.4) ** m * F[m] for m in range(5)] + [O.boys(3, 17.25)]
     np.testing.assert_allclose(res["boys"], want, rtol=1e-13)
+
+
+def test_facade_boys_masked_lanes_contract(tmp_path):
+    """Masked Boys seeds through the facade: every width, mask and lane
+    rotation, orders through 32, NaN inactive inputs and a null input array
+    with no active lane (boys_mask_contract.cpp:17-47), and the unmasked seeds
+    against the oracle's Boys function."""
+    import subprocess
+    drv = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "boys_mask_driver")
+    out = tmp_path / "seeds.txt"
+    r = subprocess.run([drv, str(out)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and r.stdout.startswith("PASS"), r.stdout + r.stderr
+    orders = (0, 1, 2, 3, 4, 5, 6, 7, 8, 12, 32)
+    # every (order, rotation, mask) block of (n + 1) x W seeds was compared
+    assert int(r.stdout.split()[1]) == sum(8 * (1 << W) * (n + 1) * W for n in orders for W in (1, 2, 4, 8))
+    args = [0.0, 0.005, 39.999999999, 40.0, 40.000000001, 43.99999999, 56.0, 1000.0]
+    seen = 0
+    for line in out.read_text().splitlines():
+        _, W, n, *vals = line.split()
+        W, n = int(W), int(n)
+        v = np.array([float(x) for x in vals]).reshape(n + 1, W)
+        for lane in range(W):
+            for m in range(n + 1):
+                want = (-1.5) ** m * O.boys(m, args[lane])
+                assert abs(v[m, lane] - want) <= 1e-14 + 1e-12 * abs(want), (W, n, lane, m, v[m, lane], want)
+                seen += 1
+    assert seen == sum((n + 1) * W for n in orders for W in (1, 2, 4, 8))

@@ -94,3 +94,45 @@ def test_mixed_precision_after_autotune():
     assert (sq1, pq1) == (sq0, pq0)
     assert 0 <= st["fp32_shell_quartets"] <= sq1
     assert ej <= 1e-7 and ek <= 1e-7
+
+
+def test_autotune_error_leaves_plan_unchanged(monkeypatch):
+    """An autotuner that fails part-way (forced from the host after it has
+    switched every class to the team evaluator and the screens off) restores
+    the plan: the next build evaluates the same quartets with the same
+    screening and precision settings and gives the same J/K."""
+    from tlno_amd import learned
+    from tlno_amd._lib import TlnoGpuError
+    params, margin = learned.load_weights()
+    b, D = configs.water_cluster(10)
+    plan = DirectPlan(b, ScreeningOptions(integral_threshold=1e-12), cartesian=True)
+    plan.set_mixed_precision(1e-6)
+    plan.set_learned_screen(params, margin)
+    J0, K0 = plan.jk(D, density_threshold=2e-9)
+    st0, r0, m0 = plan.last_build_stats(), plan.last_route_stats(), plan.last_mixed_stats()
+    monkeypatch.setenv("TLNO_TEST_AUTOTUNE_FAIL", "1")
+    with pytest.raises(TlnoGpuError, match="autotune interrupted"):
+        plan.autotune(schwarz=1e-12, repeats=1)
+    monkeypatch.delenv("TLNO_TEST_AUTOTUNE_FAIL")
+    J1, K1 = plan.jk(D, density_threshold=2e-9)
+    assert plan.last_build_stats() == st0 and plan.last_route_stats() == r0
+    assert plan.last_mixed_stats()["fp32_shell_quartets"] == m0["fp32_shell_quartets"]
+    assert np.abs(J1 - J0).max() < 1e-13 and np.abs(K1 - K0).max() < 1e-13
+
+
+def test_persistent_launch_path_matches(monkeypatch):
+    """Class launches above the grid cap run persistent (a capped grid
+    striding over the work items); forced here with a cap of 64 waves, for
+    the lane and the team evaluators, plain and routed builds."""
+    b, D = configs.water_cluster(10)
+    plan = DirectPlan(b, ScreeningOptions(integral_threshold=1e-12), cartesian=True)
+    for team in (0, 30):
+        plan.set_team_threshold(team)
+        ref = [plan.jk(D), plan.jk(D, density_threshold=2e-9)]
+        counts = plan.last_build_stats()
+        monkeypatch.setenv("TLNO_MAX_GRID_WAVES", "64")
+        got = [plan.jk(D), plan.jk(D, density_threshold=2e-9)]
+        assert plan.last_build_stats() == counts
+        monkeypatch.delenv("TLNO_MAX_GRID_WAVES")
+        for (Ja, Ka), (Jb, Kb) in zip(ref, got):
+            assert np.abs(Ja - Jb).max() < 1e-12 and np.abs(Ka - Kb).max() < 1e-12

@@ -143,3 +143,16 @@ def test_cpp_facade_fails_loudly_without_a_device(tmp_path):
     r, res = fio.run(b, D, D, str(tmp_path))
     assert r.returncode == 1 and not res
     assert "tlno_gpu" in r.stderr or "hip" in r.stderr.lower()
+
+
+@pytest.mark.parametrize("bad", [-1.0, -1e-3, float("nan"), float("inf")])
+def test_boys_and_rys_reject_invalid_arguments_on_host(bad):
+    """A negative or non-finite argument is rejected before any device work
+    (the device tables are indexed by T)."""
+    from tlno_amd import core_math
+    with pytest.raises(RuntimeError, match="boys argument must be finite and >= 0"):
+        core_math.boys([0.5, bad], 4)
+    with pytest.raises(RuntimeError, match="Rys argument must be finite and >= 0"):
+        core_math.rys(3, [bad])
+    with pytest.raises(RuntimeError, match="boys order must be 0..128"):
+        core_math.boys([0.5], 129)
