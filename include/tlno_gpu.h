@@ -92,6 +92,24 @@ int tlno_gpu_build_jk(tlno_gpu_plan* plan, const double* density, double* J,
 int tlno_gpu_build_jk_device(tlno_gpu_plan* plan, const double* d_density,
                              double* d_J, double* d_K, double schwarz);
 
+/* Allocation-free variant of tlno_gpu_build_jk_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results and screening/precision settings; the build's
+ * internal-order density, J/K accumulators and counters live in a static
+ * device array of the library (one per GPU, 24 MiB + 4 KiB, held until the
+ * process ends), so the plan may have at most 1024 Cartesian functions
+ * (larger plans are rejected with an error).  The call allocates nothing:
+ * the plan's work lists (per Schwarz threshold), routing buffers, team
+ * quartet lists and the device's side streams must already exist, i.e. one
+ * tlno_gpu_build_jk_device (or tlno_gpu_build_jk) call on this plan with the
+ * same threshold and settings has run; otherwise it returns an error and
+ * does no work.  All calls on one GPU share its array: calls that overlap in
+ * time on one GPU (several host threads, several plans) race and must be
+ * serialized by the caller. */
+int tlno_gpu_build_jk_device_noalloc(tlno_gpu_plan* plan, const double* d_density,
+                                     double* d_J, double* d_K, double schwarz);
+
 /* Density-coupled screening, EriPackedBuildOptions::density_threshold /
  * ScreeningOptions::density_threshold (twoe_integrals.hpp:58-64,
  * jk_direct.hpp:18-24): following builds also skip shell quartets with

@@ -71,8 +71,33 @@ int side_streams_for_queues() {
     return std::max(1, std::min(kSideStreams, q));
 }
 
+// ---- allocation-free builds ----
+// Scratch of tlno_gpu_build_jk_device_noalloc: the internal-order density and
+// the J/K accumulators (3 x Mcart^2 doubles, Mcart <= kNoAllocMaxM) followed
+// by the work counters (4 x kCounterSlots) and one scalar block.  A static
+// device array: every GPU the library runs on holds its own copy (24 MiB +
+// 4 KiB) until the process ends; host code reaches it only through the
+// address hipGetSymbolAddress returns per device (DeviceCtx::scratch).
+constexpr int kNoAllocMaxM = 1024;
+constexpr std::size_t kScratchTail = 4 * kCounterSlots + 8;
+constexpr std::size_t kScratchDoubles = 3 * static_cast<std::size_t>(kNoAllocMaxM) * kNoAllocMaxM + kScratchTail;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+// Set while an allocation-free build runs on this thread: every lazily
+// allocating step (work lists of a new threshold, routing buffers, team
+// quartet lists, side streams) raises instead of allocating.
+thread_local bool g_no_alloc = false;
+
+void require_alloc(const char* what) {
+    if (g_no_alloc)
+        throw std::runtime_error(std::string("tlno_gpu_build_jk_device_noalloc: ") + what +
+                                 " not prepared; run tlno_gpu_build_jk_device once on this plan with the same "
+                                 "threshold and settings first");
+}
+
 struct DeviceCtx {
     bool ready = false;
+    double* scratch = nullptr;  // this device's copy of the static scratch array
     double* boys = nullptr;
     double* rys = nullptr;
     DevTables tb{};
@@ -102,6 +127,7 @@ struct DeviceCtx {
 // creates the device's side streams and events (caller holds side_mu)
 void ensure_side_streams(DeviceCtx& c) {
     if (c.nside > 0) return;
+    require_alloc("side streams of this device");
     const int n = side_streams_for_queues();
     for (int s = 0; s < n; ++s) {
         HIP_OK(hipStreamCreateWithFlags(&c.side[s], hipStreamNonBlocking));
@@ -141,6 +167,7 @@ DeviceCtx& device_ctx(int dev) {
     HIP_OK(hipMemcpy(c.boys, t.boys.data(), t.boys.size() * sizeof(double), hipMemcpyHostToDevice));
     HIP_OK(hipMalloc(&c.rys, t.rys.size() * sizeof(double)));
     HIP_OK(hipMemcpy(c.rys, t.rys.data(), t.rys.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_OK(hipGetSymbolAddress(reinterpret_cast<void**>(&c.scratch), HIP_SYMBOL(scratch)));
     c.tb.boys = c.boys;
     c.tb.rys = c.rys;
     for (int i = 0; i < RYS_NMAX + 2; ++i) c.tb.rys_off[i] = t.rys_off[i];
@@ -1443,6 +1470,7 @@ void balance_list_slices(const tlno_gpu_plan* pl, const std::vector<int>& bk, in
 TaskSet& get_tasks(tlno_gpu_plan* pl, double thr) {
     auto it = pl->tasks.find(thr);
     if (it != pl->tasks.end()) return it->second;
+    require_alloc("work lists for this Schwarz threshold");
     TaskSet ts;
     std::vector<int4> all;
     std::vector<long long> pref(static_cast<std::size_t>(pl->npair) + 1, 0);
@@ -1584,6 +1612,7 @@ int team_slices_for(long long nq) {
 // capacity is the number of 64-ket chunks of its rows (either route).
 void ensure_route(tlno_gpu_plan* pl, TaskSet& ts, double thr) {
     if (ts.d_rbase) return;
+    require_alloc("routing buffers for this threshold");
     std::vector<int4> rows;
     std::vector<long long> rbase, rtask;
     long long q = 0, cap = 0;
@@ -1669,6 +1698,7 @@ const int2* ensure_team_quartets(tlno_gpu_plan* pl, TaskSet& ts, int X, int Y) {
     const long long nt = ts.cnt[X][Y];
     if (nt <= 0 || static_cast<long long>(cum.size()) != nt + 1 || cum.back() <= 0) return nullptr;
     long long* d_cum = nullptr;
+    require_alloc("team-evaluator quartet lists");
     HIP_OK(hipMalloc(&ts.d_tq[X][Y], cum.back() * sizeof(int2)));
     HIP_OK(hipMalloc(&d_cum, cum.size() * sizeof(long long)));
     HIP_OK(hipMemcpyAsync(d_cum, cum.data(), cum.size() * sizeof(long long), hipMemcpyHostToDevice, pl->stream));
@@ -1680,7 +1710,19 @@ const int2* ensure_team_quartets(tlno_gpu_plan* pl, TaskSet& ts, int X, int Y) {
     return ts.d_tq[X][Y];
 }
 
-void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, double thr) {
+// Work buffers of one build: the plan's own, or the device's static scratch
+// (allocation-free builds).
+struct BuildBuffers {
+    double* Dint;
+    double* Jacc;
+    double* Kacc;
+    unsigned long long* counter;
+    double* scalar;
+};
+
+BuildBuffers plan_buffers(tlno_gpu_plan* pl) { return {pl->dDint, pl->dJacc, pl->dKacc, pl->d_counter, pl->d_scalar}; }
+
+void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, double thr, const BuildBuffers& bb) {
     DeviceCtx& ctx = device_ctx(pl->device);
     HIP_OK(hipSetDevice(pl->device));
     if (!std::isfinite(thr) || thr < 0.0)
@@ -1692,36 +1734,36 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
     C2SArgs c2s;
     std::memcpy(c2s.c, c2s_table().c, sizeof(c2s.c));
     if (pl->spherical)
-        k_prep_sph<<<grid_for(mc2, 256), 256, 0, st>>>(dD, pl->dDint, pl->d_intL, pl->d_intloc, pl->d_intsb,
+        k_prep_sph<<<grid_for(mc2, 256), 256, 0, st>>>(dD, bb.Dint, pl->d_intL, pl->d_intloc, pl->d_intsb,
                                                        pl->Mcart, pl->Msph, c2s);
     else
-        k_prep_cart<<<grid_for(mc2, 256), 256, 0, st>>>(dD, pl->dDint, pl->d_int2basis, pl->Mcart);
+        k_prep_cart<<<grid_for(mc2, 256), 256, 0, st>>>(dD, bb.Dint, pl->d_int2basis, pl->Mcart);
     HIP_OK(hipGetLastError());
-    if (dJ) k_fill<<<grid_for(mc2, 256), 256, 0, st>>>(pl->dJacc, mc2, 0.0);
-    if (dK) k_fill<<<grid_for(mc2, 256), 256, 0, st>>>(pl->dKacc, mc2, 0.0);
+    if (dJ) k_fill<<<grid_for(mc2, 256), 256, 0, st>>>(bb.Jacc, mc2, 0.0);
+    if (dK) k_fill<<<grid_for(mc2, 256), 256, 0, st>>>(bb.Kacc, mc2, 0.0);
     JKArgs a{};
     a.pd = pl->pair_data();
     a.tb = ctx.tb;
-    a.D = pl->dDint;
-    a.J = pl->dJacc;
-    a.K = pl->dKacc;
+    a.D = bb.Dint;
+    a.J = bb.Jacc;
+    a.K = bb.Kacc;
     a.M = pl->Mcart;
     a.do_j = dJ != nullptr;
     a.do_k = dK != nullptr;
     double maxD = 0.0;
     if (pl->fp32_gate > 0.0 && (a.do_j || a.do_k)) {
         // max |D| in the caller's basis, the same basis as the Schwarz bounds
-        k_fill<<<1, 64, 0, st>>>(pl->d_scalar, 1, 0.0);
+        k_fill<<<1, 64, 0, st>>>(bb.scalar, 1, 0.0);
         k_absmax<<<static_cast<unsigned>(std::min<long>(1024, grid_for(mo2, 256))), 256, 0, st>>>(
-            dD, mo2, reinterpret_cast<unsigned long long*>(pl->d_scalar));
-        HIP_OK(hipMemcpyAsync(&maxD, pl->d_scalar, sizeof(double), hipMemcpyDeviceToHost, st));
+            dD, mo2, reinterpret_cast<unsigned long long*>(bb.scalar));
+        HIP_OK(hipMemcpyAsync(&maxD, bb.scalar, sizeof(double), hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
     }
     pl->last_maxD = maxD;
     a.cut = nullptr;
     a.qlist = nullptr;
-    a.counter = pl->d_counter;
-    k_fill<<<grid_for(4 * kCounterSlots, 256), 256, 0, st>>>(reinterpret_cast<double*>(pl->d_counter), 4 * kCounterSlots, 0.0);
+    a.counter = bb.counter;
+    k_fill<<<grid_for(4 * kCounterSlots, 256), 256, 0, st>>>(reinterpret_cast<double*>(bb.counter), 4 * kCounterSlots, 0.0);
     if (pl->density_thr > 0.0 && (a.do_j || a.do_k)) {
         const int ng = static_cast<int>(pl->groups.size());
         k_fill<<<grid_for(ng, 256), 256, 0, st>>>(reinterpret_cast<double*>(pl->d_gmax), ng, 0.0);
@@ -1762,7 +1804,7 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
         ra.fa = FeatArgs{pl->d_pf0, pl->d_pf1, pl->d_pcen};
         ra.margin = pl->learned_margin;
         ra.gate = pl->fp32_gate;
-        ra.maxD = pl->d_scalar;
+        ra.maxD = bb.scalar;
         ra.qlist = pl->d_qlist;
         ra.t64 = pl->d_t64;
         ra.t32 = pl->d_t32;
@@ -1803,13 +1845,17 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
             long long nwaves;
             double cost;
         };
-        std::vector<Job> jobs;
+        // at most two jobs (FP64 and FP32 share) per class pair; a fixed
+        // array keeps the host side free of heap allocations as well
+        Job jobs[72];
+        std::size_t njobs = 0;
         auto add_jobs = [&](int X, int Y, bool fp32, const JKArgs& args, long long nwaves, int team_min,
                             double cost) {
             // the FP32 share of a class whose FP32 kernels measured slower
             // runs the FP64 kernels on the same tasks
             if (fp32 && pl->fp32_slower[X][Y]) fp32 = false;
-            jobs.push_back({X, Y, team_min, fp32, args, nwaves, cost});
+            if (njobs == 72) throw std::runtime_error("tlno_gpu: too many class launches");
+            jobs[njobs++] = Job{X, Y, team_min, fp32, args, nwaves, cost};
         };
         for (int X = 5; X >= 0 && route; --X)
             for (int Y = X; Y >= 0; --Y) {
@@ -1918,7 +1964,8 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
             // splitting the chunk groups of a class over streams measured no
             // better: concurrent kernels do not add up their serial times)
             double load[kSideStreams] = {};
-            for (Job& j : jobs) {
+            for (std::size_t i = 0; i < njobs; ++i) {
+                Job& j = jobs[i];
                 int best = 0;
                 for (int q = 1; q < ctx.nside; ++q)
                     if (load[q] < load[best]) best = q;
@@ -1927,10 +1974,10 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
             }
         } else {
             // timing mode: one stream, class order, events around each class's jobs
-            for (std::size_t i = 0; i < jobs.size(); ++i) {
+            for (std::size_t i = 0; i < njobs; ++i) {
                 Job& j = jobs[i];
                 const bool first = i == 0 || jobs[i - 1].X != j.X || jobs[i - 1].Y != j.Y;
-                const bool last = i + 1 == jobs.size() || jobs[i + 1].X != j.X || jobs[i + 1].Y != j.Y;
+                const bool last = i + 1 == njobs || jobs[i + 1].X != j.X || jobs[i + 1].Y != j.Y;
                 if (first) HIP_OK(hipEventRecord(pl->ev[j.X][j.Y][0], st));
                 launch_jk(g_kernels[j.X][j.Y], j.fp32, j.args, j.nwaves, st, j.team_min);
                 if (last) HIP_OK(hipEventRecord(pl->ev[j.X][j.Y][1], st));
@@ -1944,32 +1991,32 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
     }
     if (dJ) {
         if (pl->spherical)
-            k_fin_sph<<<grid_for(mo2, 256), 256, 0, st>>>(pl->dJacc, dJ, pl->d_sphL, pl->d_sphloc, pl->d_sphib,
+            k_fin_sph<<<grid_for(mo2, 256), 256, 0, st>>>(bb.Jacc, dJ, pl->d_sphL, pl->d_sphloc, pl->d_sphib,
                                                           pl->Mcart, pl->Msph, c2s);
         else
-            k_fin_cart<<<grid_for(mo2, 256), 256, 0, st>>>(pl->dJacc, dJ, pl->d_int2basis, pl->Mcart);
+            k_fin_cart<<<grid_for(mo2, 256), 256, 0, st>>>(bb.Jacc, dJ, pl->d_int2basis, pl->Mcart);
     }
     if (dK) {
         if (pl->spherical)
-            k_fin_sph<<<grid_for(mo2, 256), 256, 0, st>>>(pl->dKacc, dK, pl->d_sphL, pl->d_sphloc, pl->d_sphib,
+            k_fin_sph<<<grid_for(mo2, 256), 256, 0, st>>>(bb.Kacc, dK, pl->d_sphL, pl->d_sphloc, pl->d_sphib,
                                                           pl->Mcart, pl->Msph, c2s);
         else
-            k_fin_cart<<<grid_for(mo2, 256), 256, 0, st>>>(pl->dKacc, dK, pl->d_int2basis, pl->Mcart);
+            k_fin_cart<<<grid_for(mo2, 256), 256, 0, st>>>(bb.Kacc, dK, pl->d_int2basis, pl->Mcart);
     }
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(st));
     {
         unsigned long long evaluated[4] = {0, 0, 0, 0};
-        std::vector<unsigned long long> slots(4 * kCounterSlots);
-        HIP_OK(hipMemcpy(slots.data(), pl->d_counter, slots.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        unsigned long long slots[4 * kCounterSlots];
+        HIP_OK(hipMemcpy(slots, bb.counter, sizeof(slots), hipMemcpyDeviceToHost));
         for (int i = 0; i < 4 * kCounterSlots; ++i) evaluated[i % 4] += slots[i];
         pl->last_evaluated = static_cast<long long>(evaluated[0]);
         pl->last_evaluated_prim = static_cast<long long>(evaluated[1]);
         pl->last_fp32_sq = static_cast<long long>(evaluated[2]);
         pl->last_fp32_pq = static_cast<long long>(evaluated[3]);
         if (route) {
-            std::vector<unsigned long long> rs(4 * kCounterSlots);
-            HIP_OK(hipMemcpy(rs.data(), pl->d_rstats, rs.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+            unsigned long long rs[4 * kCounterSlots];
+            HIP_OK(hipMemcpy(rs, pl->d_rstats, sizeof(rs), hipMemcpyDeviceToHost));
             for (int i = 0; i < 4; ++i) pl->last_route[i] = 0;
             for (int i = 0; i < 4 * kCounterSlots; ++i) pl->last_route[i % 4] += static_cast<long long>(rs[i]);
         } else {
@@ -2186,10 +2233,10 @@ std::string plan_autotune(tlno_gpu_plan* pl, double thr, int repeats) {
         for (int X = 0; X < 6; ++X)
             for (int Y = 0; Y < 6; ++Y) pl->team_sel[X][Y] = static_cast<signed char>(mode == 1 ? 1 : 0);
         pl->fp32_gate = mode == 2 ? 1e300 : 0.0;
-        jk_device(pl, dD, dJ, dK, thr);  // warm-up
+        jk_device(pl, dD, dJ, dK, thr, plan_buffers(pl));  // warm-up
         std::memset(pl->cls_ms, 0, sizeof(pl->cls_ms));
         pl->timed_builds = 0;
-        for (int r = 0; r < repeats; ++r) jk_device(pl, dD, dJ, dK, thr);
+        for (int r = 0; r < repeats; ++r) jk_device(pl, dD, dJ, dK, thr, plan_buffers(pl));
         for (int X = 0; X < 6; ++X)
             for (int Y = 0; Y < 6; ++Y) ms[mode][X][Y] = pl->cls_ms[X][Y] / std::max<long long>(1, pl->timed_builds);
         // test hook: an error in the middle of the autotuner (after the team
@@ -2550,7 +2597,7 @@ int tlno_gpu_build_jk(tlno_gpu_plan* pl, const double* density, double* J, doubl
         const std::size_t bytes = static_cast<std::size_t>(M) * M * sizeof(double);
         HIP_OK(hipSetDevice(pl->device));
         HIP_OK(hipMemcpyAsync(pl->dDin, density, bytes, hipMemcpyHostToDevice, pl->stream));
-        jk_device(pl, pl->dDin, J ? pl->dJout : nullptr, K ? pl->dKout : nullptr, schwarz);
+        jk_device(pl, pl->dDin, J ? pl->dJout : nullptr, K ? pl->dKout : nullptr, schwarz, plan_buffers(pl));
         if (J) HIP_OK(hipMemcpy(J, pl->dJout, bytes, hipMemcpyDeviceToHost));
         if (K) HIP_OK(hipMemcpy(K, pl->dKout, bytes, hipMemcpyDeviceToHost));
     });
@@ -2560,7 +2607,28 @@ int tlno_gpu_build_jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, do
     return guarded([&] {
         if (!pl) throw std::runtime_error("jk_direct: null J/K plan backend.");
         if (!dD) throw std::runtime_error("eri_ao_tensor_hgp: null density.");
-        jk_device(pl, dD, dJ, dK, schwarz);
+        jk_device(pl, dD, dJ, dK, schwarz, plan_buffers(pl));
+    });
+}
+
+int tlno_gpu_build_jk_device_noalloc(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, double schwarz) {
+    return guarded([&] {
+        if (!pl) throw std::runtime_error("jk_direct: null J/K plan backend.");
+        if (!dD) throw std::runtime_error("tlno_gpu_build_jk_device_noalloc: density must not be null");
+        if (pl->Mcart > kNoAllocMaxM)
+            throw std::runtime_error("tlno_gpu_build_jk_device_noalloc: " + std::to_string(pl->Mcart) +
+                                     " Cartesian functions exceed the scratch limit of " +
+                                     std::to_string(kNoAllocMaxM) + " (3 M^2 doubles of scratch per device)");
+        DeviceCtx& ctx = device_ctx(pl->device);  // set up when the plan was created: no allocation here
+        const std::size_t mc2 = static_cast<std::size_t>(pl->Mcart) * pl->Mcart;
+        BuildBuffers bb{ctx.scratch, ctx.scratch + mc2, ctx.scratch + 2 * mc2,
+                        reinterpret_cast<unsigned long long*>(ctx.scratch + 3 * mc2),
+                        ctx.scratch + 3 * mc2 + 4 * kCounterSlots};
+        struct Flag {
+            Flag() { g_no_alloc = true; }
+            ~Flag() { g_no_alloc = false; }
+        } flag;
+        jk_device(pl, dD, dJ, dK, schwarz, bb);
     });
 }
 

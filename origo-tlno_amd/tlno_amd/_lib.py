@@ -30,6 +30,7 @@ SIGNATURES = {
     "tlno_gpu_quartet_maxabs": (C.c_int, [C.c_void_p, C.c_int, I, I, D]),
     "tlno_gpu_build_jk": (C.c_int, [C.c_void_p, D, D, D, C.c_double]),
     "tlno_gpu_build_jk_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double]),
+    "tlno_gpu_build_jk_device_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double]),
     "tlno_gpu_last_build_stats": (C.c_int, [C.c_void_p, LL, LL]),
     "tlno_gpu_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "tlno_gpu_set_mixed_precision": (C.c_int, [C.c_void_p, C.c_double]),
