@@ -219,6 +219,51 @@ def water100_run(gate=1e-6, tau=SCREEN_TAU):
     return out
 
 
+def onee_run(repeats=5):
+    """One-electron S, T, V (Cartesian, the reference's default screening:
+    group-pair decay 1e-15, nuclear primitive pairs 1e-15 / sum|Z|) on
+    organic30 (340 functions, 30 nuclei) and water100 (2500 functions, 300
+    nuclei): device time of the S/T/V kernels (HIP events) and the wall time
+    of the call with the host copies of the three matrices; the reference
+    engine's S/T/V of organic30 on one host thread beside it."""
+    lib = _lib.load()
+    out = {}
+    for name, (b, _) in (("organic30", configs.organic30()), ("water100", configs.water100())):
+        plan = DirectPlan(b, ScreeningOptions(integral_threshold=1e-12), cartesian=True)
+        Z = np.ascontiguousarray(b.nuclear_charges, dtype=np.float64)
+        R = np.ascontiguousarray(b.nuclear_centers, dtype=np.float64)
+        M = plan.M
+        S, T, V = (np.empty((M, M)) for _ in range(3))
+        st = np.zeros(6)
+        D = _lib.D
+
+        def call():
+            _lib.check(lib.tlno_gpu_onee_screened(plan.handle, len(Z), Z.ctypes.data_as(D), R.ctypes.data_as(D),
+                                                  1e-15, 1e-15, S.ctypes.data_as(D), T.ctypes.data_as(D),
+                                                  V.ctypes.data_as(D), st.ctypes.data_as(D)))
+        call()
+        kms, wall = [], []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            call()
+            wall.append((time.perf_counter() - t0) * 1e3)
+            kms.append(float(st[5]))
+        out[name] = {"n_basis": M, "nuclei": len(Z), "shell_groups": int(st[0]), "kept_group_pairs": int(st[2]),
+                     "nuclear_prim_pairs_kept": int(st[4]), "kernel_ms": float(np.median(kms)),
+                     "call_ms_with_host_copies": float(np.median(wall)),
+                     "prim_pair_nucleus_terms_per_s": st[4] * len(Z) / (np.median(kms) * 1e-3)}
+    try:
+        from oracle import oracle as O
+        ref = O.Reference()
+        b, _ = configs.organic30()
+        t0 = time.perf_counter()
+        ref.onee(b, b.nuclear_charges, b.nuclear_centers, spherical=False, screening=True)
+        out["organic30"]["reference_cpu_ms_1_thread_incl_plan"] = (time.perf_counter() - t0) * 1e3
+    except Exception as e:  # reference not built on this host
+        out["organic30"]["reference_cpu_ms_1_thread_incl_plan"] = f"unavailable: {e}"
+    return out
+
+
 def routing_profile_run(builds=3, gate=1e-6, tau=SCREEN_TAU):
     """The learned+mixed water40 build alone (shipped weights, FP32 gate 1e-6,
     density threshold 2e-9, Schwarz 1e-12, no autotuning), `builds` times:
@@ -353,6 +398,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-water100", action="store_true")
+    ap.add_argument("--no-onee", action="store_true")
     ap.add_argument("--no-queue-probe", action="store_true",
                     help="skip the organic30 rerun with HIP's default 4 hardware queues")
     ap.add_argument("--routing-profile", type=int, default=0, metavar="BUILDS",
@@ -424,6 +470,9 @@ def main():
     if not args.no_water100:
         print("bench: water100 learned-routed builds", file=sys.stderr, flush=True)
         result["learned_water100"] = water100_run()
+    if not args.no_onee:
+        print("bench: one-electron S/T/V", file=sys.stderr, flush=True)
+        result["onee"] = onee_run()
     if not args.no_batch:
         print("bench: 256-molecule batch builds", file=sys.stderr, flush=True)
         result["batched"] = batched_run()
@@ -447,7 +496,7 @@ def queue_probe(steps, warmup, queues=4):
     import subprocess
     env = dict(os.environ, GPU_MAX_HW_QUEUES=str(queues), TLNO_BENCH_QUEUE_PROBE="1")
     cmd = [sys.executable, os.path.abspath(__file__), "--no-cpu", "--no-mixed", "--no-batch", "--no-water100",
-           "--no-queue-probe", "--steps", str(steps), "--warmup", str(warmup)]
+           "--no-onee", "--no-queue-probe", "--steps", str(steps), "--warmup", str(warmup)]
     r = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=600)
     if r.returncode != 0:
         return {"hw_queues": queues, "error": r.stderr[-400:]}

@@ -221,6 +221,24 @@ int tlno_gpu_rys(int N, int n, const double* T, double* roots, double* weights);
 int tlno_gpu_onee(tlno_gpu_plan* plan, int n_nuc, const double* Z, const double* R,
                   double* S, double* T, double* V);
 
+/* The same S, T, V with the reference's one-electron screening
+ * (integrals/one_electron/onee_matrices.cpp:100-210 group pairs,
+ * onee_integrals.cpp:541-556 nuclear primitive pairs): shell pairs whose decay
+ * estimate K_a K_b exp(-mu_min AB^2) is below decay_eps are dropped, kept
+ * pairs whose overlap bound is below it skip V, and the V contribution of a
+ * primitive pair whose monopole estimate is below nuclear_eps / sum|Z| is
+ * skipped (0 turns either screen off; the reference's defaults are 1e-15 and
+ * 1e-15).  tlno_gpu_onee is this call with both thresholds 0.  Any output may
+ * be NULL.  stats (6 doubles, may be NULL): shell groups, possible group
+ * pairs G(G+1)/2, kept pairs, primitive pairs of the pairs that evaluate V,
+ * of those kept by the nuclear screen (the counts of the reference's
+ * tlno_onee_pair_stats / tlno_onee_nuclear_pair_stats,
+ * benchmarks/one_electron_engine/src/onee_capi.cpp:168-209), and the device
+ * milliseconds of the S/T/V kernels. */
+int tlno_gpu_onee_screened(tlno_gpu_plan* plan, int n_nuc, const double* Z, const double* R,
+                           double decay_eps, double nuclear_eps, double* S, double* T, double* V,
+                           double* stats);
+
 /* ---- molecule batches (no reference counterpart: the reference builds one
  * molecule per call).  n_mol molecules in the Cartesian component basis,
  * their component arrays concatenated in molecule order (n_comp[m]

@@ -23,6 +23,14 @@ ONEE_SRC := $(REF)/integrals/one_electron/onee_integrals.cpp \
 TWOE_SRC := $(REF)/integrals/two_electron/twoe_integrals.cpp \
             $(REF)/integrals/two_electron/fock/jk_direct.cpp
 
+# the reference's existing one-electron C ABI (tlno_onee_*, with its named
+# basis tables and .inp reader): fixture generation only, AVX-512 build
+ONEE_CAPI_SRC := $(REF)/benchmarks/one_electron_engine/src/onee_capi.cpp \
+                 $(REF)/benchmarks/one_electron_engine/src/basis_export.cpp \
+                 $(REF)/benchmarks/common/molecule_io.cpp \
+                 $(wildcard $(REF)/core/basis/*.cpp) \
+                 $(wildcard $(REF)/core/molecule/*.cpp)
+
 ISAS := avx512 avx2
 march_avx512 := -march=x86-64-v4
 march_avx2 := -march=x86-64-v3
@@ -30,7 +38,12 @@ march_avx2 := -march=x86-64-v3
 API_LIBS := $(foreach i,$(ISAS),$(OUT)/libtlno_ref_$(i).so)
 AUDIT_LIBS := $(foreach i,$(ISAS),$(OUT)/libtlno_ref_audit_$(i).so)
 
-all: $(API_LIBS) $(AUDIT_LIBS)
+ONEE_CAPI_LIB := $(OUT)/libtlno_ref_onee_capi.so
+
+all: $(API_LIBS) $(AUDIT_LIBS) $(ONEE_CAPI_LIB)
+
+$(ONEE_CAPI_LIB): $(CORE_SRC) $(ONEE_SRC) $(ONEE_CAPI_SRC) | $(OUT)
+	$(CXX) -std=c++17 -O2 -fopenmp -fPIC -I$(REF) $(march_avx512) -shared -o $@ $^
 
 $(OUT):
 	mkdir -p $(OUT)

@@ -13,56 +13,19 @@
 
 #include "core/basis/types.hpp"
 #include "core/math/general_math.hpp"
-#include "integrals/one_electron/onee_matrices.hpp"
 #include "integrals/two_electron/fock/eri_direct_jk.hpp"
 #include "integrals/two_electron/fock/jk_direct.hpp"
 #include "integrals/two_electron/twoe_integrals.hpp"
+#include "facade_internal.hpp"
 #include "tlno_gpu.h"
 
 namespace {
 
 using tlno::core::basis::Shell;
-
-void check(int rc) {
-    if (rc != 0) throw std::runtime_error(tlno_gpu_last_error());
-}
-
-bool finite_nonneg(double v) { return std::isfinite(v) && v >= 0.0; }
-
-// A GPU plan plus the lock that serializes builds on it (a plan owns one set
-// of device work buffers).
-struct GpuPlan {
-    tlno_gpu_plan* h = nullptr;
-    int M = 0, n_groups = 0, n_pairs = 0;
-    long long n_prim_pairs = 0;
-    std::mutex mu;
-    ~GpuPlan() {
-        if (h) tlno_gpu_plan_destroy(h);
-    }
-};
-
-std::shared_ptr<GpuPlan> make_gpu_plan(const std::vector<Shell>& basis, int spherical) {
-    const int n = static_cast<int>(basis.size());
-    std::vector<double> centers(3 * static_cast<std::size_t>(n)), exps, coefs;
-    std::vector<int> ang(3 * static_cast<std::size_t>(n)), nprim(n);
-    for (int i = 0; i < n; ++i) {
-        for (int x = 0; x < 3; ++x) {
-            centers[3 * i + x] = basis[i].center[x];
-            ang[3 * i + x] = basis[i].ang[x];
-        }
-        nprim[i] = static_cast<int>(basis[i].primitives.size());
-        for (const auto& p : basis[i].primitives) {
-            exps.push_back(p.exponent);
-            coefs.push_back(p.coefficient);
-        }
-    }
-    check(tlno_gpu_init(0));
-    auto g = std::make_shared<GpuPlan>();
-    check(tlno_gpu_plan_create(n, centers.data(), ang.data(), nprim.data(), exps.data(), coefs.data(), spherical,
-                               &g->h));
-    check(tlno_gpu_plan_info(g->h, &g->M, &g->n_groups, &g->n_pairs, &g->n_prim_pairs));
-    return g;
-}
+using tlno_amd_facade::check;
+using tlno_amd_facade::finite_nonneg;
+using tlno_amd_facade::GpuPlan;
+using tlno_amd_facade::make_gpu_plan;
 
 void require_gpu_screens(double fastschwarz, double decay) {
     if (!finite_nonneg(fastschwarz))
@@ -454,90 +417,3 @@ DirectResult build_rhf_g(const DirectPlan& plan, const std::vector<double>& dens
 
 }  // namespace tlno::integrals::twoe::fock
 
-// ------------------------------------------------------------------ onee --
-namespace tlno::integrals::onee {
-
-struct OneElectronPlanBackend {
-    std::vector<basis::Shell> shells;
-    mutable std::mutex mu;
-    mutable std::shared_ptr<GpuPlan> cart, sph;
-    GpuPlan& get(bool spherical) const {
-        std::lock_guard<std::mutex> lk(mu);
-        auto& p = spherical ? sph : cart;
-        if (!p) p = make_gpu_plan(shells, spherical ? 1 : 0);
-        return *p;
-    }
-};
-
-namespace {
-bool g_screening = true;
-double g_decay_eps = 1e-15;
-
-OneElectronPlan make_plan(const basis::BasisBuildResult& build, bool spherical) {
-    auto b = std::make_shared<OneElectronPlanBackend>();
-    b->shells = build.basis;
-    OneElectronPlan plan;
-    plan.n_cart = static_cast<int>(build.basis.size());
-    plan.n_sph = b->get(true).M;
-    if (!spherical) (void)b->get(false);
-    plan.backend = std::move(b);
-    return plan;
-}
-
-void evaluate(const basis::BasisBuildResult& build, const OneElectronPlan& plan, bool spherical,
-              OneElectronMatrices& mats) {
-    if (!plan.backend) throw std::runtime_error("onee: null one-electron plan.");
-    if (build.nuclear_charges.size() != build.nuclear_centers.size())
-        throw std::runtime_error("onee: nuclear charges and centres differ in length.");
-    GpuPlan& g = plan.backend->get(spherical);
-    const int n = g.M;
-    std::vector<double> Z(build.nuclear_charges.begin(), build.nuclear_charges.end()), R;
-    for (const auto& c : build.nuclear_centers) R.insert(R.end(), c.begin(), c.end());
-    mats = OneElectronMatrices(n);
-    std::lock_guard<std::mutex> lk(g.mu);
-    check(tlno_gpu_onee(g.h, static_cast<int>(Z.size()), Z.data(), R.data(), mats.overlap.values.data(),
-                        mats.kinetic.values.data(), mats.nuclear.values.data()));
-}
-}  // namespace
-
-OneElectronPlan build_one_electron_plan(const basis::BasisBuildResult& build) { return make_plan(build, false); }
-OneElectronPlan build_one_electron_plan(const basis::BasisBuildResult& build, bool) { return make_plan(build, false); }
-OneElectronPlan build_one_electron_spherical_plan(const basis::BasisBuildResult& build, bool) {
-    return make_plan(build, true);
-}
-
-OneElectronMatrices evaluate_one_electron_matrices(const basis::BasisBuildResult& build, const OneElectronPlan& plan) {
-    OneElectronMatrices m;
-    evaluate(build, plan, false, m);
-    return m;
-}
-
-void evaluate_one_electron_matrices_into(const basis::BasisBuildResult& build, const OneElectronPlan& plan,
-                                         OneElectronMatrices& mats) {
-    evaluate(build, plan, false, mats);
-}
-
-void evaluate_one_electron_matrices_sph_into(const basis::BasisBuildResult& build, const OneElectronPlan& plan,
-                                             OneElectronMatrices& mats) {
-    evaluate(build, plan, true, mats);
-}
-
-OneElectronMatrices build_one_electron_matrices(const basis::BasisBuildResult& build) {
-    return evaluate_one_electron_matrices(build, build_one_electron_plan(build));
-}
-
-OneElectronMatrices build_one_electron_spherical_matrices(const basis::BasisBuildResult& build) {
-    OneElectronMatrices m;
-    evaluate(build, build_one_electron_spherical_plan(build), true, m);
-    return m;
-}
-
-void set_onee_screening_enabled(bool enabled) {
-    g_screening = enabled;
-    g_decay_eps = 1e-15;
-}
-bool onee_screening_enabled() { return g_screening; }
-void set_onee_group_pair_decay_eps(double eps) { g_decay_eps = eps; }
-double onee_group_pair_decay_eps() { return g_decay_eps; }
-
-}  // namespace tlno::integrals::onee

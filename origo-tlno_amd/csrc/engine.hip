@@ -296,6 +296,7 @@ struct GroupH {
     std::vector<double> exps, cr;  // radial coefficients c / N_A
     int comp[15];                  // basis index of each local canonical component
     int first = 0;                 // first-appearance component
+    double na_first = 1.0;         // angular normalization of that component
     int int_base = 0;
     int sph_base = 0;
 };
@@ -416,6 +417,18 @@ struct tlno_gpu_plan {
     // optional per-class kernel timing with HIP events on the plan stream
     int timing = 0;
     hipEvent_t ev[6][6][2] = {};
+    // one-electron evaluation: kept pair list of the last decay threshold and
+    // work buffers, kept between calls
+    double onee_decay = -1.0;
+    std::vector<int2> onee_list;
+    int onee_cb[7] = {0};
+    long long onee_kept = 0;
+    int2* d_onee_list = nullptr;
+    double* d_onee_X = nullptr;  // 3 Mcart^2 internal-order S, T, V
+    double* d_onee_O = nullptr;  // 3 Mout^2 caller-order S, T, V
+    double* d_onee_nuc = nullptr;
+    int onee_nuc_cap = 0;
+    hipEvent_t onee_ev[2] = {nullptr, nullptr};
     double cls_ms[6][6] = {};
     long long timed_builds = 0;
 
@@ -506,6 +519,7 @@ std::vector<GroupH> build_groups(int n, const double* centers, const int* ang, c
             for (int p = 0; p < np; ++p) G.cr[p] = cf[p] / na;
             for (int k = 0; k < 15; ++k) G.comp[k] = -1;
             G.first = i;
+            G.na_first = na;
             gi = static_cast<int>(groups.size());
             groups.push_back(std::move(G));
         }
@@ -2290,6 +2304,98 @@ void validate_density(const double* D, int M) {
 
 }  // namespace tlno_amd
 
+namespace tlno_amd {
+
+// One-electron group-pair screening of the reference
+// (integrals/one_electron/onee_matrices.cpp:100-210): with c0 the stored
+// coefficients of a group's first component, K_g = sum_i |c0_i| (pi/a_i)^{3/4}
+// and a_min its smallest exponent, a pair is dropped when
+// K_a K_b exp(-mu_min AB^2) < eps (mu_min from the two smallest exponents);
+// a kept pair evaluates V only when its overlap bound
+// sum_ij |c0a_i c0b_j| (pi/p)^{3/2} exp(-chi AB^2) reaches eps.  eps = 0
+// keeps every pair with V.  The list is grouped by pair class (plan order).
+void onee_select_pairs(tlno_gpu_plan* pl, double eps) {
+    const int ng = static_cast<int>(pl->groups.size());
+    std::vector<double> K(ng, 0.0), amin(ng, 0.0);
+    for (int g = 0; g < ng; ++g) {
+        const GroupH& G = pl->groups[g];
+        double a0 = G.exps[0], k = 0.0;
+        for (int i = 0; i < G.nprim; ++i) {
+            a0 = std::min(a0, G.exps[i]);
+            k += std::abs(G.cr[i] * G.na_first) * std::pow(3.14159265358979323846 / G.exps[i], 0.75);
+        }
+        K[g] = k;
+        amin[g] = a0;
+    }
+    std::vector<int2> list;
+    list.reserve(pl->npair);
+    long long kept = 0;
+    for (int c = 0; c < 6; ++c) {
+        pl->onee_cb[c] = static_cast<int>(list.size());
+        for (int P = pl->cls_begin[c]; P < pl->cls_begin[c + 1]; ++P) {
+            const GroupH& A = pl->groups[pl->pgA[P]];
+            const GroupH& B = pl->groups[pl->pgB[P]];
+            int do_v = 1;
+            if (eps > 0.0) {
+                double AB2 = 0.0;
+                for (int x = 0; x < 3; ++x) AB2 += (A.C[x] - B.C[x]) * (A.C[x] - B.C[x]);
+                const int ga = pl->pgA[P], gb = pl->pgB[P];
+                const double mu = amin[ga] * amin[gb] / (amin[ga] + amin[gb]);
+                if (K[ga] * K[gb] * std::exp(-mu * AB2) < eps) continue;
+                double bound = 0.0;
+                for (int i = 0; i < A.nprim; ++i)
+                    for (int j = 0; j < B.nprim; ++j) {
+                        const double p = A.exps[i] + B.exps[j];
+                        const double chi = A.exps[i] * B.exps[j] / p;
+                        bound += std::abs(A.cr[i] * A.na_first * B.cr[j] * B.na_first) * std::pow(3.14159265358979323846 / p, 1.5) *
+                                 std::exp(-chi * AB2);
+                    }
+                do_v = bound >= eps;
+            }
+            list.push_back(make_int2(P, do_v));
+            ++kept;
+        }
+    }
+    pl->onee_cb[6] = static_cast<int>(list.size());
+    if (pl->d_onee_list) HIP_OK(hipFree(pl->d_onee_list));
+    pl->d_onee_list = nullptr;
+    HIP_OK(hipMalloc(&pl->d_onee_list, std::max<std::size_t>(1, list.size()) * sizeof(int2)));
+    if (!list.empty())
+        HIP_OK(hipMemcpy(pl->d_onee_list, list.data(), list.size() * sizeof(int2), hipMemcpyHostToDevice));
+    pl->onee_list = std::move(list);
+    pl->onee_kept = kept;
+    pl->onee_decay = eps;
+}
+
+// Nuclear primitive-pair accounting of the reference's
+// tlno_onee_nuclear_pair_stats (benchmarks/one_electron_engine/src/
+// onee_capi.cpp:183-209): over kept pairs that evaluate V, all primitive
+// pairs, and those whose monopole estimate (2 pi/p) exp(-chi AB^2)
+// max|c_a| max|c_b| reaches thr.
+void onee_nuclear_counts(const tlno_gpu_plan* pl, double thr, long long* total, long long* kept) {
+    long long t = 0, k = 0;
+    for (const int2& e : pl->onee_list) {
+        if (!e.y) continue;
+        const GroupH& A = pl->groups[pl->pgA[e.x]];
+        const GroupH& B = pl->groups[pl->pgB[e.x]];
+        const double na = onee_namax(A.L), nb = onee_namax(B.L);
+        double AB2 = 0.0;
+        for (int x = 0; x < 3; ++x) AB2 += (A.C[x] - B.C[x]) * (A.C[x] - B.C[x]);
+        t += static_cast<long long>(A.nprim) * B.nprim;
+        for (int i = 0; i < A.nprim; ++i)
+            for (int j = 0; j < B.nprim; ++j) {
+                const double p = A.exps[i] + B.exps[j];
+                const double chi = A.exps[i] * B.exps[j] / p;
+                const double scr = 2.0 * 3.14159265358979323846 / p * std::exp(-chi * AB2) * std::abs(A.cr[i]) * na * std::abs(B.cr[j]) * nb;
+                if (scr >= thr) ++k;
+            }
+    }
+    *total = t;
+    *kept = k;
+}
+
+}  // namespace tlno_amd
+
 tlno_gpu_plan::~tlno_gpu_plan() {
     auto f = [](void* p) {
         if (p) (void)hipFree(p);
@@ -2301,6 +2407,9 @@ tlno_gpu_plan::~tlno_gpu_plan() {
     f(d_cut); f(d_counter);
     f(d_pf0); f(d_pf1); f(d_pcen); f(d_grows); f(d_blk); f(d_bsum); f(d_frag); f(d_mvec); f(d_mnorm);
     f(d_qlist); f(d_t64); f(d_t32); f(d_rstats);
+    f(d_onee_list); f(d_onee_X); f(d_onee_O); f(d_onee_nuc);
+    for (auto& e : onee_ev)
+        if (e) (void)hipEventDestroy(e);
     for (auto& kv : tasks) {
         f(kv.second.d_tasks);
         f(kv.second.d_bk);
@@ -2753,6 +2862,11 @@ int tlno_gpu_eri_cart_blocks(tlno_gpu_plan* pl, int n, const int* g4, const long
 
 int tlno_gpu_onee(tlno_gpu_plan* pl, int n_nuc, const double* Z, const double* R, double* S, double* T,
                   double* V) {
+    return tlno_gpu_onee_screened(pl, n_nuc, Z, R, 0.0, 0.0, S, T, V, nullptr);
+}
+
+int tlno_gpu_onee_screened(tlno_gpu_plan* pl, int n_nuc, const double* Z, const double* R, double decay_eps,
+                           double nuclear_eps, double* S, double* T, double* V, double* stats) {
     return guarded([&] {
         if (!pl) throw std::runtime_error("jk_direct: null J/K plan backend.");
         if (n_nuc < 0 || (n_nuc > 0 && (!Z || !R))) throw std::runtime_error("tlno_gpu: invalid nuclei");
@@ -2760,23 +2874,43 @@ int tlno_gpu_onee(tlno_gpu_plan* pl, int n_nuc, const double* Z, const double* R
             if (!std::isfinite(Z[c]) || !std::isfinite(R[3 * c]) || !std::isfinite(R[3 * c + 1]) ||
                 !std::isfinite(R[3 * c + 2]))
                 throw std::runtime_error("tlno_gpu: nuclear charges and centres must be finite");
+        if (!std::isfinite(decay_eps) || decay_eps < 0.0 || !std::isfinite(nuclear_eps) || nuclear_eps < 0.0)
+            throw std::runtime_error("onee: screening thresholds must be finite and >= 0.");
         DeviceCtx& ctx = device_ctx(pl->device);
         HIP_OK(hipSetDevice(pl->device));
         const hipStream_t st = pl->stream;
         const long mc2 = static_cast<long>(pl->Mcart) * pl->Mcart;
         const long mo2 = static_cast<long>(pl->Mout) * pl->Mout;
-        double *dR = nullptr, *dZ = nullptr, *dX = nullptr, *dO = nullptr;
-        HIP_OK(hipMalloc(&dR, std::max(1, 3 * n_nuc) * sizeof(double)));
-        HIP_OK(hipMalloc(&dZ, std::max(1, n_nuc) * sizeof(double)));
-        HIP_OK(hipMalloc(&dX, 3 * mc2 * sizeof(double)));
-        HIP_OK(hipMalloc(&dO, 3 * mo2 * sizeof(double)));
+        if (pl->onee_decay != decay_eps) onee_select_pairs(pl, decay_eps);
+        if (!pl->d_onee_X) {
+            HIP_OK(hipMalloc(&pl->d_onee_X, 3 * mc2 * sizeof(double)));
+            HIP_OK(hipMalloc(&pl->d_onee_O, 3 * mo2 * sizeof(double)));
+            HIP_OK(hipEventCreate(&pl->onee_ev[0]));
+            HIP_OK(hipEventCreate(&pl->onee_ev[1]));
+        }
+        if (n_nuc > pl->onee_nuc_cap) {
+            if (pl->d_onee_nuc) HIP_OK(hipFree(pl->d_onee_nuc));
+            pl->d_onee_nuc = nullptr;
+            pl->onee_nuc_cap = 0;
+            HIP_OK(hipMalloc(&pl->d_onee_nuc, 4 * static_cast<std::size_t>(n_nuc) * sizeof(double)));
+            pl->onee_nuc_cap = n_nuc;
+        }
+        double* dR = pl->d_onee_nuc;
+        double* dZ = pl->d_onee_nuc + 3 * static_cast<std::size_t>(pl->onee_nuc_cap);
         if (n_nuc > 0) {
             HIP_OK(hipMemcpyAsync(dR, R, 3 * n_nuc * sizeof(double), hipMemcpyHostToDevice, st));
             HIP_OK(hipMemcpyAsync(dZ, Z, n_nuc * sizeof(double), hipMemcpyHostToDevice, st));
         }
+        double zsum = 0.0;
+        for (int c = 0; c < n_nuc; ++c) zsum += std::abs(Z[c]);
+        const double vthr = zsum > 0.0 ? nuclear_eps / zsum : 0.0;
+        double* dX = pl->d_onee_X;
+        double* dO = pl->d_onee_O;
         k_fill<<<grid_for(3 * mc2, 256), 256, 0, st>>>(dX, 3 * mc2, 0.0);
-        launch_onee(pl->pair_data(), ctx.tb, pl->dpL, pl->dpexp, dR, dZ, n_nuc, pl->npair, pl->Mcart, dX, dX + mc2,
-                    dX + 2 * mc2, st);
+        HIP_OK(hipEventRecord(pl->onee_ev[0], st));
+        launch_onee(pl->pair_data(), ctx.tb, pl->dpL, pl->dpexp, dR, dZ, n_nuc, pl->d_onee_list, pl->onee_cb, vthr,
+                    pl->Mcart, dX, dX + mc2, dX + 2 * mc2, st);
+        HIP_OK(hipEventRecord(pl->onee_ev[1], st));
         C2SArgs c2s;
         std::memcpy(c2s.c, c2s_table().c, sizeof(c2s.c));
         for (int m = 0; m < 3; ++m) {
@@ -2792,10 +2926,19 @@ int tlno_gpu_onee(tlno_gpu_plan* pl, int n_nuc, const double* Z, const double* R
         double* outs[3] = {S, T, V};
         for (int m = 0; m < 3; ++m)
             if (outs[m]) HIP_OK(hipMemcpy(outs[m], dO + m * mo2, mo2 * sizeof(double), hipMemcpyDeviceToHost));
-        HIP_OK(hipFree(dR));
-        HIP_OK(hipFree(dZ));
-        HIP_OK(hipFree(dX));
-        HIP_OK(hipFree(dO));
+        if (stats) {
+            float ms = 0.0f;
+            HIP_OK(hipEventElapsedTime(&ms, pl->onee_ev[0], pl->onee_ev[1]));
+            long long tot = 0, kept = 0;
+            onee_nuclear_counts(pl, vthr, &tot, &kept);
+            const double ng = static_cast<double>(pl->groups.size());
+            stats[0] = ng;
+            stats[1] = ng * (ng + 1.0) / 2.0;
+            stats[2] = static_cast<double>(pl->onee_kept);
+            stats[3] = static_cast<double>(tot);
+            stats[4] = static_cast<double>(kept);
+            stats[5] = ms;
+        }
     });
 }
 

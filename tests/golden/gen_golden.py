@@ -26,6 +26,15 @@ tests) and the reference's outputs:
   checksum.npz     canonical spherical ERI stream checksums of organic30 at
                    Schwarz 1e-12 and 1e-8 (python gen_golden.py checksum
                    rewrites only it)
+  onee_capi.npz    the reference's one-electron C ABI (tlno_onee_*,
+                   benchmarks/one_electron_engine/src/onee_capi.cpp, built
+                   into oracle/_ref/libtlno_ref_onee_capi.so) on two of its
+                   own test molecules, h2o.inp with STO-3G and gly.inp with
+                   def2-SVP: the molecule texts, the basis parameters it
+                   exports (tlno_onee_basis_json) per element, nbf / nbf_sph,
+                   Cartesian and spherical S/T/V with screening on (its
+                   default) and off, pair and nuclear primitive-pair
+                   statistics (python gen_golden.py onee_capi rewrites only it)
   density.npz      4-water cluster (96 spherical functions), density
                    configs.spherical_density(96, 9, 20): the reference's
                    density-screened J, K and G (density_threshold 1e-9, 1e-7,
@@ -65,6 +74,8 @@ def main(only=None):
         return density(ref)
     if only == "checksum":
         return checksums(ref)
+    if only == "onee_capi":
+        return onee_capi()
     t0 = time.time()
     # ---------------------------------------------------------- Boys / Rys
     xs = np.concatenate([[0.0, 1e-10, 1e-3, 0.5, 5.0, 11.99, 12.0, 12.5, 25.0, 39.99, 40.0, 45.0, 80.0, 200.0],
@@ -128,7 +139,69 @@ def main(only=None):
     organic30(ref)
     density(ref)
     checksums(ref)
+    onee_capi()
     print(f"golden fixtures written in {time.time() - t0:.1f} s")
+
+
+ONEE_CASES = (("h2o", "sto-3g"), ("gly", "def2-svp"))
+
+
+def onee_capi():
+    # ---------------------------------------------------------- 1e C ABI
+    import ctypes as C
+    L = C.CDLL(os.path.join(ROOT, "oracle", "_ref", "libtlno_ref_onee_capi.so"))
+    L.tlno_onee_create_from_inp.restype = C.c_void_p
+    L.tlno_onee_create_from_inp.argtypes = [C.c_char_p, C.c_char_p]
+    L.tlno_onee_basis_json.restype = C.c_char_p
+    L.tlno_onee_basis_json.argtypes = [C.c_char_p, C.c_char_p]
+    for f in ("nbf", "nbf_sph"):
+        getattr(L, f"tlno_onee_{f}").argtypes = [C.c_void_p]
+    for f in ("prepare_plan", "eval_full", "eval_cached", "eval_full_sph", "eval_cached_sph", "destroy"):
+        getattr(L, f"tlno_onee_{f}").argtypes = [C.c_void_p]
+    D3 = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.tlno_onee_copy.argtypes = D3
+    L.tlno_onee_copy_sph.argtypes = D3
+    IP = C.POINTER(C.c_int)
+    LP = C.POINTER(C.c_longlong)
+    L.tlno_onee_pair_stats.argtypes = [C.c_void_p, IP, IP, IP]
+    L.tlno_onee_nuclear_pair_stats.argtypes = [C.c_void_p, LP, LP]
+    L.tlno_onee_set_screening.argtypes = [C.c_int]
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    out = {}
+    for mol, bas in ONEE_CASES:
+        path = os.path.join("/root/reference/test_molecules", f"{mol}.inp")
+        text = open(path).read()
+        out[f"{mol}_inp"] = np.array(text)
+        symbols = sorted({ln.split()[0] for ln in text.splitlines()
+                          if ln.strip() and ln.split()[0][0].isalpha() and ln.split()[0].lower() != "xyz"})
+        for el in symbols:
+            js = L.tlno_onee_basis_json(bas.encode(), el.encode())
+            if js:
+                out[f"{bas}_{el}_json"] = np.array(js.decode())
+        for screen in (1, 0):
+            L.tlno_onee_set_screening(screen)
+            h = L.tlno_onee_create_from_inp(path.encode(), bas.encode())
+            assert h, (mol, bas)
+            n, ns = L.tlno_onee_nbf(h), L.tlno_onee_nbf_sph(h)
+            L.tlno_onee_prepare_plan(h)
+            L.tlno_onee_eval_cached(h)
+            S, T, V = (np.zeros((n, n)) for _ in range(3))
+            L.tlno_onee_copy(h, dp(S), dp(T), dp(V))
+            L.tlno_onee_eval_cached_sph(h)
+            Ss, Ts, Vs = (np.zeros((ns, ns)) for _ in range(3))
+            L.tlno_onee_copy_sph(h, dp(Ss), dp(Ts), dp(Vs))
+            g, tot, kept = C.c_int(), C.c_int(), C.c_int()
+            L.tlno_onee_pair_stats(h, C.byref(g), C.byref(tot), C.byref(kept))
+            nt, nk = C.c_longlong(), C.c_longlong()
+            L.tlno_onee_nuclear_pair_stats(h, C.byref(nt), C.byref(nk))
+            L.tlno_onee_destroy(h)
+            tag = f"{mol}_{'on' if screen else 'off'}"
+            out.update({f"{tag}_nbf": np.array([n, ns]), f"{tag}_S": S, f"{tag}_T": T, f"{tag}_V": V,
+                        f"{tag}_Ssph": Ss, f"{tag}_Tsph": Ts, f"{tag}_Vsph": Vs,
+                        f"{tag}_pair_stats": np.array([g.value, tot.value, kept.value]),
+                        f"{tag}_nuclear_pair_stats": np.array([nt.value, nk.value])})
+    L.tlno_onee_set_screening(1)
+    np.savez_compressed(os.path.join(HERE, "onee_capi.npz"), **out)
 
 
 def checksums(ref):
