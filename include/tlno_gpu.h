@@ -119,6 +119,27 @@ int tlno_gpu_build_jk_device_noalloc(tlno_gpu_plan* plan, const double* d_densit
  * so no quartet the reference keeps is dropped.  0 turns it off (default). */
 int tlno_gpu_set_density_threshold(tlno_gpu_plan* plan, double density_threshold);
 
+/* The reference's heuristic screens, ScreeningOptions::
+ * primitive_fastschwarz_threshold and ::decay_threshold
+ * (integrals/two_electron/fock/jk_direct.hpp:18-24,
+ * integrals/two_electron/twoe_integrals.hpp:41-66), for following J/K builds:
+ *   decay_threshold > 0: shell quartets with D_ab D_cd < decay_threshold are
+ *     skipped, D_ab = K_a K_b exp(-mu_min |AB|^2) the shell-pair decay
+ *     indicator (rys/rys_engine.ipp:2205-2241); together with the Schwarz
+ *     test when schwarz > 0, alone when schwarz = 0
+ *     (rys/native_quartets.ipp:690-705); not combinable with the density
+ *     threshold (the build then fails with the reference's message);
+ *   primitive_fastschwarz_threshold t > 0: primitive quartets with
+ *     fs_ab fs_cd < t^2 are skipped, fs = 24.739... w^2 p^{-1/2}
+ *     (native_quartets.ipp:595-600).  This engine tests every primitive
+ *     quartet; the reference tests a SIMD packet of bra pairs together and
+ *     keeps a primitive quartet when any pair of the packet passes, so it
+ *     skips a subset of what this engine skips (the two agree where every
+ *     packet holds one pair).
+ * 0 turns a screen off (the default).  The stream checksum ignores both. */
+int tlno_gpu_set_primitive_screens(tlno_gpu_plan* plan, double primitive_fastschwarz_threshold,
+                                   double decay_threshold);
+
 /* Canonical ERI stream checksum: the sum of every unique pure-spherical
  * (ab|cd) (a >= b, c >= d, pair(ab) >= pair(cd)) over shell quartets with
  * Q_ab Q_cd >= schwarz, the one-element result of the reference's

@@ -64,6 +64,12 @@ int orc_eri_cart_block(const orc_basis* b, int ga, int gb, int gc, int gd, doubl
 int orc_jk(const orc_basis* b, int spherical, double schwarz, const double* D,
            double* J, double* K, int nthreads);
 
+/* orc_jk with the reference's heuristic screens: decay > 0 adds the shell-pair
+ * decay indicator test (alone when schwarz == 0), fastschwarz > 0 the
+ * primitive FASTSCHWARZ test on every primitive quartet. */
+int orc_jk_screened(const orc_basis* b, int spherical, double schwarz, double decay, double fastschwarz,
+                    const double* D, double* J, double* K, int nthreads);
+
 /* Schwarz pair bound per group pair (gi >= gj, index gi*(gi+1)/2+gj). */
 int orc_schwarz(const orc_basis* b, int spherical, double* out);
 

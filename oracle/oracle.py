@@ -75,6 +75,7 @@ def lib():
         L.orc_group_info.argtypes = [C.c_void_p, C.c_int, _I, _I, _I]
         L.orc_eri_cart_block.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _D]
         L.orc_jk.argtypes = [C.c_void_p, C.c_int, C.c_double, _D, _D, _D, C.c_int]
+        L.orc_jk_screened.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, _D, _D, _D, C.c_int]
         L.orc_schwarz.argtypes = [C.c_void_p, C.c_int, _D]
         L.orc_onee.argtypes = [C.c_void_p, C.c_int, C.c_int, _D, _D, _D, _D, _D]
         _lib = L
@@ -148,14 +149,15 @@ class OracleBasis:
             raise ValueError(lib().orc_last_error().decode())
         return out.reshape(dims)
 
-    def jk(self, D, spherical=False, schwarz=0.0, want_j=True, want_k=True, nthreads=0):
+    def jk(self, D, spherical=False, schwarz=0.0, want_j=True, want_k=True, nthreads=0, decay=0.0,
+           fastschwarz=0.0):
         n = self.nsph if spherical else self.ncart
         D = np.ascontiguousarray(D, dtype=np.float64)
         assert D.shape == (n, n)
         J = np.zeros((n, n)) if want_j else None
         K = np.zeros((n, n)) if want_k else None
-        lib().orc_jk(self.h, int(spherical), float(schwarz), _dp(D),
-                     _dp(J) if want_j else None, _dp(K) if want_k else None, int(nthreads))
+        lib().orc_jk_screened(self.h, int(spherical), float(schwarz), float(decay), float(fastschwarz), _dp(D),
+                              _dp(J) if want_j else None, _dp(K) if want_k else None, int(nthreads))
         return J, K
 
     def schwarz(self, spherical=False):
@@ -211,6 +213,8 @@ def ref_lib(kind="api"):
         L.ref_direct_build.argtypes = [C.c_int, _D, _I, _I, _D, _D, C.c_double, C.c_double,
                                        C.c_int, _D, C.c_int, _D, C.c_int,
                                        C.POINTER(C.c_longlong)]
+        L.ref_direct_build_ex.argtypes = [C.c_int, _D, _I, _I, _D, _D, C.c_double, C.c_double, C.c_double,
+                                          C.c_double, C.c_int, _D, C.c_int, _D, C.c_int, C.POINTER(C.c_longlong)]
         L.ref_eri_checksum.argtypes = [C.c_int, _D, _I, _I, _D, _D, C.c_double, _D]
         L.ref_onee.argtypes = [C.c_int, _D, _I, _I, _D, _D, C.c_int, _I, _D, C.c_int, C.c_int,
                                _D, _D, _D, _I]
@@ -261,7 +265,7 @@ class Reference:
         return M
 
     def direct(self, basis, D, which, schwarz=0.0, density_threshold=0.0, threads=1,
-               repeats=1):
+               repeats=1, fastschwarz=0.0, decay=0.0):
         """which: 'J', 'K' or 'G'. Spherical M x M density. Returns (matrix, seconds)."""
         keep, args = _basis_args(basis)
         M = D.shape[0]
@@ -269,9 +273,9 @@ class Reference:
         out = np.zeros((M, M))
         ns = C.c_longlong(0)
         self.api.ref_set_threads(int(threads))
-        rc = self.api.ref_direct_build(*args, float(schwarz), float(density_threshold),
-                                       {"J": 0, "K": 1, "G": 2}[which], _dp(D), M, _dp(out),
-                                       int(repeats), C.byref(ns))
+        rc = self.api.ref_direct_build_ex(*args, float(schwarz), float(density_threshold), float(fastschwarz),
+                                          float(decay), {"J": 0, "K": 1, "G": 2}[which], _dp(D), M, _dp(out),
+                                          int(repeats), C.byref(ns))
         if rc:
             raise ValueError(self.err())
         return out, ns.value * 1e-9 / max(1, repeats)

@@ -43,8 +43,14 @@ void orc_rys_fast(int N, double T, double* roots, double* weights);
 #define PI_D 3.141592653589793238462643383279502884
 
 /* raw (pre-N_A) block in component order [a][b][c][d] */
+/* fs2 > 0: primitive FASTSCHWARZ screen (integrals/two_electron/rys/
+ * native_quartets.ipp:595-600, rys_engine.ipp:2243-2277): a primitive quartet
+ * is left out when fs_ab fs_cd < fs2, fs = 24.739... w^2 p^{-1/2},
+ * w = K c_a c_b / p (radial coefficients), applied here to every primitive
+ * quartet on its own. */
+#define ORC_FASTSCHWARZ_PREFACTOR 24.739429451193148050228357586821944304008810699990039583
 static void eri_block(const orc_group* A, const orc_group* B, const orc_group* C,
-                      const orc_group* D, int fast_roots, double* out) {
+                      const orc_group* D, int fast_roots, double fs2, double* out) {
     const int la = A->L, lb = B->L, lc = C->L, ld = D->L;
     const int LAB = la + lb, LCD = lc + ld;
     const int N = (LAB + LCD) / 2 + 1;
@@ -81,6 +87,12 @@ static void eri_block(const orc_group* A, const orc_group* B, const orc_group* C
                     for (int k = 0; k < 3; ++k) {
                         Q[k] = C->center[k] - de / q * CD[k];
                         PQ[k] = P[k] - Q[k];
+                    }
+                    if (fs2 > 0.0) {
+                        const double wab = Kab * cab / p, wcd = Kcd * ccd / q;
+                        const double fab = ORC_FASTSCHWARZ_PREFACTOR * wab * wab * sqrt(1.0 / p);
+                        const double fcd = ORC_FASTSCHWARZ_PREFACTOR * wcd * wcd * sqrt(1.0 / q);
+                        if (fab * fcd < fs2) continue;
                     }
                     const double pq = p + q;
                     const double rho = p * q / pq;
@@ -159,7 +171,7 @@ int orc_eri_cart_block(const orc_basis* b, int ga, int gb, int gc, int gd, doubl
         orc_set_error("oracle: group index out of range");
         return 1;
     }
-    eri_block(&b->groups[ga], &b->groups[gb], &b->groups[gc], &b->groups[gd], 0, out);
+    eri_block(&b->groups[ga], &b->groups[gb], &b->groups[gc], &b->groups[gd], 0, 0.0, out);
     return 0;
 }
 
@@ -189,10 +201,10 @@ static void block_to_sph(const orc_group* G[4], const double* cart, double* sph,
 }
 
 /* block of group quartet in the output basis; returns dims and AO indices */
-static void out_block(const orc_basis* b, int spherical, int gi, int gj, int gk, int gl,
+static void out_block(const orc_basis* b, int spherical, int gi, int gj, int gk, int gl, double fs2,
                       double* cart, double* outb, double* tmp, int* dims, int* idx[4]) {
     const orc_group* G[4] = {&b->groups[gi], &b->groups[gj], &b->groups[gk], &b->groups[gl]};
-    eri_block(G[0], G[1], G[2], G[3], 1, cart);
+    eri_block(G[0], G[1], G[2], G[3], 1, fs2, cart);
     for (int k = 0; k < 4; ++k) {
         if (spherical) {
             dims[k] = 2 * G[k]->L + 1;
@@ -227,7 +239,7 @@ int orc_schwarz(const orc_basis* b, int spherical, double* out) {
         for (int gi = 0; gi < ng; ++gi)
             for (int gj = 0; gj <= gi; ++gj) {
                 int dims[4];
-                out_block(b, spherical, gi, gj, gi, gj, cart, blk, tmp, dims, idx);
+                out_block(b, spherical, gi, gj, gi, gj, 0.0, cart, blk, tmp, dims, idx);
                 double mx = 0.0;
                 for (int a = 0; a < dims[0]; ++a)
                     for (int c = 0; c < dims[1]; ++c) {
@@ -243,14 +255,49 @@ int orc_schwarz(const orc_basis* b, int spherical, double* out) {
     return 0;
 }
 
+/* Heuristic shell-pair decay indicator (rys_engine.ipp:2205-2241):
+ * K_g = sum_i |c_i| (pi/a_i)^{3/4} over a group's radial coefficients,
+ * D_ab = K_a K_b exp(-mu_min |AB|^2), mu_min from the two smallest exponents. */
+static double decay_scale(const orc_group* g, double* amin) {
+    double k = 0.0, a0 = g->exps[0];
+    for (int i = 0; i < g->nprim; ++i) {
+        if (g->exps[i] < a0) a0 = g->exps[i];
+        k += fabs(g->c_radial[i]) * pow(PI_D / g->exps[i], 0.75);
+    }
+    *amin = a0;
+    return k;
+}
+
 int orc_jk(const orc_basis* b, int spherical, double schwarz, const double* Dm,
            double* J, double* K, int nthreads) {
+    return orc_jk_screened(b, spherical, schwarz, 0.0, 0.0, Dm, J, K, nthreads);
+}
+
+/* Shell quartets: with schwarz > 0, kept when Q_ab Q_cd >= schwarz and (decay
+ * > 0) D_ab D_cd >= decay; with schwarz == 0 and decay > 0, kept when
+ * D_ab D_cd >= decay (native_quartets.ipp:690-705, 49-53). */
+int orc_jk_screened(const orc_basis* b, int spherical, double schwarz, double decay, double fastschwarz,
+                    const double* Dm, double* J, double* K, int nthreads) {
     init_tables(b);
+    const double fs2 = fastschwarz * fastschwarz;
     const int n = spherical ? b->nsph : b->ncart;
     const int ng = b->ngroups;
     const int npair = ng * (ng + 1) / 2;
     double* Q = (double*)malloc(sizeof(double) * (size_t)npair);
     if (schwarz > 0.0) orc_schwarz(b, spherical, Q);
+    double* Dc = (double*)malloc(sizeof(double) * (size_t)npair);
+    if (decay > 0.0)
+        for (int gi = 0, p = 0; gi < ng; ++gi)
+            for (int gj = 0; gj <= gi; ++gj, ++p) {
+                double ai, aj;
+                const double ki = decay_scale(&b->groups[gi], &ai), kj = decay_scale(&b->groups[gj], &aj);
+                double r2 = 0.0;
+                for (int x = 0; x < 3; ++x) {
+                    const double d = b->groups[gi].center[x] - b->groups[gj].center[x];
+                    r2 += d * d;
+                }
+                Dc[p] = ki * kj * exp(-(ai * aj / (ai + aj)) * r2);
+            }
     int* pi = (int*)malloc(sizeof(int) * (size_t)npair);
     int* pj = (int*)malloc(sizeof(int) * (size_t)npair);
     for (int gi = 0, p = 0; gi < ng; ++gi)
@@ -287,9 +334,10 @@ int orc_jk(const orc_basis* b, int spherical, double schwarz, const double* Dm,
         for (int P = 0; P < npair; ++P)
             for (int R = 0; R <= P; ++R) {
                 if (schwarz > 0.0 && Q[P] * Q[R] < schwarz) continue;
+                if (decay > 0.0 && Dc[P] * Dc[R] < decay) continue;
                 const int g[4] = {pi[P], pj[P], pi[R], pj[R]};
                 int dims[4];
-                out_block(b, spherical, g[0], g[1], g[2], g[3], cart, blk, tmp, dims, idx);
+                out_block(b, spherical, g[0], g[1], g[2], g[3], fs2, cart, blk, tmp, dims, idx);
                 /* the 8 orderings as positions of (i,j,k,l) */
                 static const int perm[8][4] = {{0, 1, 2, 3}, {1, 0, 2, 3}, {0, 1, 3, 2}, {1, 0, 3, 2},
                                                {2, 3, 0, 1}, {3, 2, 0, 1}, {2, 3, 1, 0}, {3, 2, 1, 0}};
@@ -334,6 +382,7 @@ int orc_jk(const orc_basis* b, int spherical, double schwarz, const double* Dm,
     free(Jt);
     free(Kt);
     free(Q);
+    free(Dc);
     free(pi);
     free(pj);
     return 0;

@@ -123,18 +123,41 @@ REF_API int ref_plan_M(int n, const double* centers, const int* ang,
 // density/out are M x M row-major in the spherical AO basis.
 // repeats > 1 runs the build that many times on one plan (timing use);
 // elapsed_ns (optional) receives the wall time of those builds only.
+REF_API int ref_direct_build_ex(int n, const double* centers, const int* ang,
+                                const int* nprim, const double* exps,
+                                const double* coefs, double schwarz,
+                                double density_threshold, double fastschwarz,
+                                double decay, int which,
+                                const double* density, int M, double* out,
+                                int repeats, long long* elapsed_ns);
+
 REF_API int ref_direct_build(int n, const double* centers, const int* ang,
                              const int* nprim, const double* exps,
                              const double* coefs, double schwarz,
                              double density_threshold, int which,
                              const double* density, int M, double* out,
                              int repeats, long long* elapsed_ns) {
+    return ref_direct_build_ex(n, centers, ang, nprim, exps, coefs, schwarz, density_threshold, 0.0, 0.0, which,
+                               density, M, out, repeats, elapsed_ns);
+}
+
+// The same build with the primitive FASTSCHWARZ and decay thresholds of
+// ScreeningOptions (integrals/two_electron/fock/jk_direct.hpp:18-24).
+REF_API int ref_direct_build_ex(int n, const double* centers, const int* ang,
+                                const int* nprim, const double* exps,
+                                const double* coefs, double schwarz,
+                                double density_threshold, double fastschwarz,
+                                double decay, int which,
+                                const double* density, int M, double* out,
+                                int repeats, long long* elapsed_ns) {
     try {
         namespace fock = tlno::integrals::twoe::fock;
         auto basis = make_basis(n, centers, ang, nprim, exps, coefs);
         fock::ScreeningOptions opt;
         opt.integral_threshold = schwarz;
         opt.density_threshold = density_threshold;
+        opt.primitive_fastschwarz_threshold = fastschwarz;
+        opt.decay_threshold = decay;
         auto plan = fock::build_direct_plan(basis, opt);
         if (plan.profile.M != M) {
             g_err = "ref_direct_build: M mismatch";

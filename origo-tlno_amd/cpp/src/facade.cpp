@@ -27,25 +27,37 @@ using tlno_amd_facade::finite_nonneg;
 using tlno_amd_facade::GpuPlan;
 using tlno_amd_facade::make_gpu_plan;
 
-void require_gpu_screens(double fastschwarz, double decay) {
+void check_heuristic_screens(double fastschwarz, double decay) {
     if (!finite_nonneg(fastschwarz))
         throw std::runtime_error("eri_ao_tensor_hgp: primitive_fastschwarz_threshold must be finite and >= 0.");
     if (!finite_nonneg(decay)) throw std::runtime_error("eri_ao_tensor_hgp: decay_threshold must be finite and >= 0.");
-    if (fastschwarz != 0.0 || decay != 0.0)
-        throw std::runtime_error(
-            "eri_ao_tensor_hgp: primitive FASTSCHWARZ and decay screening are not available in the GPU engine.");
 }
 
-// J and/or K with Schwarz and density thresholds on a locked plan
-void build_jk(GpuPlan& g, const std::vector<double>& density, double schwarz, double density_threshold, double* J,
-              double* K) {
+// The stream checksum takes the Schwarz threshold only.
+void require_schwarz_only(double fastschwarz, double decay) {
+    check_heuristic_screens(fastschwarz, decay);
+    if (fastschwarz != 0.0 || decay != 0.0)
+        throw std::runtime_error(
+            "eri_ao_tensor_hgp: the GPU stream checksum applies Schwarz screening only (primitive FASTSCHWARZ and "
+            "decay thresholds must be 0).");
+}
+
+// All screening thresholds of one J/K build.
+struct Screens {
+    double schwarz, density, fastschwarz, decay;
+};
+
+// J and/or K on a locked plan
+void build_jk(GpuPlan& g, const std::vector<double>& density, const Screens& sc, double* J, double* K) {
     if (density.size() != static_cast<std::size_t>(g.M) * g.M)
         throw std::runtime_error("eri_ao_tensor_hgp: density must be spherical M x M row-major.");
-    if (!finite_nonneg(density_threshold))
+    if (!finite_nonneg(sc.density))
         throw std::runtime_error("eri_ao_tensor_hgp: density_threshold must be finite and >= 0.");
+    check_heuristic_screens(sc.fastschwarz, sc.decay);
     std::lock_guard<std::mutex> lk(g.mu);
-    check(tlno_gpu_set_density_threshold(g.h, density_threshold));
-    check(tlno_gpu_build_jk(g.h, density.data(), J, K, schwarz));
+    check(tlno_gpu_set_density_threshold(g.h, sc.density));
+    check(tlno_gpu_set_primitive_screens(g.h, sc.fastschwarz, sc.decay));
+    check(tlno_gpu_build_jk(g.h, density.data(), J, K, sc.schwarz));
 }
 
 tlno::integrals::twoe::EriPackedPlanProfile profile_of(const GpuPlan& g) {
@@ -167,7 +179,7 @@ EriPackedPlan build_eri_packed_plan_for_G_rys(const std::vector<tlno::core::basi
 
 EriPackedGData build_eri_packed_for_G_rys(const EriPackedPlan& plan, const EriPackedBuildOptions& options) {
     check_schwarz(options.schwarz_threshold);
-    require_gpu_screens(options.primitive_fastschwarz_threshold, options.decay_threshold);
+    require_schwarz_only(options.primitive_fastschwarz_threshold, options.decay_threshold);
     GpuPlan& g = gpu_of(plan);
     std::lock_guard<std::mutex> lk(g.mu);
     double v = 0.0;
@@ -178,11 +190,18 @@ EriPackedGData build_eri_packed_for_G_rys(const EriPackedPlan& plan, const EriPa
 EriDirectJKResult build_jk_direct_for_G_rys(const EriPackedPlan& plan, const std::vector<double>& density,
                                             const EriPackedBuildOptions& options) {
     check_schwarz(options.schwarz_threshold);
-    require_gpu_screens(options.primitive_fastschwarz_threshold, options.decay_threshold);
+    check_heuristic_screens(options.primitive_fastschwarz_threshold, options.decay_threshold);
+    if (options.density_threshold > 0.0 && options.decay_threshold > 0.0)
+        throw std::runtime_error(
+            "eri_ao_tensor_hgp: density screening requires Schwarz bounds and cannot be combined with the heuristic "
+            "decay metric.");
     GpuPlan& g = gpu_of(plan);
     const std::size_t n = static_cast<std::size_t>(g.M) * g.M;
     std::vector<double> J(n), K(n);
-    build_jk(g, density, options.schwarz_threshold, options.density_threshold, J.data(), K.data());
+    build_jk(g, density,
+             {options.schwarz_threshold, options.density_threshold, options.primitive_fastschwarz_threshold,
+              options.decay_threshold},
+             J.data(), K.data());
     EriDirectJKResult r;
     r.M = g.M;
     r.g.resize(n);
@@ -341,7 +360,11 @@ void validate(const ScreeningOptions& o) {
         throw std::runtime_error(
             "eri_ao_tensor_hgp: density screening requires Schwarz bounds and cannot be combined with the heuristic "
             "decay metric.");
-    require_gpu_screens(o.primitive_fastschwarz_threshold, o.decay_threshold);
+    check_heuristic_screens(o.primitive_fastschwarz_threshold, o.decay_threshold);
+}
+
+Screens screens_of(const ScreeningOptions& o) {
+    return {o.integral_threshold, o.density_threshold, o.primitive_fastschwarz_threshold, o.decay_threshold};
 }
 }  // namespace
 
@@ -387,7 +410,7 @@ CoulombJResult build_coulomb_j(const DirectPlan& plan, const std::vector<double>
     CoulombJResult r;
     r.M = g.M;
     r.j.resize(static_cast<std::size_t>(g.M) * g.M);
-    build_jk(g, density, options.integral_threshold, options.density_threshold, r.j.data(), nullptr);
+    build_jk(g, density, screens_of(options), r.j.data(), nullptr);
     return r;
 }
 
@@ -398,7 +421,7 @@ ExchangeKResult build_exchange_k(const DirectPlan& plan, const std::vector<doubl
     ExchangeKResult r;
     r.M = g.M;
     r.k.resize(static_cast<std::size_t>(g.M) * g.M);
-    build_jk(g, density, options.integral_threshold, options.density_threshold, nullptr, r.k.data());
+    build_jk(g, density, screens_of(options), nullptr, r.k.data());
     return r;
 }
 
@@ -407,7 +430,7 @@ DirectResult build_rhf_g(const DirectPlan& plan, const std::vector<double>& dens
     GpuPlan& g = gpu_of(plan);
     const std::size_t n = static_cast<std::size_t>(g.M) * g.M;
     std::vector<double> J(n), K(n);
-    build_jk(g, density, options.integral_threshold, options.density_threshold, J.data(), K.data());
+    build_jk(g, density, screens_of(options), J.data(), K.data());
     DirectResult r;
     r.M = g.M;
     r.g.resize(n);

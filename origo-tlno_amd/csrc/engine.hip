@@ -379,6 +379,11 @@ struct tlno_gpu_plan {
     double* d_scalar = nullptr;  // device scratch for max|D|
     // density screen (the reference's density_threshold): 0 = off
     double density_thr = 0.0;
+    // heuristic screens of the reference's ScreeningOptions: primitive
+    // FASTSCHWARZ threshold and shell-pair decay indicator threshold (0: off),
+    // decay indicator per pair (plan order)
+    double fastschwarz = 0.0, decay_thr = 0.0;
+    double* d_decay = nullptr;
     int* d_row_group = nullptr;              // [Mout] shell of each row of the caller's basis
     int *d_pgA = nullptr, *d_pgB = nullptr, *d_pcls = nullptr, *d_clsb = nullptr;
     double* d_Q = nullptr;                   // [npair] bounds in plan order
@@ -874,6 +879,8 @@ struct RouteArgs {
     int4* t64;
     int4* t32;
     const long long* rtask;       // first task slot of each row (in t64 and in t32)
+    const double* decay;          // per pair decay indicator (nullptr: no decay test)
+    double decay_thr;
     unsigned long long* stats;    // [kCounterSlots][4]: candidates, density-kept, kept, FP32
 };
 
@@ -922,6 +929,7 @@ __global__ __launch_bounds__(256) void k_route(RouteArgs a) {
             const double fac = a.mode == 1 ? coul : (a.mode == 2 ? exch : fmax(coul, exch));
             keep = keep && qq * fac >= a.tau;
         }
+        if (a.decay) keep = keep && a.decay[P] * a.decay[Qi] >= a.decay_thr;
         cand += valid ? 1 : 0;
         dkept += keep ? 1 : 0;
         double est = qq;
@@ -1278,6 +1286,29 @@ void build_plan(tlno_gpu_plan* pl, int n, const double* centers, const int* ang,
     upload_pairs(pl, final_order, prims_P, prims_w, prims_ab);
     pl->Q.resize(np);
     for (int k = 0; k < np; ++k) pl->Q[k] = Q[order2[k]];
+    // heuristic shell-pair decay indicator (rys_engine.ipp:2205-2241):
+    // D_ab = K_a K_b exp(-mu_min |AB|^2), K_g = sum_i |c_i| (pi/a_i)^{3/4}
+    // over the radial coefficients, mu_min from the two smallest exponents
+    {
+        std::vector<double> Kg(ng), amin(ng), dec(np);
+        for (int g = 0; g < ng; ++g) {
+            const GroupH& G = pl->groups[g];
+            double k = 0.0, a0 = G.exps[0];
+            for (int i = 0; i < G.nprim; ++i) {
+                a0 = std::min(a0, G.exps[i]);
+                k += std::abs(G.cr[i]) * std::pow(3.14159265358979323846 / G.exps[i], 0.75);
+            }
+            Kg[g] = k;
+            amin[g] = a0;
+        }
+        for (int k = 0; k < np; ++k) {
+            const int a = pl->pgA[k], b = pl->pgB[k];
+            double r2 = 0.0;
+            for (int x = 0; x < 3; ++x) r2 += (pl->groups[a].C[x] - pl->groups[b].C[x]) * (pl->groups[a].C[x] - pl->groups[b].C[x]);
+            dec[k] = Kg[a] * Kg[b] * std::exp(-(amin[a] * amin[b] / (amin[a] + amin[b])) * r2);
+        }
+        dev_upload(pl->d_decay, dec);
+    }
 
     const std::size_t mc2 = static_cast<std::size_t>(pl->Mcart) * pl->Mcart;
     const std::size_t mo2 = static_cast<std::size_t>(pl->Mout) * pl->Mout;
@@ -1794,7 +1825,14 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
         HIP_OK(hipGetLastError());
     }
     a.split = 1;
-    const bool route = (pl->density_thr > 0.0 || pl->learned) && (a.do_j || a.do_k);
+    if (pl->density_thr > 0.0 && pl->decay_thr > 0.0)
+        throw std::runtime_error(
+            "eri_ao_tensor_hgp: density screening requires Schwarz bounds and cannot be combined with the "
+            "heuristic decay metric.");
+    a.fs_min = pl->fastschwarz > 0.0
+                   ? pl->fastschwarz * pl->fastschwarz / (kFastSchwarzPrefactor * kFastSchwarzPrefactor)
+                   : 0.0;
+    const bool route = (pl->density_thr > 0.0 || pl->learned || pl->decay_thr > 0.0) && (a.do_j || a.do_k);
     unsigned tcount[72] = {};
     if (route) {
         ensure_route(pl, ts, thr);
@@ -1817,6 +1855,8 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
         ra.mlp = MlpPacked{pl->d_frag, pl->d_mvec, pl->d_mnorm, pl->mb3};
         ra.fa = FeatArgs{pl->d_pf0, pl->d_pf1, pl->d_pcen};
         ra.margin = pl->learned_margin;
+        ra.decay = pl->decay_thr > 0.0 ? pl->d_decay : nullptr;
+        ra.decay_thr = pl->decay_thr;
         ra.gate = pl->fp32_gate;
         ra.maxD = bb.scalar;
         ra.qlist = pl->d_qlist;
@@ -2194,13 +2234,14 @@ double eri_checksum(tlno_gpu_plan* pl, double thr) {
 struct AutotuneScope {
     tlno_gpu_plan* pl;
     int timing, learned;
-    double gate, dthr;
+    double gate, dthr, fastschwarz, decay;
     signed char team_sel[6][6];
     bool fp32_slower[6][6];
     double *dD = nullptr, *dJ = nullptr, *dK = nullptr;
     bool done = false;
     explicit AutotuneScope(tlno_gpu_plan* p)
-        : pl(p), timing(p->timing), learned(p->learned), gate(p->fp32_gate), dthr(p->density_thr) {
+        : pl(p), timing(p->timing), learned(p->learned), gate(p->fp32_gate), dthr(p->density_thr),
+          fastschwarz(p->fastschwarz), decay(p->decay_thr) {
         std::memcpy(team_sel, p->team_sel, sizeof(team_sel));
         std::memcpy(fp32_slower, p->fp32_slower, sizeof(fp32_slower));
     }
@@ -2216,6 +2257,8 @@ struct AutotuneScope {
         pl->fp32_gate = gate;
         pl->density_thr = dthr;
         pl->learned = learned;
+        pl->fastschwarz = fastschwarz;
+        pl->decay_thr = decay;
         (void)hipFree(dD);
         (void)hipFree(dJ);
         (void)hipFree(dK);
@@ -2234,6 +2277,8 @@ std::string plan_autotune(tlno_gpu_plan* pl, double thr, int repeats) {
     pl->fp32_gate = 0.0;
     pl->density_thr = 0.0;
     pl->learned = 0;
+    pl->fastschwarz = 0.0;
+    pl->decay_thr = 0.0;
     if (!pl->timing)
         for (int X = 0; X < 6; ++X)
             for (int Y = 0; Y < 6; ++Y)
@@ -2407,7 +2452,7 @@ tlno_gpu_plan::~tlno_gpu_plan() {
     f(d_cut); f(d_counter);
     f(d_pf0); f(d_pf1); f(d_pcen); f(d_grows); f(d_blk); f(d_bsum); f(d_frag); f(d_mvec); f(d_mnorm);
     f(d_qlist); f(d_t64); f(d_t32); f(d_rstats);
-    f(d_onee_list); f(d_onee_X); f(d_onee_O); f(d_onee_nuc);
+    f(d_onee_list); f(d_onee_X); f(d_onee_O); f(d_onee_nuc); f(d_decay);
     for (auto& e : onee_ev)
         if (e) (void)hipEventDestroy(e);
     for (auto& kv : tasks) {
@@ -2571,6 +2616,19 @@ int tlno_gpu_set_density_threshold(tlno_gpu_plan* pl, double density_threshold) 
         if (!std::isfinite(density_threshold) || density_threshold < 0.0)
             throw std::runtime_error("eri_ao_tensor_hgp: density_threshold must be finite and >= 0.");
         pl->density_thr = density_threshold;
+    });
+}
+
+int tlno_gpu_set_primitive_screens(tlno_gpu_plan* pl, double primitive_fastschwarz_threshold,
+                                   double decay_threshold) {
+    return guarded([&] {
+        if (!pl) throw std::runtime_error("jk_direct: null J/K plan backend.");
+        if (!std::isfinite(primitive_fastschwarz_threshold) || primitive_fastschwarz_threshold < 0.0)
+            throw std::runtime_error("eri_ao_tensor_hgp: primitive_fastschwarz_threshold must be finite and >= 0.");
+        if (!std::isfinite(decay_threshold) || decay_threshold < 0.0)
+            throw std::runtime_error("eri_ao_tensor_hgp: decay_threshold must be finite and >= 0.");
+        pl->fastschwarz = primitive_fastschwarz_threshold;
+        pl->decay_thr = decay_threshold;
     });
 }
 

@@ -352,7 +352,8 @@ __host__ __device__ constexpr bool flat_prims(int LA, int LB, int LC, int LD) {
 template <typename R, int LA, int LB, int LC, int LD, int AC, int AN, int CC, int CN>
 __device__ __forceinline__ void quartet(const DevTables& tb, const PairData& pd, int P, int Q, int ib0, int ib1,
                                         int j0, int js, bool uniform,
-                                        R (&acc)[Shape<LA, LB, LC, LD, AC, AN, CC, CN>::NOUT]) {
+                                        R (&acc)[Shape<LA, LB, LC, LD, AC, AN, CC, CN>::NOUT],
+                                        double fs_min = 0.0) {
     using S = Shape<LA, LB, LC, LD, AC, AN, CC, CN>;
     constexpr CartT<LA> ca = make_cart<LA>();
     constexpr CartT<LB> cb = make_cart<LB>();
@@ -373,6 +374,10 @@ __device__ __forceinline__ void quartet(const DevTables& tb, const PairData& pd,
     auto prim = [&](const double4 Pp, const double wab, const double4 Qp, const double wcd) {
         const double q = Qp.w;
         const double p = Pp.w;
+        if (fs_min > 0.0) {  // primitive FASTSCHWARZ screen (this lane's quartet only)
+            const double w2 = wab * wcd;
+            if (w2 * w2 * rsqrt_nr(p * q) < fs_min) return;
+        }
         const double pq = p + q;
         const double sipq = rsqrt_nr(pq);
         const double ipq = sipq * sipq;
@@ -526,7 +531,15 @@ struct JKArgs {
     // empty waves)
     const int2* tquart;
     long long ntquart;
+    // primitive FASTSCHWARZ screen (0: off): a primitive quartet is left out
+    // when (w_ab w_cd)^2 / sqrt(p q) < fs_min, i.e. fs_ab fs_cd < t^2 with
+    // fs = 24.739... w^2 p^{-1/2} (fs_min = t^2 / 24.739...^2)
+    double fs_min;
 };
+
+// Prefactor of the FASTSCHWARZ primitive-pair estimate, sqrt(2) pi^{5/2}
+// (integrals/two_electron/rys/rys_engine.ipp:71-72).
+constexpr double kFastSchwarzPrefactor = 24.739429451193148050228357586821944304008810699990039583;
 
 struct StoreArgs {
     PairData pd;
@@ -629,7 +642,8 @@ __device__ __forceinline__ void eri_jk_task(const JKArgs& a, const int wave) {
     }
 
     R acc[S::NOUT];
-    quartet<R, LA, LB, LC, LD, AC, AN, CC, CN>(a.tb, a.pd, P, Q, ib0, active ? ib1 : ib0, sub, ibs, true, acc);
+    quartet<R, LA, LB, LC, LD, AC, AN, CC, CN>(a.tb, a.pd, P, Q, ib0, active ? ib1 : ib0, sub, ibs, true, acc,
+                                               a.fs_min);
     if (ibs > 1) {
         // the ibs lanes of a ket (lane bits above ks) hold parts of one block:
         // sum them into the first, which alone digests

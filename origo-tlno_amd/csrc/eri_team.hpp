@@ -158,7 +158,11 @@ __device__ __forceinline__ void eri_team_quartet(const JKArgs& a, double* __rest
             const double ipq = sipq * sipq;
             const double PQx = Pp.x - Qp.x, PQy = Pp.y - Qp.y, PQz = Pp.z - Qp.z;
             const double Tv = p * q * ipq * (PQx * PQx + PQy * PQy + PQz * PQz);
-            const double pref = kTwoPi52 * pd.pw[pb0 + ib] * pd.pw[pk0 + ik] * sipq;
+            double pref = kTwoPi52 * pd.pw[pb0 + ib] * pd.pw[pk0 + ik] * sipq;
+            if (a.fs_min > 0.0) {  // primitive FASTSCHWARZ screen: the combination adds zero
+                const double w2 = pd.pw[pb0 + ib] * pd.pw[pk0 + ik];
+                if (w2 * w2 * rsqrt_nr(p * q) < a.fs_min) pref = 0.0;
+            }
             const RootEval<T::NR> re(a.tb, Tv);
             double t2, wr;
             re.get(a.tb, r, t2, wr);

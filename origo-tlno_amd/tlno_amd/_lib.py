@@ -37,6 +37,7 @@ SIGNATURES = {
     "tlno_gpu_set_team_threshold": (C.c_int, [C.c_void_p, C.c_int]),
     "tlno_gpu_plan_autotune": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_char_p, C.c_int]),
     "tlno_gpu_set_density_threshold": (C.c_int, [C.c_void_p, C.c_double]),
+    "tlno_gpu_set_primitive_screens": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
     "tlno_gpu_last_mixed_stats": (C.c_int, [C.c_void_p, LL, LL, D]),
     "tlno_gpu_class_stats": (C.c_int, [C.c_void_p, LL, LL, D, LL]),
     "tlno_gpu_eri_cart_blocks": (C.c_int, [C.c_void_p, C.c_int, I, LL, D]),

@@ -180,3 +180,21 @@ def water_cluster(nwat, seed=SEED0 + 4):
     """water40-style cluster of any size at water40's number density."""
     box = 11.0 * (nwat / 40.0) ** (1.0 / 3.0)
     return water40(seed=seed, nwat=nwat, box_A=box)
+
+
+def distinct_pairs():
+    """Three centres (He, H, H charges) whose five shells (s with 1
+    primitive, p with 2, s with 3, s with 4, d with 1) give every shell pair
+    its own (L_a, L_b, primitive-pair count) class: no two pairs share a
+    bra packet of a SIMD engine that groups pairs by that key, so a
+    per-primitive-quartet screen and a per-packet one select the same
+    primitive quartets.  Spherical density: symmetric N(0, 1) entries."""
+    from .basis import Basis
+    A = np.array([0.0, 0.0, 0.0])
+    B = np.array([0.0, 0.9, 1.3])
+    C = np.array([1.4, -0.3, 0.2])
+    shells = [(A, 0, [3.1], [1.0]), (A, 1, [5.0, 0.7], [0.4, 0.8]), (B, 0, [12.0, 2.1, 0.4], [0.2, 0.5, 0.6]),
+              (C, 0, [8.0, 1.1, 0.3, 0.09], [0.1, 0.3, 0.5, 0.4]), (C, 2, [1.9], [1.0])]
+    b = Basis.from_shells(shells, [2, 1, 1], [A, B, C], name="distinct_pairs")
+    X = np.random.default_rng(SEED0 + 7).standard_normal((b.nsph(), b.nsph()))
+    return b, X + X.T

@@ -12,10 +12,13 @@ Same names, argument meaning and error behaviour:
     ``RuntimeError("eri_ao_tensor_hgp: density ...")``);
   * thresholds must be finite and >= 0.
 
-``density_threshold`` (the reference's density-coupled screen) is supported.
-The primitive FASTSCHWARZ and decay screens the reference exposes are not
-built on the GPU; they raise NotImplementedError when nonzero instead of being
-silently ignored.
+``density_threshold`` (the reference's density-coupled screen) and the
+heuristic ``decay_threshold`` (shell-pair decay indicator) and
+``primitive_fastschwarz_threshold`` screens apply to the J/K builds.  The
+FASTSCHWARZ test is made per primitive quartet; the reference tests a SIMD
+packet of bra pairs together and so skips a subset of those quartets (the two
+agree where every packet holds one pair).  The stream checksum endpoint takes
+the Schwarz threshold only and raises NotImplementedError for the other two.
 """
 from __future__ import annotations
 
@@ -70,9 +73,6 @@ def _validate_options(o: ScreeningOptions):
     if o.density_threshold > 0.0 and o.decay_threshold > 0.0:
         raise RuntimeError("eri_ao_tensor_hgp: density screening requires Schwarz bounds and cannot be combined "
                            "with the heuristic decay metric.")
-    for name in ("primitive_fastschwarz_threshold", "decay_threshold"):
-        if getattr(o, name) != 0.0:
-            raise NotImplementedError(f"{name} screening is not available in the GPU engine")
 
 
 @dataclass
@@ -132,13 +132,19 @@ class DirectPlan:
         except Exception:
             pass
 
-    def jk(self, density, want_j=True, want_k=True, schwarz=None, density_threshold=None):
+    def jk(self, density, want_j=True, want_k=True, schwarz=None, density_threshold=None, fastschwarz=None,
+           decay=None):
         """Raw J and K (either may be skipped) on host arrays."""
         thr = self.options.integral_threshold if schwarz is None else schwarz
         _check_threshold("schwarz_threshold", thr)
         dthr = self.options.density_threshold if density_threshold is None else density_threshold
         _check_threshold("density_threshold", dthr)
+        fs = self.options.primitive_fastschwarz_threshold if fastschwarz is None else fastschwarz
+        dec = self.options.decay_threshold if decay is None else decay
+        _check_threshold("primitive_fastschwarz_threshold", fs)
+        _check_threshold("decay_threshold", dec)
         _lib.check(_lib.load().tlno_gpu_set_density_threshold(self._h, float(dthr)))
+        _lib.check(_lib.load().tlno_gpu_set_primitive_screens(self._h, float(fs), float(dec)))
         M = self.M
         D = np.ascontiguousarray(density, dtype=np.float64)
         if D.shape != (M, M):
@@ -300,21 +306,24 @@ def _opts(plan, options):
 
 def build_coulomb_j(plan: DirectPlan, density, options: ScreeningOptions | None = None) -> CoulombJResult:
     o = _opts(plan, options)
-    J, _ = plan.jk(density, True, False, o.integral_threshold, o.density_threshold)
+    J, _ = plan.jk(density, True, False, o.integral_threshold, o.density_threshold, o.primitive_fastschwarz_threshold,
+                   o.decay_threshold)
     return CoulombJResult(J, plan.M)
 
 
 def build_exchange_k(plan: DirectPlan, density, options: ScreeningOptions | None = None) -> ExchangeKResult:
     """K[a,c] = sum_(b,d) P[b,d] (ab|cd), for a real symmetric density."""
     o = _opts(plan, options)
-    _, K = plan.jk(density, False, True, o.integral_threshold, o.density_threshold)
+    _, K = plan.jk(density, False, True, o.integral_threshold, o.density_threshold, o.primitive_fastschwarz_threshold,
+                   o.decay_threshold)
     return ExchangeKResult(K, plan.M)
 
 
 def build_rhf_g(plan: DirectPlan, density, options: ScreeningOptions | None = None) -> DirectResult:
     """G = J - K/2 from one traversal."""
     o = _opts(plan, options)
-    J, K = plan.jk(density, True, True, o.integral_threshold, o.density_threshold)
+    J, K = plan.jk(density, True, True, o.integral_threshold, o.density_threshold, o.primitive_fastschwarz_threshold,
+                   o.decay_threshold)
     return DirectResult(J - 0.5 * K, plan.M)
 
 
@@ -413,6 +422,11 @@ class EriDirectJKIncrementalState:
             v = getattr(o, name)
             if not (math.isfinite(v) and v >= 0.0):
                 raise RuntimeError(f"eri_ao_tensor_hgp: incremental {name} must be finite and >= 0.")
+        if ((o.delta_density_threshold > 0.0 or (o.use_first_delta_density_threshold
+                                                  and o.first_delta_density_threshold > 0.0))
+                and b.decay_threshold > 0.0):
+            raise RuntimeError("eri_ao_tensor_hgp: incremental delta-density screening cannot be combined with "
+                               "the heuristic decay metric.")
         b.density_threshold = 0.0  # exact rebuilds never screen with the full density
         self.M = plan.M
         self.reset()

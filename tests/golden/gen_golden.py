@@ -35,6 +35,15 @@ tests) and the reference's outputs:
                    Cartesian and spherical S/T/V with screening on (its
                    default) and off, pair and nuclear primitive-pair
                    statistics (python gen_golden.py onee_capi rewrites only it)
+  screens.npz      the reference's heuristic screens (ScreeningOptions
+                   primitive_fastschwarz_threshold, decay_threshold): G of
+                   the 4-water cluster of density.npz (same density) with
+                   the decay indicator at Schwarz 1e-12 (decay 1e-6, 1e-4)
+                   and alone (1e-8, 1e-5), and with FASTSCHWARZ 1e-9, 1e-7
+                   at Schwarz 1e-12; G of configs.distinct_pairs() (every
+                   shell pair its own SIMD bin) with FASTSCHWARZ 1e-6, 1e-4,
+                   1e-2 at Schwarz 0, and unscreened (python gen_golden.py
+                   screens rewrites only it)
   density.npz      4-water cluster (96 spherical functions), density
                    configs.spherical_density(96, 9, 20): the reference's
                    density-screened J, K and G (density_threshold 1e-9, 1e-7,
@@ -76,6 +85,8 @@ def main(only=None):
         return checksums(ref)
     if only == "onee_capi":
         return onee_capi()
+    if only == "screens":
+        return screens(ref)
     t0 = time.time()
     # ---------------------------------------------------------- Boys / Rys
     xs = np.concatenate([[0.0, 1e-10, 1e-3, 0.5, 5.0, 11.99, 12.0, 12.5, 25.0, 39.99, 40.0, 45.0, 80.0, 200.0],
@@ -140,6 +151,7 @@ def main(only=None):
     density(ref)
     checksums(ref)
     onee_capi()
+    screens(ref)
     print(f"golden fixtures written in {time.time() - t0:.1f} s")
 
 
@@ -202,6 +214,24 @@ def onee_capi():
                         f"{tag}_nuclear_pair_stats": np.array([nt.value, nk.value])})
     L.tlno_onee_set_screening(1)
     np.savez_compressed(os.path.join(HERE, "onee_capi.npz"), **out)
+
+
+def screens(ref):
+    # ---------------------------------------------------------- decay / FASTSCHWARZ
+    b, _ = configs.water_cluster(4)
+    M = ref.plan_M(b)
+    Ds = sph_density(M, 9, 20)
+    out = {"water4_sha": basis_hash(b)}
+    for sch, dec, tag in ((1e-12, 1e-6, "s12_d6"), (1e-12, 1e-4, "s12_d4"), (0.0, 1e-8, "d8"), (0.0, 1e-5, "d5")):
+        out[f"water4_G_{tag}"] = ref.direct(b, Ds, "G", schwarz=sch, decay=dec)[0]
+    for fs, tag in ((1e-9, "9"), (1e-7, "7")):
+        out[f"water4_G_fs{tag}"] = ref.direct(b, Ds, "G", schwarz=1e-12, fastschwarz=fs)[0]
+    b, Ds = configs.distinct_pairs()
+    out["distinct_sha"] = basis_hash(b)
+    out["distinct_G_0"] = ref.direct(b, Ds, "G", schwarz=0.0)[0]
+    for fs, tag in ((1e-6, "6"), (1e-4, "4"), (1e-2, "2")):
+        out[f"distinct_G_fs{tag}"] = ref.direct(b, Ds, "G", schwarz=0.0, fastschwarz=fs)[0]
+    np.savez_compressed(os.path.join(HERE, "screens.npz"), **out)
 
 
 def checksums(ref):
