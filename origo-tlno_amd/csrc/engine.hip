@@ -730,7 +730,8 @@ void launch_jk(const ClassKernels& k, bool fp32, JKArgs& args, long long nwaves,
     }
     const unsigned grid = grid_for(gw * 64, 256);
     for (int g = 0; g < k.ngroup; ++g) {
-        const JKKernel kern = fp32 ? k.grp[g].jk32 : k.grp[g].jk;
+        const bool fs = args.fs_min > 0.0;
+        const JKKernel kern = fp32 ? (fs ? k.grp[g].jk32fs : k.grp[g].jk32) : (fs ? k.grp[g].jkfs : k.grp[g].jk);
         HIP_OK(hipLaunchKernel(reinterpret_cast<const void*>(kern), dim3(grid, k.grp[g].n), dim3(256), params, 0,
                                st));
     }

@@ -349,7 +349,9 @@ __host__ __device__ constexpr bool flat_prims(int LA, int LB, int LC, int LD) {
 // flat_prims classes, else of the bra pairs (with every ket pair).
 // uniform: P, ib0 and ib1 are the same in every lane of the wave (lane 0
 // holds the range when some lanes pass an empty one).
-template <typename R, int LA, int LB, int LC, int LD, int AC, int AN, int CC, int CN>
+// FS: the primitive FASTSCHWARZ screen is on (fs_min > 0); a separate
+// instance, so builds without the screen run the unscreened code unchanged.
+template <typename R, int LA, int LB, int LC, int LD, int AC, int AN, int CC, int CN, bool FS = false>
 __device__ __forceinline__ void quartet(const DevTables& tb, const PairData& pd, int P, int Q, int ib0, int ib1,
                                         int j0, int js, bool uniform,
                                         R (&acc)[Shape<LA, LB, LC, LD, AC, AN, CC, CN>::NOUT],
@@ -374,16 +376,21 @@ __device__ __forceinline__ void quartet(const DevTables& tb, const PairData& pd,
     auto prim = [&](const double4 Pp, const double wab, const double4 Qp, const double wcd) {
         const double q = Qp.w;
         const double p = Pp.w;
-        if (fs_min > 0.0) {  // primitive FASTSCHWARZ screen (this lane's quartet only)
-            const double w2 = wab * wcd;
-            if (w2 * w2 * rsqrt_nr(p * q) < fs_min) return;
+        // primitive FASTSCHWARZ screen (this lane's quartet only): a failing
+        // quartet gets a zero prefactor (a mask, not a branch: the lanes of a
+        // wave run the same primitive code);
+        // (w_ab w_cd)^2 / sqrt(pq) >= fs_min  <=>  (w_ab w_cd)^4 >= fs_min^2 p q
+        double keep = 1.0;
+        if constexpr (FS) {
+            const double w2 = wab * wcd, w4 = (w2 * w2) * (w2 * w2);
+            keep = w4 < (fs_min * fs_min) * (p * q) ? 0.0 : 1.0;
         }
         const double pq = p + q;
         const double sipq = rsqrt_nr(pq);
         const double ipq = sipq * sipq;
         const double PQx = Pp.x - Qp.x, PQy = Pp.y - Qp.y, PQz = Pp.z - Qp.z;
         const double T = p * q * ipq * (PQx * PQx + PQy * PQy + PQz * PQz);
-        const double pref = kTwoPi52 * wab * wcd * sipq;
+        const double pref = FS ? kTwoPi52 * wab * wcd * sipq * keep : kTwoPi52 * wab * wcd * sipq;
         double rt[S::NR], wt[S::NR];
         rys_roots<S::NR>(tb, T, rt, wt);
         const R qo = R(q * ipq), po = R(p * ipq);
@@ -533,7 +540,8 @@ struct JKArgs {
     long long ntquart;
     // primitive FASTSCHWARZ screen (0: off): a primitive quartet is left out
     // when (w_ab w_cd)^2 / sqrt(p q) < fs_min, i.e. fs_ab fs_cd < t^2 with
-    // fs = 24.739... w^2 p^{-1/2} (fs_min = t^2 / 24.739...^2)
+    // fs = 24.739... w^2 p^{-1/2} (fs_min = t^2 / 24.739...^2); evaluated as
+    // (w_ab w_cd)^4 < fs_min^2 p q
     double fs_min;
 };
 
@@ -569,7 +577,7 @@ __device__ __forceinline__ int ket_stride(int nq) {
 // ib0 | ib1 << 16} (range mode) or a list-mode task (see JKArgs).  Lane l
 // takes ket l % ks of the task (ks = ket_stride(kets)) and share l / ks of
 // the 64 / ks shares of its primitive quartets (see quartet).
-template <typename R, int LA, int LB, int LC, int LD, int AC, int AN, int CC, int CN>
+template <typename R, int LA, int LB, int LC, int LD, int AC, int AN, int CC, int CN, bool FS = false>
 __device__ __forceinline__ void eri_jk_task(const JKArgs& a, const int wave) {
     using S = Shape<LA, LB, LC, LD, AC, AN, CC, CN>;
     const int lane = threadIdx.x & 63;
@@ -642,8 +650,8 @@ __device__ __forceinline__ void eri_jk_task(const JKArgs& a, const int wave) {
     }
 
     R acc[S::NOUT];
-    quartet<R, LA, LB, LC, LD, AC, AN, CC, CN>(a.tb, a.pd, P, Q, ib0, active ? ib1 : ib0, sub, ibs, true, acc,
-                                               a.fs_min);
+    quartet<R, LA, LB, LC, LD, AC, AN, CC, CN, FS>(a.tb, a.pd, P, Q, ib0, active ? ib1 : ib0, sub, ibs, true, acc,
+                                                   a.fs_min);
     if (ibs > 1) {
         // the ibs lanes of a ket (lane bits above ks) hold parts of one block:
         // sum them into the first, which alone digests
@@ -802,12 +810,14 @@ __device__ __forceinline__ void eri_jk_task(const JKArgs& a, const int wave) {
 }
 
 // Grid entry: one work item per wave, or a persistent loop over work items.
-template <typename R, int LA, int LB, int LC, int LD, int AC, int AN, int CC, int CN>
+// FS: the instance with the primitive FASTSCHWARZ screen (a kernel of its
+// own, so the unscreened kernels keep their registers and occupancy).
+template <typename R, int LA, int LB, int LC, int LD, int AC, int AN, int CC, int CN, bool FS = false>
 __device__ __forceinline__ void eri_jk_body(const JKArgs& a) {
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int end = a.stride > 0 ? a.nwork : wave + 1;
     const int step = a.stride > 0 ? a.stride : 1;
-    for (int w = wave; w < end; w += step) eri_jk_task<R, LA, LB, LC, LD, AC, AN, CC, CN>(a, w);
+    for (int w = wave; w < end; w += step) eri_jk_task<R, LA, LB, LC, LD, AC, AN, CC, CN, FS>(a, w);
 }
 
 // Writes the lane's slice of the unit-diagonal Cartesian block of (P|Q) into
@@ -844,6 +854,8 @@ constexpr int kMaxChunkGroups = 36;
 struct JKGroup {
     JKKernel jk = nullptr;    // FP64 recurrences
     JKKernel jk32 = nullptr;  // FP32 recurrences (mixed-precision route)
+    JKKernel jkfs = nullptr;  // the same two with the primitive FASTSCHWARZ screen
+    JKKernel jk32fs = nullptr;
     int c0 = 0, n = 0;        // chunks [c0, c0 + n)
 };
 struct ClassKernels {

@@ -55,11 +55,13 @@ struct ChunkPolicy {
     static constexpr int cc(int i) { return mode == 2 ? (i % ncg) * cn : -1; }
 };
 
-template <typename R, int LA, int LB, int LC, int LD, int C0, int... J>
+template <typename R, int LA, int LB, int LC, int LD, bool FS, int C0, int... J>
 __global__ __launch_bounds__(256) void k_eri_jk(JKArgs a) {
     using CP = ChunkPolicy<LA, LB, LC, LD>;
     const int ch = C0 + static_cast<int>(blockIdx.y);
-    ((ch == C0 + J ? (eri_jk_body<R, LA, LB, LC, LD, CP::ac(C0 + J), CP::an(C0 + J), CP::cc(C0 + J), CP::cn>(a), 0) : 0), ...);
+    ((ch == C0 + J ? (eri_jk_body<R, LA, LB, LC, LD, CP::ac(C0 + J), CP::an(C0 + J), CP::cc(C0 + J), CP::cn, FS>(a), 0)
+                   : 0),
+     ...);
 }
 
 template <int LA, int LB, int LC, int LD, int... I>
@@ -77,8 +79,10 @@ void reg_store(ClassKernels& k, std::integer_sequence<int, I...>) {
 template <int LA, int LB, int LC, int LD, int C0, int... J>
 void reg_group(ClassKernels& k, std::integer_sequence<int, J...>) {
     JKGroup& g = k.grp[k.ngroup++];
-    g.jk = k_eri_jk<double, LA, LB, LC, LD, C0, J...>;
-    g.jk32 = k_eri_jk<float, LA, LB, LC, LD, C0, J...>;
+    g.jk = k_eri_jk<double, LA, LB, LC, LD, false, C0, J...>;
+    g.jk32 = k_eri_jk<float, LA, LB, LC, LD, false, C0, J...>;
+    g.jkfs = k_eri_jk<double, LA, LB, LC, LD, true, C0, J...>;
+    g.jk32fs = k_eri_jk<float, LA, LB, LC, LD, true, C0, J...>;
     g.c0 = C0;
     g.n = static_cast<int>(sizeof...(J));
 }

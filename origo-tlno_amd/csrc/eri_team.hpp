@@ -160,8 +160,8 @@ __device__ __forceinline__ void eri_team_quartet(const JKArgs& a, double* __rest
             const double Tv = p * q * ipq * (PQx * PQx + PQy * PQy + PQz * PQz);
             double pref = kTwoPi52 * pd.pw[pb0 + ib] * pd.pw[pk0 + ik] * sipq;
             if (a.fs_min > 0.0) {  // primitive FASTSCHWARZ screen: the combination adds zero
-                const double w2 = pd.pw[pb0 + ib] * pd.pw[pk0 + ik];
-                if (w2 * w2 * rsqrt_nr(p * q) < a.fs_min) pref = 0.0;
+                const double w2 = pd.pw[pb0 + ib] * pd.pw[pk0 + ik], w4 = (w2 * w2) * (w2 * w2);
+                if (w4 < (a.fs_min * a.fs_min) * (p * q)) pref = 0.0;
             }
             const RootEval<T::NR> re(a.tb, Tv);
             double t2, wr;
