@@ -329,6 +329,7 @@ struct TaskSet {
     int4* d_rows = nullptr;
     long long* d_rbase = nullptr;
     long long* d_rtask = nullptr;  // first task slot of each row
+    int* d_rcnt = nullptr;         // learned builds: density survivors per row
     long long nrows = 0;
     long long qtotal = 0;
     long long toff[36] = {}, tcap[36] = {};
@@ -408,6 +409,8 @@ struct tlno_gpu_plan {
     float mb3 = 0.0f;
     int* d_qlist = nullptr;
     long long qlist_cap = 0;
+    int* d_cand = nullptr;  // learned builds: density survivors, laid out like qlist
+    long long cand_cap = 0;
     int4 *d_t64 = nullptr, *d_t32 = nullptr;
     long long tlist_cap = 0;
     unsigned long long* d_rstats = nullptr;  // [kCounterSlots][4]
@@ -882,6 +885,9 @@ struct RouteArgs {
     const long long* rtask;       // first task slot of each row (in t64 and in t32)
     const double* decay;          // per pair decay indicator (nullptr: no decay test)
     double decay_thr;
+    unsigned long long fp32_cls;  // bit 6 X + Y: class pair whose FP32 share runs the FP32 kernels
+    int* cand;                    // learned builds: density survivors of each row (at its base)
+    int* rcnt;                    // and their number per row
     unsigned long long* stats;    // [kCounterSlots][4]: candidates, density-kept, kept, FP32
 };
 
@@ -890,9 +896,34 @@ __device__ inline int4 list_task(int P, long long off, int cnt) {
                      cnt | static_cast<int>((off >> 32) << 8), 0);
 }
 
-// LEARNED = false leaves the network out of the kernel (fewer registers,
-// more waves in flight for the density test's gathers)
-template <bool LEARNED>
+// A row's list tasks in its fixed slots: n64 FP64 kets from the front of its
+// qlist region, n32 FP32 kets from the back, 64 per task; unused slots get
+// empty tasks (they leave at once).
+__device__ inline void route_tasks(const RouteArgs& a, long long row, int P, long long base, long long len,
+                                   int n64, int n32, bool cls32) {
+    const int lane = threadIdx.x & 63;
+    const long long slot = a.rtask[row];
+    const int ncap = static_cast<int>((len + 63) >> 6);
+    for (int i = lane; i < ncap; i += 64) {
+        const int c64 = max(0, min(64, n64 - 64 * i));
+        a.t64[slot + i] = list_task(P, base + 64ll * i, c64);
+        if (cls32) {
+            const int c32 = max(0, min(64, n32 - 64 * i));
+            a.t32[slot + i] = list_task(P, base + len - 64ll * i - c32, c32);
+        }
+    }
+}
+
+// Per-row density (and decay) test.  One wave per row of candidate kets.
+// CAND = false: the final routing of builds without the learned screen --
+// kept kets compacted into the row's qlist region (FP64 from the front, FP32
+// from the back), list tasks into the row's fixed task slots.
+// CAND = true: the first pass of a learned build -- kept kets compacted into
+// the row's candidate region (a.cand at the row's base), their number into
+// a.rcnt[row]; k_route_net runs the network on them.  This pass holds no
+// network registers, so it runs at full occupancy, which its dependent
+// gathers (Q, pair groups, shell-block maxima) need.
+template <bool CAND>
 __global__ __launch_bounds__(256) void k_route(RouteArgs a) {
     const long long wave = (static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
@@ -902,7 +933,7 @@ __global__ __launch_bounds__(256) void k_route(RouteArgs a) {
     const int k0 = __builtin_amdgcn_readfirstlane(r.y);
     const int kend = __builtin_amdgcn_readfirstlane(r.z);
     const int xy = __builtin_amdgcn_readfirstlane(r.w);
-    const int X = xy / 6, Y = xy - 6 * (xy / 6);
+    const int Y = xy - 6 * (xy / 6);
     int qe = kend;
     if (a.cut) qe = min(qe, a.cut[6 * P + Y]);
     const long long base = a.rbase[wave];
@@ -911,8 +942,11 @@ __global__ __launch_bounds__(256) void k_route(RouteArgs a) {
     const int ga = a.pgA[P], gb = a.pgB[P];
     const long long ng = a.ng;
     const double rhoP = a.blk ? a.blk[ga * ng + gb] : 0.0;
-    const double sigP = a.blk ? a.bsum[ga * ng + gb] : 0.0;
     const double maxD = a.gate > 0.0 ? *a.maxD : 0.0;
+    // classes whose FP32 share would run FP64 kernels anyway (team evaluator,
+    // or FP32 kernels measured slower) keep every quartet in the FP64 list:
+    // one list fills its tasks better than two
+    const bool cls32 = !CAND && a.gate > 0.0 && ((a.fp32_cls >> xy) & 1ull);
     const unsigned long long below = (1ull << lane) - 1ull;
     int n64 = 0, n32 = 0;
     unsigned long long cand = 0, dkept = 0;
@@ -922,8 +956,8 @@ __global__ __launch_bounds__(256) void k_route(RouteArgs a) {
         const int Qi = valid ? Qr : qe - 1;
         const double qq = QP * a.Q[Qi];
         bool keep = valid;
-        const long long gc = a.pgA[Qi], gd = a.pgB[Qi];
         if (a.blk) {
+            const long long gc = a.pgA[Qi], gd = a.pgB[Qi];
             const double coul = 4.0 * fmax(rhoP, a.blk[gc * ng + gd]);
             const double exch = fmax(fmax(a.blk[gc * ng + ga], a.blk[gd * ng + ga]),
                                      fmax(a.blk[gc * ng + gb], a.blk[gd * ng + gb]));
@@ -933,37 +967,18 @@ __global__ __launch_bounds__(256) void k_route(RouteArgs a) {
         if (a.decay) keep = keep && a.decay[P] * a.decay[Qi] >= a.decay_thr;
         cand += valid ? 1 : 0;
         dkept += keep ? 1 : 0;
-        double est = qq;
-        if (LEARNED && __ballot(keep) != 0ull) {  // wave-uniform: every lane runs the network
-            float x[MLP_IN];
-            quartet_features(a.fa, P, Qi, X, Y, x);
-            const float y = mlp_forward_wave(a.mlp, x);
-            est = qq * fmin(1.0, exp(2.302585092994046 * (static_cast<double>(y) + a.margin)));
-            if (a.blk) {
-                const double coul = 4.0 * fmax(sigP, a.bsum[gc * ng + gd]);
-                const double exch = fmax(fmax(a.bsum[gc * ng + ga], a.bsum[gd * ng + ga]),
-                                         fmax(a.bsum[gc * ng + gb], a.bsum[gd * ng + gb]));
-                const double fac = a.mode == 1 ? coul : (a.mode == 2 ? exch : fmax(coul, exch));
-                keep = keep && est * fac >= a.tau;
-            }
-            keep = keep && est >= a.thr;
-        }
-        const bool to32 = keep && a.gate > 0.0 && est * maxD < a.gate;
-        const bool to64 = keep && !to32;
-        const unsigned long long m64 = __ballot(to64), m32 = __ballot(to32);
-        if (to64) a.qlist[base + n64 + __popcll(m64 & below)] = Qr;
-        if (to32) a.qlist[base + len - 1 - (n32 + __popcll(m32 & below))] = Qr;
-        n64 += __popcll(m64);
-        n32 += __popcll(m32);
-    }
-    const long long slot = a.rtask[wave];
-    const int ncap = static_cast<int>((len + 63) >> 6);
-    for (int i = lane; i < ncap; i += 64) {
-        const int c64 = max(0, min(64, n64 - 64 * i));
-        a.t64[slot + i] = list_task(P, base + 64ll * i, c64);
-        if (a.gate > 0.0) {
-            const int c32 = max(0, min(64, n32 - 64 * i));
-            a.t32[slot + i] = list_task(P, base + len - 64ll * i - c32, c32);
+        if constexpr (CAND) {
+            const unsigned long long m = __ballot(keep);
+            if (keep) a.cand[base + n64 + __popcll(m & below)] = Qr;
+            n64 += __popcll(m);
+        } else {
+            const bool to32 = keep && cls32 && qq * maxD < a.gate;
+            const bool to64 = keep && !to32;
+            const unsigned long long m64 = __ballot(to64), m32 = __ballot(to32);
+            if (to64) a.qlist[base + n64 + __popcll(m64 & below)] = Qr;
+            if (to32) a.qlist[base + len - 1 - (n32 + __popcll(m32 & below))] = Qr;
+            n64 += __popcll(m64);
+            n32 += __popcll(m32);
         }
     }
 #pragma unroll
@@ -971,10 +986,88 @@ __global__ __launch_bounds__(256) void k_route(RouteArgs a) {
         cand += __shfl_xor(cand, m, 64);
         dkept += __shfl_xor(dkept, m, 64);
     }
+    unsigned long long* st = a.stats + 4 * (wave % kCounterSlots);
+    if constexpr (CAND) {
+        if (lane == 0) {
+            a.rcnt[wave] = n64;
+            atomicAdd(st + 0, cand);
+            atomicAdd(st + 1, dkept);
+        }
+    } else {
+        route_tasks(a, wave, P, base, len, n64, n32, cls32);
+        if (lane == 0) {
+            atomicAdd(st + 0, cand);
+            atomicAdd(st + 1, dkept);
+            atomicAdd(st + 2, static_cast<unsigned long long>(n64 + n32));
+            atomicAdd(st + 3, static_cast<unsigned long long>(n32));
+        }
+    }
+}
+
+// Second pass of a learned build: one wave per row, over the row's density
+// survivors (a.cand) in dense chunks of 64.  Each chunk's 32 features go
+// through the network on the matrix cores; est = Q_ab Q_cd min(1,
+// 10^(y + margin)) must reach the Schwarz threshold and pass the density
+// test again with shell-block sums of |D| in place of maxima (est estimates
+// one integral; a block adds up to the sum); FP32 iff est max|D| < gate.
+// Survivors compacted into the row's qlist region, list tasks into its slots.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_route_net(RouteArgs a) {
+    // the network's packed weights, read by every chunk, sit in LDS (13 KB per
+    // block) instead of being fetched from the cache hierarchy per chunk
+    __shared__ uint4 sfrag[MLP_NFRAG * 64];
+    __shared__ float svec[192];
+    for (int i = threadIdx.x; i < MLP_NFRAG * 64; i += blockDim.x) sfrag[i] = a.mlp.frag[i];
+    for (int i = threadIdx.x; i < 192; i += blockDim.x) svec[i] = a.mlp.vec[i];
+    __syncthreads();
+    const MlpPacked mlp{sfrag, svec, a.mlp.norm, a.mlp.b3};
+    const long long wave = (static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (wave >= a.nrows) return;  // whole wave leaves together
+    const int4 r = a.rows[wave];
+    const int P = __builtin_amdgcn_readfirstlane(r.x);
+    const int xy = __builtin_amdgcn_readfirstlane(r.w);
+    const int X = xy / 6, Y = xy - 6 * (xy / 6);
+    const long long base = a.rbase[wave];
+    const long long len = r.z - r.y;
+    const int n = __builtin_amdgcn_readfirstlane(a.rcnt[wave]);
+    const double QP = a.Q[P];
+    const int ga = a.pgA[P], gb = a.pgB[P];
+    const long long ng = a.ng;
+    const double sigP = a.blk ? a.bsum[ga * ng + gb] : 0.0;
+    const double maxD = a.gate > 0.0 ? *a.maxD : 0.0;
+    const bool cls32 = a.gate > 0.0 && ((a.fp32_cls >> xy) & 1ull);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int n64 = 0, n32 = 0;
+    for (int c0 = 0; c0 < n; c0 += 64) {
+        const bool valid = c0 + lane < n;
+        const int Qr = a.cand[base + (valid ? c0 + lane : c0)];  // invalid lanes run a stand-in quartet
+        const double qq = QP * a.Q[Qr];
+        float x[MLP_IN];
+        quartet_features(a.fa, P, Qr, X, Y, x);
+        const float y = mlp_forward_wave(mlp, x);  // every lane of the wave
+        // 10^(y + margin) in single precision (the screen's estimate, not a
+        // result: single-precision rounding of the factor is immaterial)
+        const double est = qq * fminf(1.0f, exp2f(3.3219280948873623f * (y + static_cast<float>(a.margin))));
+        bool keep = valid && est >= a.thr;
+        if (a.blk) {
+            const long long gc = a.pgA[Qr], gd = a.pgB[Qr];
+            const double coul = 4.0 * fmax(sigP, a.bsum[gc * ng + gd]);
+            const double exch = fmax(fmax(a.bsum[gc * ng + ga], a.bsum[gd * ng + ga]),
+                                     fmax(a.bsum[gc * ng + gb], a.bsum[gd * ng + gb]));
+            const double fac = a.mode == 1 ? coul : (a.mode == 2 ? exch : fmax(coul, exch));
+            keep = keep && est * fac >= a.tau;
+        }
+        const bool to32 = keep && cls32 && est * maxD < a.gate;
+        const bool to64 = keep && !to32;
+        const unsigned long long m64 = __ballot(to64), m32 = __ballot(to32);
+        if (to64) a.qlist[base + n64 + __popcll(m64 & below)] = Qr;
+        if (to32) a.qlist[base + len - 1 - (n32 + __popcll(m32 & below))] = Qr;
+        n64 += __popcll(m64);
+        n32 += __popcll(m32);
+    }
+    route_tasks(a, wave, P, base, len, n64, n32, cls32);
     if (lane == 0) {
         unsigned long long* st = a.stats + 4 * (wave % kCounterSlots);
-        atomicAdd(st + 0, cand);
-        atomicAdd(st + 1, dkept);
         atomicAdd(st + 2, static_cast<unsigned long long>(n64 + n32));
         atomicAdd(st + 3, static_cast<unsigned long long>(n32));
     }
@@ -1715,6 +1808,23 @@ void ensure_route(tlno_gpu_plan* pl, TaskSet& ts, double thr) {
     }
 }
 
+// Buffers of the learned build's two-pass routing: the density survivors of
+// every row (same layout as qlist) and their count per row.
+void ensure_route_net(tlno_gpu_plan* pl, TaskSet& ts) {
+    if (!ts.d_rcnt) {
+        require_alloc("learned-screen routing buffers");
+        HIP_OK(hipMalloc(&ts.d_rcnt, std::max<long long>(1, ts.nrows) * sizeof(int)));
+    }
+    if (ts.qtotal > pl->cand_cap) {
+        require_alloc("learned-screen routing buffers");
+        if (pl->d_cand) HIP_OK(hipFree(pl->d_cand));
+        pl->d_cand = nullptr;
+        pl->cand_cap = 0;
+        HIP_OK(hipMalloc(&pl->d_cand, ts.qtotal * sizeof(int)));
+        pl->cand_cap = ts.qtotal;
+    }
+}
+
 // bra primitive slices per list-mode task so that a small launch still
 // spreads over the machine (same policy as the range-mode task split)
 int list_split(long long ntask, int nchunk) {
@@ -1858,6 +1968,12 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
         ra.margin = pl->learned_margin;
         ra.decay = pl->decay_thr > 0.0 ? pl->d_decay : nullptr;
         ra.decay_thr = pl->decay_thr;
+        ra.fp32_cls = 0;
+        if (pl->fp32_gate > 0.0)
+            for (int X = 0; X < 6; ++X)
+                for (int Y = 0; Y <= X; ++Y)
+                    if (!pl->fp32_slower[X][Y] && !use_team(g_kernels[X][Y], false, a, class_team_min(pl, X, Y)))
+                        ra.fp32_cls |= 1ull << (6 * X + Y);
         ra.gate = pl->fp32_gate;
         ra.maxD = bb.scalar;
         ra.qlist = pl->d_qlist;
@@ -1865,16 +1981,26 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
         ra.t32 = pl->d_t32;
         ra.rtask = ts.d_rtask;
         ra.stats = pl->d_rstats;
+        ra.cand = nullptr;
+        ra.rcnt = nullptr;
         if (ts.nrows > 0) {
-            if (pl->learned) k_route<true><<<grid_for(ts.nrows * 64, 256), 256, 0, st>>>(ra);
-            else k_route<false><<<grid_for(ts.nrows * 64, 256), 256, 0, st>>>(ra);
+            const unsigned grid = grid_for(ts.nrows * 64, 256);
+            if (pl->learned) {
+                ensure_route_net(pl, ts);
+                ra.cand = pl->d_cand;
+                ra.rcnt = ts.d_rcnt;
+                k_route<true><<<grid, 256, 0, st>>>(ra);
+                k_route_net<<<grid, 256, 0, st>>>(ra);
+            } else {
+                k_route<false><<<grid, 256, 0, st>>>(ra);
+            }
         }
         HIP_OK(hipGetLastError());
         // every row's tasks sit in its fixed slots: each class launch covers
         // the class's slot range (empty tasks leave at once)
         for (int xy = 0; xy < 36; ++xy) {
             tcount[xy] = static_cast<unsigned>(ts.tcap[xy]);
-            tcount[36 + xy] = pl->fp32_gate > 0.0 ? static_cast<unsigned>(ts.tcap[xy]) : 0u;
+            tcount[36 + xy] = ((ra.fp32_cls >> xy) & 1ull) ? static_cast<unsigned>(ts.tcap[xy]) : 0u;
         }
     }
     if (a.do_j || a.do_k) {
@@ -2452,7 +2578,7 @@ tlno_gpu_plan::~tlno_gpu_plan() {
     f(d_row_group); f(d_pgA); f(d_pgB); f(d_pcls); f(d_clsb); f(d_Q); f(d_gmax); f(d_clsmax); f(d_pairfac);
     f(d_cut); f(d_counter);
     f(d_pf0); f(d_pf1); f(d_pcen); f(d_grows); f(d_blk); f(d_bsum); f(d_frag); f(d_mvec); f(d_mnorm);
-    f(d_qlist); f(d_t64); f(d_t32); f(d_rstats);
+    f(d_qlist); f(d_t64); f(d_t32); f(d_rstats); f(d_cand);
     f(d_onee_list); f(d_onee_X); f(d_onee_O); f(d_onee_nuc); f(d_decay);
     for (auto& e : onee_ev)
         if (e) (void)hipEventDestroy(e);
@@ -2463,6 +2589,7 @@ tlno_gpu_plan::~tlno_gpu_plan() {
         f(kv.second.d_rows);
         f(kv.second.d_rbase);
         f(kv.second.d_rtask);
+        f(kv.second.d_rcnt);
         for (int X = 0; X < 6; ++X)
             for (int Y = 0; Y < 6; ++Y) f(kv.second.d_tq[X][Y]);
     }

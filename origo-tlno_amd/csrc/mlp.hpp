@@ -151,7 +151,17 @@ __device__ inline mlp_bf16x8 mlp_as_bf16x8(uint4 v) {
     return r;
 }
 
-__device__ inline uint32_t mlp_pack2(float lo, float hi) { return bf16_bits(lo) | (bf16_bits(hi) << 16); }
+typedef __bf16 mlp_bf16x2 __attribute__((ext_vector_type(2)));
+typedef float mlp_f32x2 __attribute__((ext_vector_type(2)));
+
+// Two values rounded to bf16 (nearest even, as bf16_round for finite values)
+// and packed low / high: one v_cvt_pk_bf16_f32 on gfx950.
+__device__ inline uint32_t mlp_pack2(float lo, float hi) {
+    const mlp_bf16x2 b = __builtin_convertvector(mlp_f32x2{lo, hi}, mlp_bf16x2);
+    uint32_t u;
+    __builtin_memcpy(&u, &b, 4);
+    return u;
+}
 
 __device__ inline uint4 mlp_pack8(const float* v) {
     return make_uint4(mlp_pack2(v[0], v[1]), mlp_pack2(v[2], v[3]), mlp_pack2(v[4], v[5]), mlp_pack2(v[6], v[7]));
