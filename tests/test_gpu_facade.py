@@ -68,7 +68,11 @@ def test_facade_matches_reference_and_mirror(name, tmp_path):
     from tlno_amd import onee
     T_mirror = onee.build_one_electron_matrices(b).kinetic
     assert np.abs(res["T_cart"] - T_mirror.ravel()).max() < 1e-12
+    # against the reference only where its one-sided form keeps its digits;
+    # against the oracle (pinned to 40-digit values) everywhere
     assert np.abs(res["T_cart"] - g["T0"].ravel()).max() < (1e-10 if name == "water7" else 1e-6)
+    _, To, _ = O.OracleBasis(b).onee(b.nuclear_charges, b.nuclear_centers)
+    assert np.all(np.abs(res["T_cart"] - To.ravel()) <= 1e-14 + 1e-12 * np.abs(To.ravel()))
     assert np.abs(res["V_cart"] - g["V0"].ravel()).max() < 1e-10
     assert np.abs(res["S_sph"] - g["S1"].ravel()).max() < 1e-10
     assert np.abs(res["V_sph"] - g["V1"].ravel()).max() < 1e-10
