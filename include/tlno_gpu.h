@@ -147,6 +147,18 @@ int tlno_gpu_set_primitive_screens(tlno_gpu_plan* plan, double primitive_fastsch
  * rys_engine.ipp:17096-17103).  Spherical plans only. */
 int tlno_gpu_eri_checksum(tlno_gpu_plan* plan, double schwarz, double* checksum);
 
+/* Indexed stream fingerprint of the same canonical values, as the
+ * reference's TLNO_VALIDATE_RYS_STREAM_FINGERPRINT build accumulates it
+ * (rys_engine.ipp:87-105, 270-300; EriStreamFP) and returns it
+ * (native_quartets.ipp:759-765; eri_direct_jk.hpp:25-33):
+ *   out[7] = {sum x, sum |x|, sum x^2, m1, m2, m3, max |x|},
+ *   m_k = sum x w_k(i), i = qidx(pair(a,b), pair(c,d)) the canonical index,
+ *   w_k(i) = ((splitmix64(i ^ seed_k) >> 40) - 2^23) / 2^23 with the
+ *   reference's seeds; *count = number of canonical values.  Unlike the
+ * checksum it changes when a value lands in the wrong slot.  Spherical
+ * plans only; allocates scratch for the Cartesian blocks (up to 256 MiB). */
+int tlno_gpu_eri_fingerprint(tlno_gpu_plan* plan, double schwarz, double* out, unsigned long long* count);
+
 /* Learned screen (no reference counterpart).  params: TLNO_GPU_MLP_NALL
  * floats of a network trained with tlno_gpu_mlp_* (NULL turns the screen
  * off).  Following builds evaluate the network on the matrix cores for every

@@ -178,6 +178,51 @@ class OracleBasis:
 
 
 # --------------------------------------------------------------------------
+# stream fingerprint of a canonical ERI stream
+# --------------------------------------------------------------------------
+# Restates the reference's indexed fingerprint (rys_engine.ipp:87-105 for the
+# weights, 270-300 for the accumulation): splitmix64's output function of
+# state v (golden-gamma increment, then two xor-shift-multiply rounds and a
+# final xor-shift), bits 40..63 centred and scaled to [-1, 1), three seeds,
+# long-double sums.
+FP_SEEDS = (0x243f6a8885a308d3, 0x13198a2e03707344, 0xa4093822299f31d0)
+_U = np.uint64
+
+
+def fp_mix(v):
+    v = np.asarray(v, dtype=_U)
+    with np.errstate(over="ignore"):
+        v = v + _U(0x9e3779b97f4a7c15)
+        v = (v ^ (v >> _U(30))) * _U(0xbf58476d1ce4e5b9)
+        v = (v ^ (v >> _U(27))) * _U(0x94d049bb133111eb)
+    return v ^ (v >> _U(31))
+
+
+def fp_weight(idx, seed):
+    m = (fp_mix(np.asarray(idx, dtype=_U) ^ _U(seed)) >> _U(40)) & _U(0xffffff)
+    return (m.astype(np.int64) - 0x800000).astype(np.longdouble) / np.longdouble(8388608)
+
+
+def canonical_index(i, j):
+    """hi (hi + 1) / 2 + lo (twoe_integrals.hpp:17-34), elementwise."""
+    i = np.asarray(i, dtype=_U)
+    j = np.asarray(j, dtype=_U)
+    hi, lo = np.maximum(i, j), np.minimum(i, j)
+    return hi * (hi + _U(1)) // _U(2) + lo
+
+
+def stream_fingerprint(lhs_pair, rhs_pair, values):
+    """Fingerprint of a canonical stream given as (lhs pair, rhs pair, value)."""
+    x = np.asarray(values, dtype=np.longdouble)
+    idx = canonical_index(lhs_pair, rhs_pair)
+    out = {"checksum": x.sum(), "absolute_sum": np.abs(x).sum(), "square_sum": (x * x).sum(),
+           "max_abs": np.abs(x).max() if len(x) else np.longdouble(0), "canonical_count": len(x)}
+    for k, seed in enumerate(FP_SEEDS):
+        out[f"indexed_moment_{k + 1}"] = (x * fp_weight(idx, seed)).sum()
+    return {k: (v if k == "canonical_count" else float(v)) for k, v in out.items()}
+
+
+# --------------------------------------------------------------------------
 # the compiled reference (oracle/_ref), optional
 # --------------------------------------------------------------------------
 def host_isa():
@@ -197,7 +242,8 @@ def ref_lib(kind="api"):
     if kind in _ref:
         return _ref[kind]
     isa = host_isa()
-    name = f"libtlno_ref_{isa}.so" if kind == "api" else f"libtlno_ref_audit_{isa}.so"
+    name = {"api": f"libtlno_ref_{isa}.so", "audit": f"libtlno_ref_audit_{isa}.so",
+            "fp": "libtlno_ref_fp_avx512.so"}[kind]
     path = os.path.join(_HERE, "_ref", name)
     if not os.path.exists(path):
         return None
@@ -218,6 +264,9 @@ def ref_lib(kind="api"):
         L.ref_eri_checksum.argtypes = [C.c_int, _D, _I, _I, _D, _D, C.c_double, _D]
         L.ref_onee.argtypes = [C.c_int, _D, _I, _I, _D, _D, C.c_int, _I, _D, C.c_int, C.c_int,
                                _D, _D, _D, _I]
+    elif kind == "fp":
+        L.ref_fp_last_error.restype = C.c_char_p
+        L.ref_fp_stream.argtypes = [C.c_int, _D, _I, _I, _D, _D, C.c_double, _D]
     else:
         L.ref_audit_last_error.restype = C.c_char_p
         L.ref_audit_groups.argtypes = [C.c_int, _D, _I, _I, _D, _D, C.c_int, _I, _I, _I]
@@ -287,6 +336,19 @@ class Reference:
         if self.api.ref_eri_checksum(*args, float(schwarz), _dp(v)):
             raise ValueError(self.err())
         return float(v[0])
+
+    def fingerprint(self, basis, schwarz=0.0):
+        """The reference's stream fingerprint {checksum, m1, m2, m3, count}
+        (its TLNO_VALIDATE_RYS_STREAM_FINGERPRINT build, oracle/ref_fp_shim.cpp)."""
+        L = ref_lib("fp")
+        if L is None:
+            raise RuntimeError("compiled reference fingerprint build (oracle/_ref) is not available")
+        keep, args = _basis_args(basis)
+        v = np.zeros(5)
+        self.api.ref_set_threads(1)  # the packed endpoint runs on one thread
+        if L.ref_fp_stream(*args, float(schwarz), _dp(v)):
+            raise ValueError(L.ref_fp_last_error().decode())
+        return v
 
     def onee(self, basis, Z, R, spherical=False, screening=True):
         keep, args = _basis_args(basis)

@@ -39,8 +39,14 @@ API_LIBS := $(foreach i,$(ISAS),$(OUT)/libtlno_ref_$(i).so)
 AUDIT_LIBS := $(foreach i,$(ISAS),$(OUT)/libtlno_ref_audit_$(i).so)
 
 ONEE_CAPI_LIB := $(OUT)/libtlno_ref_onee_capi.so
+# the engine with its stream-fingerprint validation switch on (fixture
+# generation only, AVX-512 build)
+FP_LIB := $(OUT)/libtlno_ref_fp_avx512.so
 
-all: $(API_LIBS) $(AUDIT_LIBS) $(ONEE_CAPI_LIB)
+all: $(API_LIBS) $(AUDIT_LIBS) $(ONEE_CAPI_LIB) $(FP_LIB)
+
+$(FP_LIB): oracle/ref_fp_shim.cpp $(CORE_SRC) | $(OUT)
+	$(CXX) $(CXXFLAGS_COMMON) $(march_avx512) -shared -o $@ $^
 
 $(ONEE_CAPI_LIB): $(CORE_SRC) $(ONEE_SRC) $(ONEE_CAPI_SRC) | $(OUT)
 	$(CXX) -std=c++17 -O2 -fopenmp -fPIC -I$(REF) $(march_avx512) -shared -o $@ $^

@@ -36,8 +36,8 @@ inline constexpr double kEriPackedDefaultSchwarzThreshold = 1.0e-12;
 
 struct EriPackedBuildOptions {
     double schwarz_threshold = kEriPackedDefaultSchwarzThreshold;
-    double primitive_fastschwarz_threshold = 0.0;  // must be 0 on the GPU engine
-    double decay_threshold = 0.0;                  // must be 0 on the GPU engine
+    double primitive_fastschwarz_threshold = 0.0;  // J/K builds; 0 for the stream endpoints
+    double decay_threshold = 0.0;                  // J/K builds; 0 for the stream endpoints
     double density_threshold = 0.0;                // J/K builds only
 };
 
@@ -75,6 +75,15 @@ EriPackedPlan build_eri_packed_plan_for_G_rys(const std::vector<tlno::core::basi
                                               const EriPackedBuildOptions& options);
 
 EriPackedGData build_eri_packed_for_G_rys(const EriPackedPlan& plan, const EriPackedBuildOptions& options);
+
+// What build_eri_packed_for_G_rys returns in the reference's
+// TLNO_VALIDATE_RYS_STREAM_FINGERPRINT build (native_quartets.ipp:759-765):
+// values = {checksum, indexed_moment_1, indexed_moment_2, indexed_moment_3,
+// canonical_count}, the indexed moments weighting each canonical value by
+// hashes of its quartet index (rys_engine.ipp:87-105).  Here a separate
+// entry point rather than a build switch.
+EriPackedGData build_eri_packed_fingerprint_for_G_rys(const EriPackedPlan& plan,
+                                                      const EriPackedBuildOptions& options);
 
 }  // namespace tlno::integrals::twoe
 

@@ -187,6 +187,18 @@ EriPackedGData build_eri_packed_for_G_rys(const EriPackedPlan& plan, const EriPa
     return EriPackedGData{{v}};
 }
 
+EriPackedGData build_eri_packed_fingerprint_for_G_rys(const EriPackedPlan& plan,
+                                                      const EriPackedBuildOptions& options) {
+    check_schwarz(options.schwarz_threshold);
+    require_schwarz_only(options.primitive_fastschwarz_threshold, options.decay_threshold);
+    GpuPlan& g = gpu_of(plan);
+    std::lock_guard<std::mutex> lk(g.mu);
+    double f[7] = {};
+    unsigned long long n = 0;
+    check(tlno_gpu_eri_fingerprint(g.h, options.schwarz_threshold, f, &n));
+    return EriPackedGData{{f[0], f[3], f[4], f[5], static_cast<double>(n)}};
+}
+
 EriDirectJKResult build_jk_direct_for_G_rys(const EriPackedPlan& plan, const std::vector<double>& density,
                                             const EriPackedBuildOptions& options) {
     check_schwarz(options.schwarz_threshold);

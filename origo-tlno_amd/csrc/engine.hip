@@ -61,6 +61,7 @@ constexpr int kSideStreams = tlno_amd_side_streams;
 // ss, ps, pp, ds, dp, dd
 constexpr int kClassLAB[6] = {0, 1, 2, 2, 3, 4};
 constexpr int kClassNcart[6] = {1, 3, 9, 6, 18, 36};
+constexpr int kClassL[6][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
 
 int side_streams_for_queues() {
     int q = 4;  // HIP's default
@@ -1101,7 +1102,7 @@ void run_store(tlno_gpu_plan* pl, int X, int Y, const std::vector<int2>& qs, con
     HIP_OK(hipMalloc(&doff, off.size() * sizeof(long long)));
     HIP_OK(hipMemcpyAsync(dq, qs.data(), qs.size() * sizeof(int2), hipMemcpyHostToDevice, pl->stream));
     HIP_OK(hipMemcpyAsync(doff, off.data(), off.size() * sizeof(long long), hipMemcpyHostToDevice, pl->stream));
-    StoreArgs a;
+    StoreArgs a{};
     a.pd = pl->pair_data();
     a.tb = ctx.tb;
     a.quartets = dq;
@@ -2352,6 +2353,244 @@ double eri_checksum(tlno_gpu_plan* pl, double thr) {
     return total;
 }
 
+// ------------------------------------------------ stream fingerprint ---
+// The reference's indexed fingerprint of the canonical spherical stream
+// (rys_engine.ipp:87-105 weights, 270-300 accumulation; the values its
+// validation build returns, native_quartets.ipp:759-765): over every unique
+// (ab|cd) x with canonical index i = qidx(pair(a,b), pair(c,d)),
+//   sum x, sum |x|, sum x^2, sum x w_k(i) (k = 1..3), max |x|, count,
+// w_k(i) = (bits 40..63 of splitmix64(i ^ seed_k) - 2^23) / 2^23.
+// Cartesian blocks of a batch of shell quartets are written by the block
+// kernel, then one wave per quartet projects its block to spherical in LDS,
+// applies the unique-index rule and adds its sums into 64 slots (slot =
+// quartet mod 64; the host sums the slots in long double).
+constexpr unsigned long long kFpSeed[3] = {0x243f6a8885a308d3ull, 0x13198a2e03707344ull, 0xa4093822299f31d0ull};
+
+struct C2SSparse {
+    int n[3][5];
+    int idx[3][5][6];
+    double c[3][5][6];
+};
+
+struct FPArgs {
+    const double* blocks;  // nblk doubles per quartet, pair orientation
+    const int2* quartets;
+    int n;
+    int LA, LB, LC, LD;
+    const int4* pinfo;     // per pair {spherical base of A, of B, same shell, 0}
+    C2SSparse cs;
+    double* acc;                  // 64 slots x 6 sums
+    unsigned long long* maxbits;  // 64 slots (bit patterns of non-negative doubles)
+    unsigned long long* count;    // 64 slots
+};
+
+__device__ __forceinline__ unsigned long long fp_mix(unsigned long long v) {
+    v += 0x9e3779b97f4a7c15ull;
+    v = (v ^ (v >> 30)) * 0xbf58476d1ce4e5b9ull;
+    v = (v ^ (v >> 27)) * 0x94d049bb133111ebull;
+    return v ^ (v >> 31);
+}
+
+__device__ __forceinline__ double fp_weight(unsigned long long i, unsigned long long seed) {
+    const long long m = static_cast<long long>((fp_mix(i ^ seed) >> 40) & 0xffffffull) - 0x800000ll;
+    return static_cast<double>(m) * (1.0 / 8388608.0);
+}
+
+__device__ __forceinline__ long long canon_pair(long long i, long long j) {
+    return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
+}
+
+__global__ __launch_bounds__(64) void k_fingerprint(FPArgs a) {
+    __shared__ double sblk[1296];
+    __shared__ double shalf[900];
+    __shared__ double ssph[625];
+    const int q = blockIdx.x;
+    if (q >= a.n) return;
+    const int lane = threadIdx.x;
+    const int na = ncart(a.LA), nb = ncart(a.LB), nc = ncart(a.LC), nd = ncart(a.LD);
+    const int ncd = nc * nd, nblk = na * nb * ncd;
+    const int sa = 2 * a.LA + 1, sb = 2 * a.LB + 1, sc = 2 * a.LC + 1, sd = 2 * a.LD + 1;
+    const double* blk = a.blocks + static_cast<long long>(q) * nblk;
+    for (int i = lane; i < nblk; i += 64) sblk[i] = blk[i];
+    __syncthreads();
+    for (int i = lane; i < sa * sb * ncd; i += 64) {
+        const int s = i / (sb * ncd), t = (i / ncd) % sb, cd = i % ncd;
+        double v = 0.0;
+        for (int k1 = 0; k1 < a.cs.n[a.LA][s]; ++k1)
+            for (int k2 = 0; k2 < a.cs.n[a.LB][t]; ++k2)
+                v += a.cs.c[a.LA][s][k1] * a.cs.c[a.LB][t][k2] *
+                     sblk[(a.cs.idx[a.LA][s][k1] * nb + a.cs.idx[a.LB][t][k2]) * ncd + cd];
+        shalf[i] = v;
+    }
+    __syncthreads();
+    const int nsph = sa * sb * sc * sd;
+    for (int i = lane; i < nsph; i += 64) {
+        const int st = i / (sc * sd), u = (i / sd) % sc, w = i % sd;
+        double v = 0.0;
+        for (int k1 = 0; k1 < a.cs.n[a.LC][u]; ++k1)
+            for (int k2 = 0; k2 < a.cs.n[a.LD][w]; ++k2)
+                v += a.cs.c[a.LC][u][k1] * a.cs.c[a.LD][w][k2] *
+                     shalf[st * ncd + a.cs.idx[a.LC][u][k1] * nd + a.cs.idx[a.LD][w][k2]];
+        ssph[i] = v;
+    }
+    __syncthreads();
+    const int2 pq = a.quartets[q];
+    const int4 iP = a.pinfo[pq.x], iQ = a.pinfo[pq.y];
+    const bool same_pair = pq.x == pq.y;
+    double sum = 0.0, sabs = 0.0, ssq = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0, mx = 0.0;
+    int cnt = 0;
+    for (int i = lane; i < nsph; i += 64) {
+        const int s = i / (sb * sc * sd), t = (i / (sc * sd)) % sb, u = (i / sd) % sc, w = i % sd;
+        const long long ia = iP.x + s, ib = iP.y + t, ic = iQ.x + u, id = iQ.y + w;
+        if (iP.z && ib > ia) continue;
+        if (iQ.z && id > ic) continue;
+        const long long p1 = canon_pair(ia, ib), p2 = canon_pair(ic, id);
+        if (same_pair && p2 > p1) continue;
+        const unsigned long long idx = static_cast<unsigned long long>(canon_pair(p1, p2));
+        const double x = ssph[i];
+        const double ax = fabs(x);
+        sum += x;
+        sabs += ax;
+        ssq += x * x;
+        m1 += x * fp_weight(idx, kFpSeed[0]);
+        m2 += x * fp_weight(idx, kFpSeed[1]);
+        m3 += x * fp_weight(idx, kFpSeed[2]);
+        mx = fmax(mx, ax);
+        ++cnt;
+    }
+    sum = wave_sum(sum);
+    sabs = wave_sum(sabs);
+    ssq = wave_sum(ssq);
+    m1 = wave_sum(m1);
+    m2 = wave_sum(m2);
+    m3 = wave_sum(m3);
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        mx = fmax(mx, __shfl_xor(mx, m, 64));
+        cnt += __shfl_xor(cnt, m, 64);
+    }
+    if (lane == 0) {
+        double* slot = a.acc + 6 * (q & 63);
+        atomicAdd(slot + 0, sum);
+        atomicAdd(slot + 1, sabs);
+        atomicAdd(slot + 2, ssq);
+        atomicAdd(slot + 3, m1);
+        atomicAdd(slot + 4, m2);
+        atomicAdd(slot + 5, m3);
+        atomicMax(a.maxbits + (q & 63), static_cast<unsigned long long>(__double_as_longlong(mx)));
+        atomicAdd(a.count + (q & 63), static_cast<unsigned long long>(cnt));
+    }
+}
+
+// out = {checksum, sum |x|, sum x^2, m1, m2, m3, max |x|}, count = number of
+// canonical values, over shell quartets with Q_ab Q_cd >= thr.
+void eri_fingerprint(tlno_gpu_plan* pl, double thr, double* out, unsigned long long* count) {
+    if (!pl->spherical)
+        throw std::runtime_error("eri_ao_tensor_hgp: the ERI stream is pure spherical; build a spherical plan.");
+    if (!std::isfinite(thr) || thr < 0.0)
+        throw std::runtime_error("eri_ao_tensor_hgp: schwarz_threshold must be finite and >= 0.");
+    require_alloc("the stream fingerprint");
+    DeviceCtx& ctx = device_ctx(pl->device);
+    HIP_OK(hipSetDevice(pl->device));
+    const hipStream_t st = pl->stream;
+    FPArgs f{};
+    const C2S& c2 = c2s_table();
+    for (int L = 0; L < 3; ++L)
+        for (int s = 0; s < 2 * L + 1; ++s) {
+            int k = 0;
+            for (int c = 0; c < ncart(L); ++c)
+                if (c2.c[L][s][c] != 0.0) {
+                    f.cs.idx[L][s][k] = c;
+                    f.cs.c[L][s][k] = c2.c[L][s][c];
+                    ++k;
+                }
+            f.cs.n[L][s] = k;
+        }
+    std::vector<int4> pinfo(static_cast<std::size_t>(pl->npair));
+    for (int P = 0; P < pl->npair; ++P)
+        pinfo[P] = make_int4(pl->groups[pl->pgA[P]].sph_base, pl->groups[pl->pgB[P]].sph_base,
+                             pl->pgA[P] == pl->pgB[P] ? 1 : 0, 0);
+    int4* dpi = nullptr;
+    double* dacc = nullptr;
+    unsigned long long *dmax = nullptr, *dcnt = nullptr;
+    double* dblk = nullptr;
+    constexpr long long kBatchDoubles = 32ll << 20;  // 256 MiB of blocks per launch pair
+    struct Freer {
+        std::vector<void*> p;
+        ~Freer() {
+            for (void* x : p) (void)hipFree(x);
+        }
+    } fr;
+    HIP_OK(hipMalloc(&dpi, pinfo.size() * sizeof(int4)));
+    fr.p.push_back(dpi);
+    HIP_OK(hipMalloc(&dacc, 6 * 64 * sizeof(double)));
+    fr.p.push_back(dacc);
+    HIP_OK(hipMalloc(&dmax, 2 * 64 * sizeof(unsigned long long)));
+    fr.p.push_back(dmax);
+    dcnt = dmax + 64;
+    HIP_OK(hipMemcpyAsync(dpi, pinfo.data(), pinfo.size() * sizeof(int4), hipMemcpyHostToDevice, st));
+    k_fill<<<grid_for(6 * 64, 256), 256, 0, st>>>(dacc, 6 * 64, 0.0);
+    k_fill<<<1, 128, 0, st>>>(reinterpret_cast<double*>(dmax), 128, 0.0);
+    HIP_OK(hipGetLastError());
+    TaskSet& ts = get_tasks(pl, thr);
+    f.pinfo = dpi;
+    f.acc = dacc;
+    f.maxbits = dmax;
+    f.count = dcnt;
+    for (int X = 5; X >= 0; --X)
+        for (int Y = X; Y >= 0; --Y) {
+            if (ts.cnt[X][Y] == 0) continue;
+            const int2* tq = ensure_team_quartets(pl, ts, X, Y);
+            const long long nq = ts.cum_sq[X][Y].empty() ? 0 : ts.cum_sq[X][Y].back();
+            if (!tq || nq <= 0) continue;
+            f.LA = kClassL[X][0];
+            f.LB = kClassL[X][1];
+            f.LC = kClassL[Y][0];
+            f.LD = kClassL[Y][1];
+            const long long nblk = static_cast<long long>(ncart(f.LA)) * ncart(f.LB) * ncart(f.LC) * ncart(f.LD);
+            const long long batch = std::min(nq, std::max(1ll, kBatchDoubles / nblk));
+            if (!dblk) {
+                HIP_OK(hipMalloc(&dblk, kBatchDoubles * sizeof(double)));
+                fr.p.push_back(dblk);
+            }
+            for (long long b0 = 0; b0 < nq; b0 += batch) {
+                const int nb = static_cast<int>(std::min(batch, nq - b0));
+                StoreArgs sa{};
+                sa.pd = pl->pair_data();
+                sa.tb = ctx.tb;
+                sa.quartets = tq + b0;
+                sa.off = nullptr;
+                sa.stride = nblk;
+                sa.n = nb;
+                sa.out = dblk;
+                launch_store(g_kernels[X][Y], sa, st);
+                f.blocks = dblk;
+                f.quartets = tq + b0;
+                f.n = nb;
+                k_fingerprint<<<static_cast<unsigned>(nb), 64, 0, st>>>(f);
+                HIP_OK(hipGetLastError());
+            }
+        }
+    std::vector<double> hacc(6 * 64);
+    std::vector<unsigned long long> hm(128);
+    HIP_OK(hipMemcpyAsync(hacc.data(), dacc, hacc.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(hm.data(), dmax, hm.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    long double tot[6] = {};
+    double mx = 0.0;
+    unsigned long long c = 0;
+    for (int k = 0; k < 64; ++k) {
+        for (int j = 0; j < 6; ++j) tot[j] += hacc[6 * k + j];
+        double m;
+        std::memcpy(&m, &hm[k], sizeof(double));
+        mx = std::max(mx, m);
+        c += hm[64 + k];
+    }
+    for (int j = 0; j < 6; ++j) out[j] = static_cast<double>(tot[j]);
+    out[6] = mx;
+    *count = c;
+}
+
 // Per class pair, time the lane and the team evaluator on this plan (FP64,
 // J and K, a constant density, serialized launches bracketed by events) and
 // keep the faster one; returns a JSON summary.
@@ -2824,6 +3063,14 @@ int tlno_gpu_eri_checksum(tlno_gpu_plan* pl, double schwarz, double* checksum) {
         if (!pl) throw std::runtime_error("eri_ao_tensor_hgp: null packed ERI plan.");
         if (!checksum) throw std::runtime_error("tlno_gpu: null output");
         *checksum = eri_checksum(pl, schwarz);
+    });
+}
+
+int tlno_gpu_eri_fingerprint(tlno_gpu_plan* pl, double schwarz, double* out, unsigned long long* count) {
+    return guarded([&] {
+        if (!pl) throw std::runtime_error("eri_ao_tensor_hgp: null packed ERI plan.");
+        if (!out || !count) throw std::runtime_error("tlno_gpu: null output");
+        eri_fingerprint(pl, schwarz, out, count);
     });
 }
 

@@ -553,7 +553,8 @@ struct StoreArgs {
     PairData pd;
     DevTables tb;
     const int2* quartets;   // {P, Q}
-    const long long* off;   // output offset of each quartet's full block
+    const long long* off;   // output offset of each quartet's full block (nullptr: i * stride)
+    long long stride;
     int n;
     double* out;
 };
@@ -830,7 +831,7 @@ __device__ __forceinline__ void eri_store_body(const StoreArgs& a) {
     const int2 pq = a.quartets[i];
     double acc[S::NOUT];
     quartet<double, LA, LB, LC, LD, AC, AN, CC, CN>(a.tb, a.pd, pq.x, pq.y, 0, a.pd.pn[pq.x], 0, 1, false, acc);
-    double* out = a.out + a.off[i];
+    double* out = a.out + (a.off ? a.off[i] : static_cast<long long>(i) * a.stride);
     int o = 0;
 #pragma unroll
     for (int ai = 0; ai < S::NAL; ++ai)

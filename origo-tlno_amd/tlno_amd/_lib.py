@@ -46,6 +46,7 @@ SIGNATURES = {
     "tlno_gpu_onee": (C.c_int, [C.c_void_p, C.c_int, D, D, D, D, D]),
     "tlno_gpu_onee_screened": (C.c_int, [C.c_void_p, C.c_int, D, D, C.c_double, C.c_double, D, D, D, D]),
     "tlno_gpu_eri_checksum": (C.c_int, [C.c_void_p, C.c_double, D]),
+    "tlno_gpu_eri_fingerprint": (C.c_int, [C.c_void_p, C.c_double, D, C.POINTER(C.c_ulonglong)]),
     "tlno_gpu_batch_create": (C.c_int, [C.c_int, I, D, I, I, D, D, C.POINTER(C.c_void_p)]),
     "tlno_gpu_batch_destroy": (None, [C.c_void_p]),
     "tlno_gpu_batch_info": (C.c_int, [C.c_void_p, I, LL, I, LL]),

@@ -185,6 +185,23 @@ class DirectPlan:
         _lib.check(_lib.load().tlno_gpu_eri_checksum(self._h, float(thr), C.byref(v)))
         return v.value
 
+    def eri_fingerprint(self, schwarz=None):
+        """The reference's indexed stream fingerprint of the same canonical
+        values (its TLNO_VALIDATE_RYS_STREAM_FINGERPRINT build,
+        rys_engine.ipp:87-105, 270-300): checksum, absolute_sum, square_sum,
+        indexed_moment_1..3 (values weighted by splitmix64 hashes of their
+        canonical quartet index), max_abs and canonical_count."""
+        thr = self.options.integral_threshold if schwarz is None else schwarz
+        _check_threshold("schwarz_threshold", thr)
+        out = np.zeros(7)
+        n = C.c_ulonglong(0)
+        _lib.check(_lib.load().tlno_gpu_eri_fingerprint(self._h, float(thr), out.ctypes.data_as(_lib.D), C.byref(n)))
+        keys = ("checksum", "absolute_sum", "square_sum", "indexed_moment_1", "indexed_moment_2",
+                "indexed_moment_3", "max_abs")
+        r = {k: float(v) for k, v in zip(keys, out)}
+        r["canonical_count"] = int(n.value)
+        return r
+
     def set_learned_screen(self, params=None, margin=0.0):
         """Turn the learned shell-quartet screen on (params: 6401 floats of a
         trained network, margin in decades) or off (params None)."""
@@ -339,11 +356,14 @@ def adopt_eri_plan_for_transition(plan: DirectPlan) -> DirectPlan:
 @dataclass
 class EriPackedGData:
     """values = [checksum of the canonical spherical stream]
-    (twoe_integrals.hpp:37-39, rys_engine.ipp:17096-17103)."""
+    (twoe_integrals.hpp:37-39, rys_engine.ipp:17096-17103); with
+    fingerprint=True [checksum, indexed_moment_1..3, canonical_count], what
+    the reference's fingerprint build returns (native_quartets.ipp:759-765)."""
     values: list = field(default_factory=list)
 
 
-def build_eri_packed_for_G_rys(plan: DirectPlan, options: EriPackedBuildOptions | None = None) -> EriPackedGData:
+def build_eri_packed_for_G_rys(plan: DirectPlan, options: EriPackedBuildOptions | None = None,
+                               fingerprint: bool = False) -> EriPackedGData:
     o = options or screening_to_eri_options(plan.options)
     _check_threshold("schwarz_threshold", o.schwarz_threshold)
     _check_threshold("decay_threshold", o.decay_threshold)
@@ -352,6 +372,10 @@ def build_eri_packed_for_G_rys(plan: DirectPlan, options: EriPackedBuildOptions 
         if getattr(o, name) != 0.0:
             raise NotImplementedError(f"{name} screening is not available in the GPU engine")
     # the stream ignores density_threshold: there is no density to bound against
+    if fingerprint:
+        f = plan.eri_fingerprint(o.schwarz_threshold)
+        return EriPackedGData([f["checksum"], f["indexed_moment_1"], f["indexed_moment_2"], f["indexed_moment_3"],
+                               float(f["canonical_count"])])
     return EriPackedGData([plan.eri_checksum(o.schwarz_threshold)])
 
 

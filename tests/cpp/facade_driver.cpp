@@ -81,6 +81,7 @@ int main(int argc, char** argv) {
         twoe::EriPackedBuildOptions eo;
         const twoe::EriPackedPlan eplan = twoe::build_eri_packed_plan_for_G_rys(build.basis, eo);
         put(out, "checksum", twoe::build_eri_packed_for_G_rys(eplan, eo).values);
+        put(out, "fingerprint", twoe::build_eri_packed_fingerprint_for_G_rys(eplan, eo).values);
         put(out, "G_twoe", twoe::build_jk_direct_for_G_rys(eplan, D0, eo).g);
         const fock::DirectPlan adopted = fock::adopt_eri_plan_for_transition(eplan);
         put(out, "G_adopted", fock::build_rhf_g(adopted, D1, so).g);

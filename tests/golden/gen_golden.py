@@ -44,6 +44,25 @@ tests) and the reference's outputs:
                    shell pair its own SIMD bin) with FASTSCHWARZ 1e-6, 1e-4,
                    1e-2 at Schwarz 0, and unscreened (python gen_golden.py
                    screens rewrites only it)
+  batch256.npz     configs.batch256() (256 molecules, 25-460 Cartesian
+                   functions), every 16th molecule and the largest: the
+                   oracle's Cartesian J and K at Schwarz 1e-12 as row sketches
+                   J X, K X (X seeded N(0, 1), M x 4), diagonals and Frobenius
+                   norms.  The reference's direct build is spherical only, so
+                   these come from the oracle, whose J/K are pinned against
+                   the reference's on water7, organic5, organic30 and the
+                   4-water cluster (python gen_golden.py batch256 rewrites
+                   only it)
+  water10.npz      configs.water_cluster(10) (250 Cartesian functions): the
+                   oracle's Cartesian J and K at Schwarz 1e-12 in full, the
+                   CPU reference of the mixed-precision route (python
+                   gen_golden.py water10 rewrites only it)
+  fingerprint.npz  the reference's indexed stream fingerprint ({checksum,
+                   indexed_moment_1..3, canonical_count}, its
+                   TLNO_VALIDATE_RYS_STREAM_FINGERPRINT build in
+                   oracle/_ref/libtlno_ref_fp_avx512.so) of water7 at Schwarz
+                   0, organic5 at 0 and 1e-12, organic30 at 1e-12
+                   (python gen_golden.py fingerprint rewrites only it)
   density.npz      4-water cluster (96 spherical functions), density
                    configs.spherical_density(96, 9, 20): the reference's
                    density-screened J, K and G (density_threshold 1e-9, 1e-7,
@@ -87,6 +106,12 @@ def main(only=None):
         return onee_capi()
     if only == "screens":
         return screens(ref)
+    if only == "fingerprint":
+        return fingerprint(ref)
+    if only == "batch256":
+        return batch256()
+    if only == "water10":
+        return water10()
     t0 = time.time()
     # ---------------------------------------------------------- Boys / Rys
     xs = np.concatenate([[0.0, 1e-10, 1e-3, 0.5, 5.0, 11.99, 12.0, 12.5, 25.0, 39.99, 40.0, 45.0, 80.0, 200.0],
@@ -152,6 +177,9 @@ def main(only=None):
     checksums(ref)
     onee_capi()
     screens(ref)
+    batch256()
+    water10()
+    fingerprint(ref)
     print(f"golden fixtures written in {time.time() - t0:.1f} s")
 
 
@@ -269,6 +297,50 @@ def organic30(ref):
                         U=U, Vp=Vv, J_probe=np.einsum("im,mn,in->i", U, J, Vv),
                         K_probe=np.einsum("im,mn,in->i", U, K, Vv), J_fro=np.linalg.norm(J),
                         K_fro=np.linalg.norm(K), ref_seconds=np.array([tg, tj, tk]))
+
+
+def fingerprint(ref):
+    # ---------------------------------------------------------- stream fingerprints
+    out = {}
+    cases = (("water7", configs.water7()[0], (0.0,)), ("organic5", configs.organic30(natoms=5)[0], (0.0, 1e-12)),
+             ("organic30", configs.organic30()[0], (1e-12,)))
+    for name, b, thrs in cases:
+        out[f"{name}_sha"] = basis_hash(b)
+        for thr in thrs:
+            tag = "0" if thr == 0.0 else "12"
+            out[f"{name}_{tag}"] = ref.fingerprint(b, thr)
+            print(f"fingerprint {name} at {thr}: {out[f'{name}_{tag}']}", flush=True)
+    np.savez_compressed(os.path.join(HERE, "fingerprint.npz"), **out)
+
+
+BATCH256_PICK = tuple(range(0, 256, 16))
+
+
+def batch256():
+    # ---------------------------------------------------------- batch256 sketches
+    mols = configs.batch256()
+    Ms = [D.shape[0] for _, D in mols]
+    pick = sorted(set(BATCH256_PICK) | {int(np.argmax(Ms))})
+    out = {"pick": np.array(pick, dtype=np.int32)}
+    for i in pick:
+        b, D = mols[i]
+        J, K = O.OracleBasis(b).jk(D, spherical=False, schwarz=1e-12)
+        X = np.random.default_rng(1000 + i).standard_normal((D.shape[0], 4))
+        out.update({f"sha{i}": basis_hash(b), f"X{i}": X, f"JX{i}": J @ X, f"KX{i}": K @ X,
+                    f"Jd{i}": np.diag(J).copy(), f"Kd{i}": np.diag(K).copy(),
+                    f"Jf{i}": np.linalg.norm(J), f"Kf{i}": np.linalg.norm(K)})
+        print(f"batch256 molecule {i}: M {D.shape[0]}", flush=True)
+    np.savez_compressed(os.path.join(HERE, "batch256.npz"), **out)
+
+
+def water10():
+    # ---------------------------------------------------------- water10 FP64 reference
+    b, D = configs.water_cluster(10)
+    J, K = O.OracleBasis(b).jk(D, spherical=False, schwarz=1e-12)
+    # the density's generator rounds differently on other hosts' BLAS: its
+    # sum and norm identify it to that rounding
+    np.savez_compressed(os.path.join(HERE, "water10.npz"), basis_sha=basis_hash(b), D_sum=D.sum(),
+                        D_fro=np.linalg.norm(D), J=J, K=K)
 
 
 if __name__ == "__main__":

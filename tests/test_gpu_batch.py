@@ -32,6 +32,36 @@ def test_batch_matches_oracle_per_molecule():
     assert st["shell_quartets"] > 0
 
 
+def test_batch256_against_oracle_sketches():
+    """The real 256-molecule batch (25-460 Cartesian functions) in one
+    batched build at Schwarz 1e-12, against the oracle's J/K of every 16th
+    molecule and the largest (tests/golden/batch256.npz): row sketches
+    J X (X seeded N(0, 1), M x 4), diagonals and Frobenius norms.  An entry
+    bound |dJ| <= 1e-10 implies |(dJ X)_ij| <= 1e-10 sum_k |X_kj|, the bound
+    asserted here; a single entry off by 1e-7 already breaks it."""
+    import os
+    from test_oracle import basis_hash
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "batch256.npz"))
+    mols = configs.batch256()
+    bp = BatchPlan([b for b, _ in mols])
+    Js, Ks = bp.jk([D for _, D in mols], schwarz=1e-12)
+    worst = 0.0
+    for i in g["pick"]:
+        i = int(i)
+        b, _ = mols[i]
+        assert basis_hash(b) == str(g[f"sha{i}"])
+        X = g[f"X{i}"]
+        lim = 1e-10 * np.abs(X).sum(0)
+        for W, w in ((Js[i], "J"), (Ks[i], "K")):
+            d = np.abs(W @ X - g[f"{w}X{i}"])
+            assert np.all(d <= lim), (i, w, (d / lim).max())
+            assert np.abs(np.diag(W) - g[f"{w}d{i}"]).max() <= 1e-10, (i, w)
+            assert abs(np.linalg.norm(W) - float(g[f"{w}f{i}"])) <= 1e-10 * W.shape[0], (i, w)
+            worst = max(worst, (d / lim).max())
+    print(f"batch256: {len(g['pick'])} molecules against the oracle, worst sketch error "
+          f"{worst:.2e} of the 1e-10 entry bound")
+
+
 def test_batch_matches_single_plans_and_shapes():
     mols = _mols(n=6, atoms=(8, 14), seed=5)
     bp = BatchPlan([b for b, _ in mols])

@@ -53,6 +53,11 @@ def test_facade_matches_reference_and_mirror(name, tmp_path):
     assert np.abs(res["G"] - gref.ravel()).max() < 1e-10
     assert res["checksum"][0] == pytest.approx(plan.eri_checksum(1e-12), rel=1e-13)
     assert res["checksum"][0] == pytest.approx(float(g["checksum"]), rel=1e-9)
+    fp = plan.eri_fingerprint(1e-12)
+    assert len(res["fingerprint"]) == 5 and res["fingerprint"][4] == fp["canonical_count"]
+    assert res["fingerprint"][0] == pytest.approx(res["checksum"][0], rel=1e-12)
+    for k in (1, 2, 3):
+        assert abs(res["fingerprint"][k] - fp[f"indexed_moment_{k}"]) <= 1e-13 * fp["absolute_sum"]
     # incremental state: exact, incremental, skipped, incremental, periodic exact, forced exact
     assert res["status"].tolist() == [1, 2, 3, 2, 1, 1]
     assert res["counters"].tolist() == [6, 3, 2, 1]

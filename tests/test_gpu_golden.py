@@ -160,3 +160,48 @@ def test_eri_stream_checksums_vs_reference():
     assert c30["8"] == pytest.approx(float(gc["organic30_8"]), rel=1e-11)
     with pytest.raises(Exception, match="pure spherical"):
         DirectPlan(b7, cartesian=True).eri_checksum(0.0)
+
+
+def test_eri_stream_fingerprints_vs_reference():
+    """The reference's indexed stream fingerprint (its
+    TLNO_VALIDATE_RYS_STREAM_FINGERPRINT build: checksum, three moments of
+    the values weighted by hashes of their canonical quartet index, count) on
+    water7 and organic5 unscreened, organic5 and organic30 at Schwarz 1e-12.
+    The count is exact; the sums agree to 1e-12 of sum |x| (a value in the
+    wrong slot moves a moment by about its own size).  water7's unindexed
+    components are checked against the restated fingerprint of the
+    reference's recorded stream, to the bounds that the per-value tolerance
+    1e-14 + 1e-12 |x| implies."""
+    from oracle import oracle as O
+    from tlno_amd.fock import build_eri_packed_for_G_rys, EriPackedBuildOptions, build_eri_packed_plan_for_G_rys
+    f = load("fingerprint.npz")
+    cases = [("water7", configs.water7()[0], 0.0, "0"), ("organic5", configs.organic30(natoms=5)[0], 0.0, "0"),
+             ("organic5", configs.organic30(natoms=5)[0], 1e-12, "12"), ("organic30", configs.organic30()[0], 1e-12, "12")]
+    for name, b, thr, tag in cases:
+        plan = DirectPlan(b, ScreeningOptions(integral_threshold=thr))
+        fp = plan.eri_fingerprint(thr)
+        ref = f[f"{name}_{tag}"]
+        tol = 1e-12 * fp["absolute_sum"]
+        d = [abs(fp[k] - ref[i]) for i, k in enumerate(("checksum", "indexed_moment_1", "indexed_moment_2",
+                                                          "indexed_moment_3"))]
+        print(f"{name} at {thr}: {fp['canonical_count']} values (reference {int(ref[4])}); "
+              f"max |d| {max(d):.2e} of sum|x| {fp['absolute_sum']:.4e}")
+        assert fp["canonical_count"] == int(ref[4])
+        assert max(d) <= tol
+        assert fp["checksum"] == pytest.approx(plan.eri_checksum(thr), rel=1e-12)
+    g7 = load("water7.npz")
+    o7 = O.stream_fingerprint(g7["stream_lhs"], g7["stream_rhs"], g7["stream_val"])
+    b7, _ = configs.water7()
+    fp7 = DirectPlan(b7, ScreeningOptions(integral_threshold=0.0)).eri_fingerprint(0.0)
+    # per value |d| <= 1e-14 + 1e-12 |x| bounds the sums of |x| and x^2 and the maximum
+    n7 = o7["canonical_count"]
+    assert abs(fp7["absolute_sum"] - o7["absolute_sum"]) <= n7 * 1e-14 + 1e-12 * o7["absolute_sum"]
+    assert abs(fp7["square_sum"] - o7["square_sum"]) <= 2 * o7["max_abs"] * (n7 * 1e-14 + 1e-12 * o7["absolute_sum"])
+    assert abs(fp7["max_abs"] - o7["max_abs"]) <= 1e-14 + 1e-12 * o7["max_abs"]
+    # the packed endpoint's fingerprint form returns what the reference's build returns
+    p7 = build_eri_packed_plan_for_G_rys(b7, EriPackedBuildOptions(schwarz_threshold=0.0))
+    v = build_eri_packed_for_G_rys(p7, EriPackedBuildOptions(schwarz_threshold=0.0), fingerprint=True).values
+    assert len(v) == 5 and v[4] == 406.0
+    assert np.abs(np.array(v[:4]) - f["water7_0"][:4]).max() <= 1e-12 * fp7["absolute_sum"]
+    with pytest.raises(Exception, match="pure spherical"):
+        DirectPlan(b7, cartesian=True).eri_fingerprint(0.0)

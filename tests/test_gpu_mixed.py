@@ -1,6 +1,7 @@
 """GPU: mixed-precision J/K (FP32 recurrences for quartets with
 Q_ab Q_cd max|D| < 1e-6, FP64 roots and FP64 accumulation) against the FP64
-GPU build, which is itself checked against the oracle and the reference.
+GPU build, which is itself checked against the oracle and the reference, and
+on water10 against the oracle's FP64 J/K directly.
 
 Tolerance: max-abs J/K error <= 1e-7 (the target for the mixed route); the
 RMS error is reported.
@@ -35,6 +36,37 @@ def test_mixed_precision_error_bound(nwat):
     # the gate is off again: bit-identical to the first FP64 build
     J2, K2 = plan.jk(D)
     assert np.array_equal(J2, J0) or np.abs(J2 - J0).max() < 1e-13
+
+
+def test_mixed_and_learned_routes_against_the_oracle_water10():
+    """The mixed route, and learned screening over it, against the CPU
+    oracle's FP64 J/K of water10 (tests/golden/water10.npz, Cartesian,
+    Schwarz 1e-12): max-abs <= 1e-7 for both routes, <= 1e-10 for the plain
+    FP64 GPU build."""
+    import os
+    from tlno_amd import learned
+    from test_oracle import basis_hash
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "water10.npz"))
+    b, D = configs.water_cluster(10)
+    assert basis_hash(b) == str(g["basis_sha"])
+    assert abs(D.sum() - float(g["D_sum"])) <= 1e-12 and abs(np.linalg.norm(D) - float(g["D_fro"])) <= 1e-12
+    Jo, Ko = g["J"], g["K"]
+    plan = DirectPlan(b, ScreeningOptions(integral_threshold=1e-12), cartesian=True)
+    J0, K0 = plan.jk(D)
+    e0 = max(np.abs(J0 - Jo).max(), np.abs(K0 - Ko).max())
+    plan.set_mixed_precision(1e-6)
+    J1, K1 = plan.jk(D)
+    e1 = max(np.abs(J1 - Jo).max(), np.abs(K1 - Ko).max())
+    r1 = np.sqrt(np.mean(np.concatenate([(J1 - Jo).ravel(), (K1 - Ko).ravel()]) ** 2))
+    params, margin = learned.load_weights()
+    plan.set_learned_screen(params, margin)
+    J2, K2 = plan.jk(D, density_threshold=2e-9)
+    e2 = max(np.abs(J2 - Jo).max(), np.abs(K2 - Ko).max())
+    plan.set_learned_screen(None)
+    plan.set_mixed_precision(0.0)
+    print(f"water10 vs oracle: FP64 {e0:.2e}; mixed {e1:.2e} (rms {r1:.2e}); learned+mixed {e2:.2e}")
+    assert e0 <= 1e-10
+    assert e1 <= 1e-7 and e2 <= 1e-7
 
 
 def test_mixed_gate_zero_and_huge():

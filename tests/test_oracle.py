@@ -67,6 +67,18 @@ def test_generators_match_fixtures():
     assert basis_hash(configs.water7()[0]) == str(load("water7.npz")["basis_sha"])
     assert basis_hash(configs.organic30(natoms=5)[0]) == str(load("organic5.npz")["basis_sha"])
     assert basis_hash(configs.organic30()[0]) == str(load("organic30.npz")["basis_sha"])
+    b10, D10 = configs.water_cluster(10)
+    g10 = load("water10.npz")
+    assert basis_hash(b10) == str(g10["basis_sha"])
+    assert abs(D10.sum() - float(g10["D_sum"])) <= 1e-12 and abs(np.linalg.norm(D10) - float(g10["D_fro"])) <= 1e-12
+    g = load("batch256.npz")
+    mols = configs.batch256()
+    for i in g["pick"]:
+        assert basis_hash(mols[int(i)][0]) == str(g[f"sha{int(i)}"])
+    assert int(np.argmax([D.shape[0] for _, D in mols])) in set(int(i) for i in g["pick"])
+    f = load("fingerprint.npz")
+    assert basis_hash(configs.water7()[0]) == str(f["water7_sha"])
+    assert basis_hash(configs.organic30()[0]) == str(f["organic30_sha"])
 
 
 # ------------------------------------------------------------------- Boys
@@ -344,3 +356,42 @@ def test_sph_transform_pins_reference_numbering():
     S, _, _ = ob.onee(b.nuclear_charges, b.nuclear_centers, spherical=True)
     assert np.allclose(S, g["S1"], atol=1e-13)
     assert np.allclose(np.diag(S), 1.0, atol=1e-13)
+
+
+# ------------------------------------------------------- stream fingerprint
+def test_fingerprint_hash_known_answers():
+    """fp_mix is splitmix64's output function: its published first two
+    outputs from state 0 are 0xe220a8397b1dcdaf and 0x6e789e6aa1b965f4;
+    weights lie in [-1, 1) on the 2^-23 grid."""
+    assert int(O.fp_mix(0)) == 0xe220a8397b1dcdaf
+    assert int(O.fp_mix(0x9e3779b97f4a7c15)) == 0x6e789e6aa1b965f4
+    w = O.fp_weight(np.arange(100000, dtype=np.uint64), O.FP_SEEDS[0])
+    assert w.min() >= -1 and w.max() < 1
+    assert np.all(w * 8388608 == np.round(w * 8388608))
+    assert abs(float(w.mean())) < 0.01  # centred
+
+
+def test_fingerprint_restatement_vs_reference_water7():
+    """The restated fingerprint of the reference's own water7 stream (all 406
+    canonical ERIs, recorded from its production route) equals what the
+    reference's fingerprint build returns for water7."""
+    g = load("water7.npz")
+    f = load("fingerprint.npz")["water7_0"]
+    ours = O.stream_fingerprint(g["stream_lhs"], g["stream_rhs"], g["stream_val"])
+    assert ours["canonical_count"] == int(f[4]) == 406
+    for k, key in enumerate(("checksum", "indexed_moment_1", "indexed_moment_2", "indexed_moment_3")):
+        assert abs(ours[key] - f[k]) <= 1e-14 * ours["absolute_sum"], key
+
+
+def test_fingerprint_sees_a_value_in_the_wrong_slot():
+    """Planted in the oracle: exchanging two values of the stream keeps the
+    checksum and every unindexed component but moves the indexed moments."""
+    g = load("water7.npz")
+    val = g["stream_val"].copy()
+    a, b = int(np.argmax(np.abs(val))), int(np.argmin(np.abs(val)))
+    base = O.stream_fingerprint(g["stream_lhs"], g["stream_rhs"], val)
+    val[[a, b]] = val[[b, a]]
+    moved = O.stream_fingerprint(g["stream_lhs"], g["stream_rhs"], val)
+    for key in ("checksum", "absolute_sum", "square_sum", "max_abs"):
+        assert abs(moved[key] - base[key]) <= 1e-14 * base["absolute_sum"]
+    assert max(abs(moved[f"indexed_moment_{k}"] - base[f"indexed_moment_{k}"]) for k in (1, 2, 3)) > 1e-3
