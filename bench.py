@@ -403,13 +403,15 @@ def main():
                     help="skip the organic30 rerun with HIP's default 4 hardware queues")
     ap.add_argument("--routing-profile", type=int, default=0, metavar="BUILDS",
                     help="only run BUILDS learned+mixed water40 builds (the routing-kernel profile workload)")
+    ap.add_argument("--onee-profile", action="store_true",
+                    help="only run the S/T/V leg (organic30 and water100) and print it as one JSON line")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first copy a buffer of this many MiB device to device (a copy kernel of known traffic "
                          "for calibrating hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the J/K build shards by "
-              "quartet tasks with one all-reduce of J and K; see DESIGN.md)", file=sys.stderr)
+              "quartet tasks with one all-reduce of J and K, not built here)", file=sys.stderr)
         sys.exit(2)
     thr = 1e-12
     if args.copy_probe_mib > 0:
@@ -421,6 +423,9 @@ def main():
         dst.free()
     if args.routing_profile > 0:
         print(json.dumps(routing_profile_run(args.routing_profile)), flush=True)
+        return
+    if args.onee_profile:
+        print(json.dumps(onee_run()), flush=True)
         return
     b, D = configs.organic30() if args.workload == "organic30" else configs.water40()
     print(f"bench: {args.workload} J/K builds ({args.warmup} warm-up, {args.steps} timed)", file=sys.stderr,

@@ -55,7 +55,12 @@ struct TeamShape {
     static constexpr int NOUT = NA * NB * NC * ND;
     static constexpr int OPL = (NOUT + 63) / 64;
     static constexpr int GS = (LAB + 1) * (LCD + 1);       // one axis table
-    static constexpr int CS = 3 * GS + 1;                  // one combination (padded)
+    // one combination's three tables, padded to an odd number of doubles:
+    // phase A has every lane write the same table entry of its own
+    // combination, CS doubles apart, and an even stride maps lanes onto the
+    // same LDS banks ((pp|pp), (dd|ds), (dd|dd) lost 8-16 % of their time to
+    // bank conflicts with 3 GS + 1 even)
+    static constexpr int CS = (3 * GS) | 1;
     // combinations per batch: 64 (one per lane) while their tables fit in
     // TLNO_TEAM_LDS_KB of LDS, else 32 (the upper half-wave then evaluates
     // the z tables of the lower half's combinations), so that two waves per
@@ -202,6 +207,7 @@ __device__ __forceinline__ void eri_team_quartet(const JKArgs& a, double* __rest
         // ---- phase B: accumulate the owned intermediates over this part's
         // combinations of the batch (three LDS rows per combination)
         const int nb = min(T::CB, ncomb - c0);
+#pragma unroll 2
         for (int j = part; j < nb; j += T::NP) {
             const double* g = lds + j * T::CS;
             double rx[T::LCD + 1], ry[T::LCD + 1], rz[T::LCD + 1];
