@@ -1864,6 +1864,19 @@ const int2* ensure_team_quartets(tlno_gpu_plan* pl, TaskSet& ts, int X, int Y) {
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(pl->stream));
     HIP_OK(hipFree(d_cum));
+    // heaviest quartets first (primitive quartets, stable): a wave lasts as
+    // long as its quartet's combinations, which range over three orders of
+    // magnitude where contraction lengths differ, and waves start in list
+    // order, so the longest ones no longer trail at the end of the launch.
+    // The team evaluator always takes a class's whole list (it keeps the
+    // FP32 share of a mixed build too), so the order is free.
+    std::vector<int2> h(static_cast<std::size_t>(cum.back()));
+    HIP_OK(hipMemcpy(h.data(), ts.d_tq[X][Y], h.size() * sizeof(int2), hipMemcpyDeviceToHost));
+    std::stable_sort(h.begin(), h.end(), [&](const int2& u, const int2& v) {
+        return static_cast<long long>(pl->pn_host[u.x]) * pl->pn_host[u.y] >
+               static_cast<long long>(pl->pn_host[v.x]) * pl->pn_host[v.y];
+    });
+    HIP_OK(hipMemcpy(ts.d_tq[X][Y], h.data(), h.size() * sizeof(int2), hipMemcpyHostToDevice));
     return ts.d_tq[X][Y];
 }
 
