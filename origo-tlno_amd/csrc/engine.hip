@@ -322,6 +322,7 @@ struct TaskSet {
     // 64 kets of one group; lcnt = 0 where the range tasks are used.
     int* d_bk = nullptr;
     int4* d_ltasks = nullptr;
+    int4* d_ltasks_lpt = nullptr;  // the same list tasks by descending cost (inside d_ltasks' allocation)
     long long loff[6][6] = {}, lcnt[6][6] = {};
     std::vector<double> lbmax[6][6];
     // per-quartet routing (density test, learned screen): rows of at most
@@ -1541,7 +1542,7 @@ void bucket_list_tasks(const tlno_gpu_plan* pl, double thr, int X, int Y, const 
 // serialized launches 15-40 % shorter.
 void balance_list_slices(const tlno_gpu_plan* pl, const std::vector<int>& bk, int X, int Y,
                          const std::vector<int4>& in, const std::vector<double>& in_bound, std::vector<int4>& out,
-                         std::vector<double>& out_bound) {
+                         std::vector<double>& out_bound, std::vector<double>& out_cost) {
     const bool flat = flat_prims(kClassLAB[X], 0, kClassLAB[Y], 0);
     const long long nchunk = std::max(1, g_kernels[X][Y].nchunk);
     const std::size_t n = in.size();
@@ -1603,6 +1604,8 @@ void balance_list_slices(const tlno_gpu_plan* pl, const std::vector<int>& bk, in
             t.w = i0 | (i1 << 16);
             out.push_back(t);
             out_bound.push_back(in_bound[i]);
+            const double nbs = i1 - i0;
+            out_cost.push_back(flat ? std::ceil(nbs * pkn[i] / ibs[i]) : std::ceil(nbs / ibs[i]) * pkn[i]);
         }
     }
 }
@@ -1703,7 +1706,8 @@ TaskSet& get_tasks(tlno_gpu_plan* pl, double thr) {
             kgroups[c].push_back({b, static_cast<int>(bk.size())});
         }
     }
-    std::vector<int4> lall;
+    std::vector<int4> lall, lpt;
+    std::vector<double> lcost;
     for (int X = 0; X < 6; ++X)
         for (int Y = 0; Y <= X; ++Y) {
             ts.loff[X][Y] = static_cast<long long>(lall.size());
@@ -1711,15 +1715,23 @@ TaskSet& get_tasks(tlno_gpu_plan* pl, double thr) {
                 std::vector<int4> cls;
                 std::vector<double> bound;
                 bucket_list_tasks(pl, thr, X, Y, bk, kgroups[Y], cls, bound);
-                balance_list_slices(pl, bk, X, Y, cls, bound, lall, ts.lbmax[X][Y]);
+                balance_list_slices(pl, bk, X, Y, cls, bound, lall, ts.lbmax[X][Y], lcost);
             }
             ts.lcnt[X][Y] = static_cast<long long>(lall.size()) - ts.loff[X][Y];
+            // the same tasks heaviest first, for builds that run the class's
+            // whole list on one kernel (no FP32 suffix to keep at the end)
+            std::vector<std::size_t> ord(static_cast<std::size_t>(ts.lcnt[X][Y]));
+            for (std::size_t i = 0; i < ord.size(); ++i) ord[i] = static_cast<std::size_t>(ts.loff[X][Y]) + i;
+            std::stable_sort(ord.begin(), ord.end(), [&](std::size_t u, std::size_t v) { return lcost[u] > lcost[v]; });
+            for (std::size_t i : ord) lpt.push_back(lall[i]);
         }
     if (!lall.empty()) {
         HIP_OK(hipMalloc(&ts.d_bk, bk.size() * sizeof(int)));
         HIP_OK(hipMemcpy(ts.d_bk, bk.data(), bk.size() * sizeof(int), hipMemcpyHostToDevice));
-        HIP_OK(hipMalloc(&ts.d_ltasks, lall.size() * sizeof(int4)));
+        HIP_OK(hipMalloc(&ts.d_ltasks, 2 * lall.size() * sizeof(int4)));
         HIP_OK(hipMemcpy(ts.d_ltasks, lall.data(), lall.size() * sizeof(int4), hipMemcpyHostToDevice));
+        ts.d_ltasks_lpt = ts.d_ltasks + lall.size();
+        HIP_OK(hipMemcpy(ts.d_ltasks_lpt, lpt.data(), lpt.size() * sizeof(int4), hipMemcpyHostToDevice));
     }
     return pl->tasks.emplace(thr, ts).first->second;
 }
@@ -2105,13 +2117,15 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
                     al.cut = nullptr;
                     al.qlist = ts.d_bk;
                     if (n64 > 0) {
-                        al.tasks = ts.d_ltasks + ts.loff[X][Y];
+                        // a whole FP64 list runs heaviest slices first (the
+                        // launch's tail is its last-started waves)
+                        al.tasks = (n64 == nt ? ts.d_ltasks_lpt : ts.d_ltasks) + ts.loff[X][Y];
                         al.ntasks = static_cast<int>(n64);
                         al.split = 0;  // the tasks carry their cost-balanced slices
                         add_jobs(X, Y, false, al, n64, class_team_min(pl, X, Y), c * n64);
                     }
                     if (nt > n64) {
-                        al.tasks = ts.d_ltasks + ts.loff[X][Y] + n64;
+                        al.tasks = (n64 == 0 ? ts.d_ltasks_lpt : ts.d_ltasks + n64) + ts.loff[X][Y];
                         al.ntasks = static_cast<int>(nt - n64);
                         al.split = 0;
                         add_jobs(X, Y, true, al, nt - n64, 0, c * (nt - n64));
