@@ -62,9 +62,10 @@ int tlno_gpu_plan_info(const tlno_gpu_plan* plan, int* M, int* n_groups,
 int tlno_gpu_plan_groups(const tlno_gpu_plan* plan, int max_groups, int* L, int* comps);
 
 /* Shell pairs of the plan in the plan's order (class ss, ps, pp, ds, dp, dd,
- * then descending Schwarz bound): bra/ket-side group of each pair (higher L
- * first), Schwarz bound Q (sqrt max|(ab|ab)| in the plan's basis), and the
- * first pair index of each class, class_begin[0..6].  Any pointer may be
+ * fs, fp, fd, ff, then descending Schwarz bound): bra/ket-side group of each
+ * pair (higher L first), Schwarz bound Q (sqrt max|(ab|ab)| in the plan's
+ * basis), and the first pair index of each class, class_begin[0..10]
+ * (class_begin[10] = number of pairs).  Any pointer may be
  * NULL.  Counterpart of the reference's pair-group list and pair_group_max
  * (rys_engine.ipp:2436-2460, 16164-16258). */
 int tlno_gpu_plan_pairs(const tlno_gpu_plan* plan, int* group_a, int* group_b, double* Q,

@@ -57,11 +57,15 @@ thread_local std::string g_err;
 #endif
 constexpr int kSideStreams = tlno_amd_side_streams;
 
-// total angular momentum and Cartesian components of the six pair classes
-// ss, ps, pp, ds, dp, dd
-constexpr int kClassLAB[6] = {0, 1, 2, 2, 3, 4};
-constexpr int kClassNcart[6] = {1, 3, 9, 6, 18, 36};
-constexpr int kClassL[6][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
+// total angular momentum and Cartesian components of the pair classes ss,
+// ps, pp, ds, dp, dd (lane and team evaluators) and fs, fp, fd, ff (team
+// evaluator only, their pairs after all others in plan order)
+constexpr int kClassLAB[kNumClasses] = {0, 1, 2, 2, 3, 4, 3, 4, 5, 6};
+constexpr int kClassNcart[kNumClasses] = {1, 3, 9, 6, 18, 36, 10, 30, 60, 100};
+constexpr int kClassL[kNumClasses][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1},
+                                         {2, 2}, {3, 0}, {3, 1}, {3, 2}, {3, 3}};
+constexpr int kNumHigh = kNumClasses - kNumLowClasses;
+constexpr int kMaxComp = ncart(kMaxShellL);  // Cartesian components of the largest shell
 
 int side_streams_for_queues() {
     int q = 4;  // HIP's default
@@ -139,6 +143,7 @@ void ensure_side_streams(DeviceCtx& c) {
 }
 
 ClassKernels g_kernels[6][6];
+HighKernels g_high[kNumHigh][kNumClasses];
 std::once_flag g_kernels_once;
 std::mutex g_dev_mutex;
 DeviceCtx g_dev[16];
@@ -154,6 +159,13 @@ void register_all_kernels() {
     register_kernels_bra5(g_kernels);
     register_kernels_bra5b(g_kernels);
     register_kernels_bra5c(g_kernels);
+    register_kernels_hi6(g_high);
+    register_kernels_hi7(g_high);
+    register_kernels_hi8a(g_high);
+    register_kernels_hi8b(g_high);
+    register_kernels_hi9a(g_high);
+    register_kernels_hi9b(g_high);
+    register_kernels_hi9c(g_high);
 }
 
 DeviceCtx& device_ctx(int dev) {
@@ -184,7 +196,7 @@ DeviceCtx& device_ctx(int dev) {
 // normalized in the unit-diagonal metric.  Row s, column = local Cartesian
 // component (descending lex).
 struct C2S {
-    double c[3][5][6] = {};  // L <= 2
+    double c[kMaxShellL + 1][2 * kMaxShellL + 1][kMaxComp] = {};
 };
 
 double dfact(int n) {
@@ -214,8 +226,8 @@ const C2S& c2s_table() {
     static C2S tab = [] {
         C2S out;
         // polynomials over monomials of degree l, index local_index
-        std::vector<std::vector<std::vector<double>>> S(3);
-        for (int l = 0; l <= 2; ++l)
+        std::vector<std::vector<std::vector<double>>> S(kMaxShellL + 1);
+        for (int l = 0; l <= kMaxShellL; ++l)
             S[l].assign(2 * l + 1, std::vector<double>(static_cast<std::size_t>(ncart(l)), 0.0));
         S[0][0][0] = 1.0;
         auto mul_axis = [](const std::vector<double>& a, int L, int axis, double s, std::vector<double>& o) {
@@ -239,7 +251,7 @@ const C2S& c2s_table() {
                 }
             }
         };
-        for (int l = 0; l < 2; ++l) {
+        for (int l = 0; l < kMaxShellL; ++l) {
             const double f = std::sqrt((l == 0 ? 2.0 : 1.0) * (2.0 * l + 1.0) / (2.0 * l + 2.0));
             mul_axis(S[l][2 * l], l, 0, f, S[l + 1][2 * l + 2]);
             if (l > 0) mul_axis(S[l][0], l, 1, -f, S[l + 1][2 * l + 2]);
@@ -253,12 +265,12 @@ const C2S& c2s_table() {
                            S[l + 1][m + l + 1]);
             }
         }
-        for (int L = 0; L <= 2; ++L) {
+        for (int L = 0; L <= kMaxShellL; ++L) {
             const int nc = ncart(L);
             for (int s = 0; s < 2 * L + 1; ++s) {
                 const int m = (L == 1) ? (s == 0 ? 1 : (s == 1 ? -1 : 0)) : s - L;
                 const auto& P = S[L][m + L];
-                double row[6];
+                double row[kMaxComp];
                 for (int c = 0; c < nc; ++c) {
                     int t[3];
                     local_tuple(L, c, t);
@@ -340,6 +352,10 @@ struct TaskSet {
     // of its range tasks in task order (cum_sq gives the prefix of the first
     // n tasks); built on the first team launch of the class
     int2* d_tq[6][6] = {};
+    // quartets with an f shell (team evaluator only): per class pair
+    // (X >= 6, Y <= X) the Schwarz-surviving {P, Q}, heaviest first
+    int2* d_hq = nullptr;
+    long long hoff[kNumHigh][kNumClasses] = {}, hcnt[kNumHigh][kNumClasses] = {};
 };
 
 }  // namespace tlno_amd
@@ -351,7 +367,7 @@ struct tlno_gpu_plan {
     std::vector<tlno_amd::GroupH> groups;
     std::vector<int> int2basis;
     int npair = 0;
-    int cls_begin[7] = {0};
+    int cls_begin[tlno_amd::kNumClasses + 1] = {0};
     std::vector<int> pgA, pgB;
     std::vector<double> Q;
     long long nprim_pairs = 0;
@@ -391,7 +407,7 @@ struct tlno_gpu_plan {
     int *d_pgA = nullptr, *d_pgB = nullptr, *d_pcls = nullptr, *d_clsb = nullptr;
     double* d_Q = nullptr;                   // [npair] bounds in plan order
     unsigned long long* d_gmax = nullptr;    // [ngroups]
-    unsigned long long* d_clsmax = nullptr;  // [6]
+    unsigned long long* d_clsmax = nullptr;  // [kNumClasses]
     double* d_pairfac = nullptr;             // [npair]
     int* d_cut = nullptr;                    // [6 npair]
     unsigned long long* d_counter = nullptr; // shell quartets evaluated by the last build
@@ -401,7 +417,7 @@ struct tlno_gpu_plan {
     // per-quartet routing: pair features, shell rows, shell-block max |D|,
     // learned-screen operands, and the work lists the routing kernel fills
     float4 *d_pf0 = nullptr, *d_pf1 = nullptr, *d_pcen = nullptr;
-    int* d_grows = nullptr;                  // [ngroups][6] rows of each shell in the caller's basis (-1 pad)
+    int* d_grows = nullptr;                  // [ngroups][kMaxComp] rows of each shell in the caller's basis (-1 pad)
     double* d_blk = nullptr;                 // [ngroups^2] max |D| over each shell block
     double* d_bsum = nullptr;                // [ngroups^2] sum |D| over each shell block
     int learned = 0;
@@ -431,7 +447,7 @@ struct tlno_gpu_plan {
     // work buffers, kept between calls
     double onee_decay = -1.0;
     std::vector<int2> onee_list;
-    int onee_cb[7] = {0};
+    int onee_cb[tlno_amd::kNumClasses + 1] = {0};
     long long onee_kept = 0;
     int2* d_onee_list = nullptr;
     double* d_onee_X = nullptr;  // 3 Mcart^2 internal-order S, T, V
@@ -472,7 +488,7 @@ bool finite(double v) { return std::isfinite(v); }
 
 int class_of(int LA, int LB) {
     // LA >= LB
-    static const int t[3][3] = {{0, -1, -1}, {1, 2, -1}, {3, 4, 5}};
+    static const int t[4][4] = {{0, -1, -1, -1}, {1, 2, -1, -1}, {3, 4, 5, -1}, {6, 7, 8, 9}};
     return t[LA][LB];
 }
 
@@ -501,9 +517,9 @@ std::vector<GroupH> build_groups(int n, const double* centers, const int* ang, c
                     "eri_ao_tensor_hgp: primitive exponents must be finite and positive and coefficients "
                     "must be finite.");
         const int L = a[0] + a[1] + a[2];
-        if (L > 2)
+        if (L > kMaxShellL)
             throw std::runtime_error(
-                "eri_ao_tensor_hgp: this build supports shell L=0..2 (s, p, d).");
+                "eri_ao_tensor_hgp: this build supports shell L=0..3 (s, p, d, f).");
         const double na = 1.0 / std::sqrt(dfact(a[0]) * dfact(a[1]) * dfact(a[2]));
         int gi = -1;
         for (int g = 0; g < static_cast<int>(groups.size()) && gi < 0; ++g) {
@@ -551,7 +567,7 @@ std::vector<GroupH> build_groups(int n, const double* centers, const int* ang, c
 
 // ------------------------------------------------------ transform kernels --
 struct C2SArgs {
-    double c[3][5][6];
+    double c[kMaxShellL + 1][2 * kMaxShellL + 1][kMaxComp];
 };
 
 // D_int[i][j] = D_user[perm[i]][perm[j]]  (Cartesian mode)
@@ -744,6 +760,20 @@ void launch_jk(const ClassKernels& k, bool fp32, JKArgs& args, long long nwaves,
     args.nwork = saved_nwork;
 }
 
+// team kernel of a class pair with an f shell over its quartet list (one
+// 64-thread block per quartet and slice; capped grids stride)
+void launch_high(const HighKernels& k, JKArgs& args, hipStream_t st) {
+    void* params[] = {&args};
+    const long long ns = std::max(1, args.team_slices);
+    const long long work = args.ntquart * ns;
+    if (work <= 0) return;
+    const long long cap = max_grid_waves();
+    args.stride = work > cap ? static_cast<int>(cap) : 0;
+    HIP_OK(hipLaunchKernel(reinterpret_cast<const void*>(k.team), dim3(static_cast<unsigned>(std::min(work, cap))),
+                           dim3(64), params, 0, st));
+    args.stride = 0;
+}
+
 // ---- density screen (per build) ----
 // gmax[g] = max |D[i][j]| over the rows i of shell g (caller's basis)
 __global__ void k_group_rowmax(const double* __restrict__ D, const int* row_group, int M,
@@ -826,11 +856,11 @@ __global__ void k_block_max(const double* __restrict__ D, const int* __restrict_
     if (t >= static_cast<long long>(ng) * ng) return;
     const int gi = static_cast<int>(t / ng), gj = static_cast<int>(t - static_cast<long long>(gi) * ng);
     double m = 0.0, sum = 0.0;
-    for (int a = 0; a < 6; ++a) {
-        const int ra = grows[6 * gi + a];
+    for (int a = 0; a < kMaxComp; ++a) {
+        const int ra = grows[kMaxComp * gi + a];
         if (ra < 0) break;
-        for (int b = 0; b < 6; ++b) {
-            const int rb = grows[6 * gj + b];
+        for (int b = 0; b < kMaxComp; ++b) {
+            const int rb = grows[kMaxComp * gj + b];
             if (rb < 0) break;
             const double v = fmax(fabs(D[static_cast<long long>(ra) * M + rb]), fabs(D[static_cast<long long>(rb) * M + ra]));
             m = fmax(m, v);
@@ -1110,8 +1140,15 @@ void run_store(tlno_gpu_plan* pl, int X, int Y, const std::vector<int2>& qs, con
     a.off = doff;
     a.n = static_cast<int>(qs.size());
     a.out = d_out;
-    const ClassKernels& k = g_kernels[X][Y];
-    launch_store(k, a, pl->stream);
+    if (X >= kNumLowClasses) {
+        // classes with an f shell: team-evaluator blocks, one wave per quartet
+        void* params[] = {&a};
+        const long long nb = std::min<long long>(a.n, max_grid_waves());
+        HIP_OK(hipLaunchKernel(reinterpret_cast<const void*>(g_high[X - kNumLowClasses][Y].store),
+                               dim3(static_cast<unsigned>(nb)), dim3(64), params, 0, pl->stream));
+    } else {
+        launch_store(g_kernels[X][Y], a, pl->stream);
+    }
     HIP_OK(hipStreamSynchronize(pl->stream));
     HIP_OK(hipFree(dq));
     HIP_OK(hipFree(doff));
@@ -1316,16 +1353,16 @@ void build_plan(tlno_gpu_plan* pl, int n, const double* centers, const int* ang,
     for (int k = 0; k < np; ++k) cls_sorted[k] = pcls[order[k]];
     const std::vector<int> pgA0 = pl->pgA, pgB0 = pl->pgB;
     upload_pairs(pl, order, prims_P, prims_w, prims_ab);
-    for (int c = 0; c <= 6; ++c)
+    for (int c = 0; c <= kNumClasses; ++c)
         pl->cls_begin[c] =
             static_cast<int>(std::lower_bound(cls_sorted.begin(), cls_sorted.end(), c) - cls_sorted.begin());
 
     // Schwarz bounds from the diagonal blocks (ab|ab), on the GPU
     std::vector<double> Q(np, 0.0);
-    for (int X = 0; X < 6; ++X) {
+    for (int X = 0; X < kNumClasses; ++X) {
         const int b0 = pl->cls_begin[X], b1 = pl->cls_begin[X + 1];
         if (b1 <= b0) continue;
-        const int LA = (X == 0 ? 0 : (X <= 2 ? 1 : 2)), LB = (X == 0 || X == 1 || X == 3) ? 0 : (X == 2 || X == 4 ? 1 : 2);
+        const int LA = kClassL[X][0], LB = kClassL[X][1];
         const int na = ncart(LA), nb = ncart(LB), nab = na * nb;
         std::vector<int2> qs;
         std::vector<long long> off;
@@ -1371,7 +1408,7 @@ void build_plan(tlno_gpu_plan* pl, int n, const double* centers, const int* ang,
     // sort inside each class by descending bound
     std::vector<int> order2(np);
     for (int i = 0; i < np; ++i) order2[i] = i;
-    for (int c = 0; c < 6; ++c)
+    for (int c = 0; c < kNumClasses; ++c)
         std::stable_sort(order2.begin() + pl->cls_begin[c], order2.begin() + pl->cls_begin[c + 1],
                          [&](int x, int y) { return Q[x] > Q[y]; });
     // compose with the first ordering
@@ -1418,7 +1455,7 @@ void build_plan(tlno_gpu_plan* pl, int n, const double* centers, const int* ang,
     upload_pair_features(pl, final_order, prims_P, prims_w, prims_ab);
     // density-screen metadata
     {
-        std::vector<int> row_group(pl->Mout), pcls(np), clsb(7);
+        std::vector<int> row_group(pl->Mout), pcls(np), clsb(kNumClasses + 1);
         for (int g = 0; g < ng; ++g) {
             const GroupH& G = pl->groups[g];
             if (pl->spherical)
@@ -1426,17 +1463,17 @@ void build_plan(tlno_gpu_plan* pl, int n, const double* centers, const int* ang,
             else
                 for (int k = 0; k < ncart(G.L); ++k) row_group[G.comp[k]] = g;
         }
-        for (int c = 0; c < 6; ++c)
+        for (int c = 0; c < kNumClasses; ++c)
             for (int P = pl->cls_begin[c]; P < pl->cls_begin[c + 1]; ++P) pcls[P] = c;
-        for (int c = 0; c <= 6; ++c) clsb[c] = pl->cls_begin[c];
+        for (int c = 0; c <= kNumClasses; ++c) clsb[c] = pl->cls_begin[c];
         dev_upload(pl->d_row_group, row_group);
-        std::vector<int> grows(6 * static_cast<std::size_t>(std::max(1, ng)), -1);
+        std::vector<int> grows(kMaxComp * static_cast<std::size_t>(std::max(1, ng)), -1);
         for (int g = 0; g < ng; ++g) {
             const GroupH& G = pl->groups[g];
             if (pl->spherical)
-                for (int s = 0; s < 2 * G.L + 1; ++s) grows[6 * g + s] = G.sph_base + s;
+                for (int s = 0; s < 2 * G.L + 1; ++s) grows[kMaxComp * g + s] = G.sph_base + s;
             else
-                for (int k = 0; k < ncart(G.L); ++k) grows[6 * g + k] = G.comp[k];
+                for (int k = 0; k < ncart(G.L); ++k) grows[kMaxComp * g + k] = G.comp[k];
         }
         dev_upload(pl->d_grows, grows);
         HIP_OK(hipMalloc(&pl->d_blk, std::max<std::size_t>(1, static_cast<std::size_t>(ng) * ng) * sizeof(double)));
@@ -1448,7 +1485,7 @@ void build_plan(tlno_gpu_plan* pl, int n, const double* centers, const int* ang,
         dev_upload(pl->d_clsb, clsb);
         dev_upload(pl->d_Q, pl->Q);
         HIP_OK(hipMalloc(&pl->d_gmax, std::max(1, ng) * sizeof(unsigned long long)));
-        HIP_OK(hipMalloc(&pl->d_clsmax, 8 * sizeof(unsigned long long)));
+        HIP_OK(hipMalloc(&pl->d_clsmax, 16 * sizeof(unsigned long long)));
         HIP_OK(hipMalloc(&pl->d_pairfac, std::max(1, np) * sizeof(double)));
         HIP_OK(hipMalloc(&pl->d_cut, 6 * static_cast<std::size_t>(std::max(1, np)) * sizeof(int)));
         HIP_OK(hipMalloc(&pl->d_counter, 4 * kCounterSlots * sizeof(unsigned long long)));
@@ -1733,6 +1770,46 @@ TaskSet& get_tasks(tlno_gpu_plan* pl, double thr) {
         ts.d_ltasks_lpt = ts.d_ltasks + lall.size();
         HIP_OK(hipMemcpy(ts.d_ltasks_lpt, lpt.data(), lpt.size() * sizeof(int4), hipMemcpyHostToDevice));
     }
+    // Quartets with an f shell: the bra is the pair of the higher class (the
+    // later one in plan order when both are of one class), the kets surviving
+    // Schwarz a prefix of the bound-ordered ket class; one list per class
+    // pair, heaviest quartets first (the team evaluator's order)
+    std::vector<int2> hall;
+    for (int X = kNumLowClasses; X < kNumClasses; ++X)
+        for (int Y = 0; Y <= X; ++Y) {
+            const int h = X - kNumLowClasses;
+            ts.hoff[h][Y] = static_cast<long long>(hall.size());
+            const int k0 = pl->cls_begin[Y], k1 = pl->cls_begin[Y + 1];
+            std::vector<int2> cls;
+            for (int P = pl->cls_begin[X]; P < pl->cls_begin[X + 1] && k1 > k0; ++P) {
+                int end = k1;
+                if (thr > 0.0) {
+                    int lo = k0, hi = k1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) / 2;
+                        if (pl->Q[P] * pl->Q[mid] >= thr) lo = mid + 1;
+                        else hi = mid;
+                    }
+                    end = lo;
+                }
+                if (X == Y) end = std::min(end, P + 1);
+                for (int Q = k0; Q < end; ++Q) cls.push_back(make_int2(P, Q));
+                if (end > k0) {
+                    ts.shell_quartets += end - k0;
+                    ts.prim_quartets += (pref[end] - pref[k0]) * pl->pn_host[P];
+                }
+            }
+            std::stable_sort(cls.begin(), cls.end(), [&](const int2& u, const int2& v) {
+                return static_cast<long long>(pl->pn_host[u.x]) * pl->pn_host[u.y] >
+                       static_cast<long long>(pl->pn_host[v.x]) * pl->pn_host[v.y];
+            });
+            hall.insert(hall.end(), cls.begin(), cls.end());
+            ts.hcnt[h][Y] = static_cast<long long>(hall.size()) - ts.hoff[h][Y];
+        }
+    if (!hall.empty()) {
+        HIP_OK(hipMalloc(&ts.d_hq, hall.size() * sizeof(int2)));
+        HIP_OK(hipMemcpy(ts.d_hq, hall.data(), hall.size() * sizeof(int2), hipMemcpyHostToDevice));
+    }
     return pl->tasks.emplace(thr, ts).first->second;
 }
 
@@ -1949,13 +2026,14 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
     if (pl->density_thr > 0.0 && (a.do_j || a.do_k)) {
         const int ng = static_cast<int>(pl->groups.size());
         k_fill<<<grid_for(ng, 256), 256, 0, st>>>(reinterpret_cast<double*>(pl->d_gmax), ng, 0.0);
-        k_fill<<<1, 64, 0, st>>>(reinterpret_cast<double*>(pl->d_clsmax), 8, 0.0);
+        k_fill<<<1, 64, 0, st>>>(reinterpret_cast<double*>(pl->d_clsmax), 16, 0.0);
         k_group_rowmax<<<pl->Mout, 256, 0, st>>>(dD, pl->d_row_group, pl->Mout, pl->d_gmax);
         k_pair_factor<<<grid_for(pl->npair, 256), 256, 0, st>>>(pl->d_gmax, pl->d_pgA, pl->d_pgB, pl->d_pcls,
                                                                 pl->npair, pl->d_pairfac, pl->d_clsmax);
-        k_density_cut<<<grid_for(pl->npair, 256), 256, 0, st>>>(pl->d_Q, pl->d_pcls, pl->d_clsb, pl->d_pairfac,
-                                                                pl->d_clsmax, pl->npair, pl->density_thr,
-                                                                pl->d_cut);
+        const int nlow = pl->cls_begin[kNumLowClasses];  // f pairs: per-quartet test in the team kernel
+        if (nlow > 0)
+            k_density_cut<<<grid_for(nlow, 256), 256, 0, st>>>(pl->d_Q, pl->d_pcls, pl->d_clsb, pl->d_pairfac,
+                                                               pl->d_clsmax, nlow, pl->density_thr, pl->d_cut);
         const long long ng2 = static_cast<long long>(ng) * ng;
         if (ng2 > 0)
             k_block_max<<<grid_for(ng2, 256), 256, 0, st>>>(dD, pl->d_grows, ng, pl->Mout, pl->d_blk, pl->d_bsum);
@@ -2051,18 +2129,21 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
             JKArgs args;
             long long nwaves;
             double cost;
+            bool high;  // class pair with an f shell (team kernel of g_high)
         };
-        // at most two jobs (FP64 and FP32 share) per class pair; a fixed
-        // array keeps the host side free of heap allocations as well
-        Job jobs[72];
+        // at most two jobs (FP64 and FP32 share) per class pair of shells up
+        // to d, one per class pair with an f shell; a fixed array keeps the
+        // host side free of heap allocations as well
+        constexpr std::size_t kMaxJobs = 72 + kNumHigh * kNumClasses;
+        Job jobs[kMaxJobs];
         std::size_t njobs = 0;
         auto add_jobs = [&](int X, int Y, bool fp32, const JKArgs& args, long long nwaves, int team_min,
                             double cost) {
             // the FP32 share of a class whose FP32 kernels measured slower
             // runs the FP64 kernels on the same tasks
             if (fp32 && pl->fp32_slower[X][Y]) fp32 = false;
-            if (njobs == 72) throw std::runtime_error("tlno_gpu: too many class launches");
-            jobs[njobs++] = Job{X, Y, team_min, fp32, args, nwaves, cost};
+            if (njobs == kMaxJobs) throw std::runtime_error("tlno_gpu: too many class launches");
+            jobs[njobs++] = Job{X, Y, team_min, fp32, args, nwaves, cost, false};
         };
         for (int X = 5; X >= 0 && route; --X)
             for (int Y = X; Y >= 0; --Y) {
@@ -2167,6 +2248,42 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
                     add_jobs(X, Y, true, a, nt - n64, 0, c * (nt - n64));
                 }
             }
+        // Class pairs with an f shell: the team evaluator over the plan's
+        // quartet lists, in FP64, with the density and decay screens applied
+        // per quartet inside the kernel (the routing kernel and the learned
+        // screen's network cover shells up to d; these quartets take the
+        // density rule alone)
+        for (int X = kNumClasses - 1; X >= kNumLowClasses; --X)
+            for (int Y = X; Y >= 0; --Y) {
+                const int h = X - kNumLowClasses;
+                const long long n = ts.hcnt[h][Y];
+                if (n == 0) continue;
+                JKArgs ah = a;
+                ah.tasks = nullptr;
+                ah.ntasks = 0;
+                ah.qlist = nullptr;
+                ah.cut = nullptr;
+                ah.split = 1;
+                ah.tquart = ts.d_hq + ts.hoff[h][Y];
+                ah.ntquart = n;
+                ah.team_slices = team_slices_for(n);
+                if (pl->density_thr > 0.0) {
+                    ah.sblk = pl->d_blk;
+                    ah.spgA = pl->d_pgA;
+                    ah.spgB = pl->d_pgB;
+                    ah.sQ = pl->d_Q;
+                    ah.sng = static_cast<int>(pl->groups.size());
+                    ah.smode = (a.do_j && a.do_k) ? 0 : (a.do_j ? 1 : 2);
+                    ah.stau = pl->density_thr;
+                }
+                if (pl->decay_thr > 0.0) {
+                    ah.sdecay = pl->d_decay;
+                    ah.sdecay_thr = pl->decay_thr;
+                }
+                if (njobs == kMaxJobs) throw std::runtime_error("tlno_gpu: too many class launches");
+                const double cost = 1e-6 * static_cast<double>(n) * kClassNcart[X] * kClassNcart[Y];
+                jobs[njobs++] = Job{X, Y, 1, false, ah, n, cost, true};
+            }
         if (concurrent) {
             // each job onto the side stream with the least estimated work so
             // far (list scheduling in class order; ordering by cost or
@@ -2179,12 +2296,19 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
                 for (int q = 1; q < ctx.nside; ++q)
                     if (load[q] < load[best]) best = q;
                 load[best] += j.cost;
-                launch_jk(g_kernels[j.X][j.Y], j.fp32, j.args, j.nwaves, ctx.side[best], j.team_min);
+                if (j.high)
+                    launch_high(g_high[j.X - kNumLowClasses][j.Y], j.args, ctx.side[best]);
+                else
+                    launch_jk(g_kernels[j.X][j.Y], j.fp32, j.args, j.nwaves, ctx.side[best], j.team_min);
             }
         } else {
             // timing mode: one stream, class order, events around each class's jobs
             for (std::size_t i = 0; i < njobs; ++i) {
                 Job& j = jobs[i];
+                if (j.high) {  // not part of the per-class timing table
+                    launch_high(g_high[j.X - kNumLowClasses][j.Y], j.args, st);
+                    continue;
+                }
                 const bool first = i == 0 || jobs[i - 1].X != j.X || jobs[i - 1].Y != j.Y;
                 const bool last = i + 1 == njobs || jobs[i + 1].X != j.X || jobs[i + 1].Y != j.Y;
                 if (first) HIP_OK(hipEventRecord(pl->ev[j.X][j.Y][0], st));
@@ -2317,11 +2441,11 @@ double eri_checksum(tlno_gpu_plan* pl, double thr) {
         if (i < j) std::swap(i, j);
         return i * (i + 1) / 2 + j;
     };
-    for (int X = 0; X < 6; ++X) {
+    for (int X = 0; X < kNumClasses; ++X) {
         const int b0 = pl->cls_begin[X], b1 = pl->cls_begin[X + 1];
         std::vector<int2> qs;
         std::vector<long long> off;
-        const int LA = (X == 0 ? 0 : (X <= 2 ? 1 : 2)), LB = (X == 0 || X == 1 || X == 3) ? 0 : (X == 2 || X == 4 ? 1 : 2);
+        const int LA = kClassL[X][0], LB = kClassL[X][1];
         const int na = ncart(LA), nb = ncart(LB), nab = na * nb;
         for (int P = b0; P < b1; ++P)
             if (pl->Q[P] * pl->Q[P] >= thr) {
@@ -2377,6 +2501,62 @@ double eri_checksum(tlno_gpu_plan* pl, double thr) {
                 }
         }
     }
+    // quartets with an f shell and P != Q: Cartesian blocks from the team
+    // kernel, folded on the host with the same pair weights
+    auto pair_weights = [&](int P, std::vector<double>& w) {
+        const int LA = pl->groups[pl->pgA[P]].L, LB = pl->groups[pl->pgB[P]].L;
+        const int na = ncart(LA), nb = ncart(LB);
+        const bool same = pl->pgA[P] == pl->pgB[P];
+        w.assign(static_cast<std::size_t>(na) * nb, 0.0);
+        for (int a1 = 0; a1 < na; ++a1)
+            for (int b1 = 0; b1 < nb; ++b1) {
+                double v = 0.0;
+                if (!same) {
+                    double ua = 0.0, ub = 0.0;
+                    for (int t = 0; t < 2 * LA + 1; ++t) ua += cs.c[LA][t][a1];
+                    for (int t = 0; t < 2 * LB + 1; ++t) ub += cs.c[LB][t][b1];
+                    v = ua * ub;
+                } else {
+                    for (int s1 = 0; s1 < 2 * LA + 1; ++s1)
+                        for (int t1 = 0; t1 <= s1; ++t1) v += cs.c[LA][s1][a1] * cs.c[LB][t1][b1];
+                }
+                w[static_cast<std::size_t>(a1) * nb + b1] = v;
+            }
+    };
+    for (int X = kNumLowClasses; X < kNumClasses; ++X)
+        for (int Y = 0; Y <= X; ++Y) {
+            const int nab = kClassNcart[X], ncd = kClassNcart[Y];
+            std::vector<int2> all;
+            for (int P = pl->cls_begin[X]; P < pl->cls_begin[X + 1]; ++P)
+                for (int Q = pl->cls_begin[Y]; Q < pl->cls_begin[Y + 1] && Q < P; ++Q)
+                    if (pl->Q[P] * pl->Q[Q] >= thr) all.push_back(make_int2(P, Q));
+            constexpr std::size_t kBatchDoubles = std::size_t(1) << 24;
+            const std::size_t per = static_cast<std::size_t>(nab) * ncd;
+            const std::size_t nbatch = std::max<std::size_t>(1, kBatchDoubles / per);
+            std::vector<double> wp, wq;
+            for (std::size_t q0 = 0; q0 < all.size(); q0 += nbatch) {
+                const std::size_t q1 = std::min(all.size(), q0 + nbatch);
+                std::vector<int2> qs(all.begin() + q0, all.begin() + q1);
+                std::vector<long long> off(qs.size());
+                for (std::size_t k = 0; k < qs.size(); ++k) off[k] = static_cast<long long>(k * per);
+                double* dout = nullptr;
+                HIP_OK(hipMalloc(&dout, qs.size() * per * sizeof(double)));
+                run_store(pl, X, Y, qs, off, dout);
+                std::vector<double> h(qs.size() * per);
+                HIP_OK(hipMemcpy(h.data(), dout, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+                HIP_OK(hipFree(dout));
+                for (std::size_t k = 0; k < qs.size(); ++k) {
+                    pair_weights(qs[k].x, wp);
+                    pair_weights(qs[k].y, wq);
+                    const double* blk = h.data() + k * per;
+                    for (int ab = 0; ab < nab; ++ab) {
+                        double v = 0.0;
+                        for (int cd = 0; cd < ncd; ++cd) v += wq[cd] * blk[static_cast<std::size_t>(ab) * ncd + cd];
+                        total += wp[ab] * v;
+                    }
+                }
+            }
+        }
     return total;
 }
 
@@ -2514,6 +2694,9 @@ __global__ __launch_bounds__(64) void k_fingerprint(FPArgs a) {
 void eri_fingerprint(tlno_gpu_plan* pl, double thr, double* out, unsigned long long* count) {
     if (!pl->spherical)
         throw std::runtime_error("eri_ao_tensor_hgp: the ERI stream is pure spherical; build a spherical plan.");
+    if (pl->cls_begin[kNumLowClasses] != pl->npair)
+        throw std::runtime_error(
+            "tlno_gpu: the stream fingerprint covers shells up to d in this build (the checksum covers f)");
     if (!std::isfinite(thr) || thr < 0.0)
         throw std::runtime_error("eri_ao_tensor_hgp: schwarz_threshold must be finite and >= 0.");
     require_alloc("the stream fingerprint");
@@ -2768,7 +2951,7 @@ void onee_select_pairs(tlno_gpu_plan* pl, double eps) {
     std::vector<int2> list;
     list.reserve(pl->npair);
     long long kept = 0;
-    for (int c = 0; c < 6; ++c) {
+    for (int c = 0; c < kNumClasses; ++c) {
         pl->onee_cb[c] = static_cast<int>(list.size());
         for (int P = pl->cls_begin[c]; P < pl->cls_begin[c + 1]; ++P) {
             const GroupH& A = pl->groups[pl->pgA[P]];
@@ -2794,7 +2977,7 @@ void onee_select_pairs(tlno_gpu_plan* pl, double eps) {
             ++kept;
         }
     }
-    pl->onee_cb[6] = static_cast<int>(list.size());
+    pl->onee_cb[kNumClasses] = static_cast<int>(list.size());
     if (pl->d_onee_list) HIP_OK(hipFree(pl->d_onee_list));
     pl->d_onee_list = nullptr;
     HIP_OK(hipMalloc(&pl->d_onee_list, std::max<std::size_t>(1, list.size()) * sizeof(int2)));
@@ -2856,6 +3039,7 @@ tlno_gpu_plan::~tlno_gpu_plan() {
         f(kv.second.d_rbase);
         f(kv.second.d_rtask);
         f(kv.second.d_rcnt);
+        f(kv.second.d_hq);
         for (int X = 0; X < 6; ++X)
             for (int Y = 0; Y < 6; ++Y) f(kv.second.d_tq[X][Y]);
     }
@@ -2949,7 +3133,7 @@ int tlno_gpu_plan_pairs(const tlno_gpu_plan* pl, int* group_a, int* group_b, dou
             if (Q) Q[k] = pl->Q[k];
         }
         if (class_begin)
-            for (int c = 0; c <= 6; ++c) class_begin[c] = pl->cls_begin[c];
+            for (int c = 0; c <= kNumClasses; ++c) class_begin[c] = pl->cls_begin[c];
     });
 }
 
@@ -3065,6 +3249,10 @@ int tlno_gpu_quartet_features(tlno_gpu_plan* pl, int n, const int* pair_p, const
         for (int i = 0; i < n; ++i)
             if (pair_p[i] < 0 || pair_p[i] >= pl->npair || pair_q[i] < 0 || pair_q[i] >= pl->npair)
                 throw std::runtime_error("tlno_gpu: pair index out of range");
+        const int nlow = pl->cls_begin[kNumLowClasses];
+        for (int i = 0; i < n; ++i)
+            if (pair_p[i] >= nlow || pair_q[i] >= nlow)
+                throw std::runtime_error("tlno_gpu: the screening network's features cover shells up to d");
         if (n == 0) return;
         HIP_OK(hipSetDevice(pl->device));
         int *dp = nullptr, *dq = nullptr;
@@ -3458,6 +3646,10 @@ int tlno_gpu_rys(int N, int n, const double* T, double* roots, double* weights) 
             case 3: k_rys<3><<<g, 128>>>(ctx.tb, dT, dR, dW, n); break;
             case 4: k_rys<4><<<g, 128>>>(ctx.tb, dT, dR, dW, n); break;
             case 5: k_rys<5><<<g, 128>>>(ctx.tb, dT, dR, dW, n); break;
+            case 6: k_rys<6><<<g, 128>>>(ctx.tb, dT, dR, dW, n); break;
+            case 7: k_rys<7><<<g, 128>>>(ctx.tb, dT, dR, dW, n); break;
+            case 8: k_rys<8><<<g, 128>>>(ctx.tb, dT, dR, dW, n); break;
+            case 9: k_rys<9><<<g, 128>>>(ctx.tb, dT, dR, dW, n); break;
         }
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpy(roots, dR, nr * sizeof(double), hipMemcpyDeviceToHost));
@@ -3942,6 +4134,8 @@ int tlno_gpu_batch_create(int n_mol, const int* n_comp, const double* centers, c
             long long np = 0;
             for (int i = 0; i < n_comp[m]; ++i) np += nprim[ci + i];
             build_plan(pl.get(), n_comp[m], centers + 3 * ci, ang + 3 * ci, nprim + ci, exps + pi, coefs + pi);
+            if (pl->cls_begin[kNumLowClasses] != pl->npair)
+                throw std::runtime_error("tlno_gpu: molecule batches take shells up to d in this build");
             ci += n_comp[m];
             pi += np;
             bt->mols.push_back(std::move(pl));

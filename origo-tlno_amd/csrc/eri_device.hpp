@@ -177,7 +177,7 @@ __device__ __forceinline__ void rys_roots(const DevTables& tb, double T, double*
             }
         }
     } else {
-        constexpr int ttab[6] = {0, 41, 49, 55, 61, 67};
+        constexpr int ttab[10] = {0, 41, 49, 55, 61, 67, 72, 77, 83, 88};
         if (T < ttab[N]) {
             const int k = max(static_cast<int>(T), 0);
             const double u = T - k - 0.5;
@@ -217,7 +217,7 @@ struct RootEval {
             F0 = F[0];
             F1 = F[1];
         } else {
-            constexpr int ttab[6] = {0, 41, 49, 55, 61, 67};
+            constexpr int ttab[10] = {0, 41, 49, 55, 61, 67, 72, 77, 83, 88};
             if (T < ttab[N]) {
                 const int k = max(static_cast<int>(T), 0);
                 u = T - k - 0.5;
@@ -543,6 +543,19 @@ struct JKArgs {
     // fs = 24.739... w^2 p^{-1/2} (fs_min = t^2 / 24.739...^2); evaluated as
     // (w_ab w_cd)^4 < fs_min^2 p q
     double fs_min;
+    // per-quartet screens inside the team evaluator's quartet mode (lists
+    // the routing kernel does not build: quartets with an f shell): the
+    // density test with the shell-block maxima sblk (nullptr: off) and the
+    // decay indicator sdecay (nullptr: off), the rules of the routing kernel
+    const double* sblk;
+    const int* spgA;
+    const int* spgB;
+    const double* sQ;
+    int sng;
+    int smode;          // 0 J and K, 1 J only, 2 K only
+    double stau;
+    const double* sdecay;
+    double sdecay_thr;
 };
 
 // Prefactor of the FASTSCHWARZ primitive-pair estimate, sqrt(2) pi^{5/2}
@@ -816,6 +829,10 @@ __device__ __forceinline__ void eri_jk_task(const JKArgs& a, const int wave) {
 template <typename R, int LA, int LB, int LC, int LD, int AC, int AN, int CC, int CN, bool FS = false>
 __device__ __forceinline__ void eri_jk_body(const JKArgs& a) {
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    // persistent: the grid is whole blocks of four waves, so a cap that is
+    // not a multiple of four launches waves beyond the stride; they leave
+    // (their items belong to the waves below the stride)
+    if (a.stride > 0 && wave >= a.stride) return;
     const int end = a.stride > 0 ? a.nwork : wave + 1;
     const int step = a.stride > 0 ? a.stride : 1;
     for (int w = wave; w < end; w += step) eri_jk_task<R, LA, LB, LC, LD, AC, AN, CC, CN, FS>(a, w);
@@ -859,6 +876,19 @@ struct JKGroup {
     JKKernel jk32fs = nullptr;
     int c0 = 0, n = 0;        // chunks [c0, c0 + n)
 };
+// Class pairs with an f shell run the team evaluator only: its J/K kernel
+// and its block store (Schwarz diagonal and audit blocks).
+struct HighKernels {
+    JKKernel team = nullptr;
+    StoreKernel store = nullptr;
+    int nout = 0;
+};
+// pair classes: ss ps pp ds dp dd (the lane and team evaluators), then
+// fs fp fd ff (team evaluator only)
+constexpr int kNumLowClasses = 6;
+constexpr int kNumClasses = 10;
+constexpr int kMaxShellL = 3;
+
 struct ClassKernels {
     int nchunk = 0;
     int ngroup = 0;
@@ -880,5 +910,15 @@ TLNO_AMD_REG(5)
 TLNO_AMD_REG(5b)
 TLNO_AMD_REG(5c)
 #undef TLNO_AMD_REG
+// pair classes with an f shell: 6 fs, 7 fp, 8 fd, 9 ff (team evaluator only)
+#define TLNO_AMD_REG_HI(tag) void register_kernels_##tag(HighKernels (&tab)[kNumClasses - kNumLowClasses][kNumClasses]);
+TLNO_AMD_REG_HI(hi6)
+TLNO_AMD_REG_HI(hi7)
+TLNO_AMD_REG_HI(hi8a)
+TLNO_AMD_REG_HI(hi8b)
+TLNO_AMD_REG_HI(hi9a)
+TLNO_AMD_REG_HI(hi9b)
+TLNO_AMD_REG_HI(hi9c)
+#undef TLNO_AMD_REG_HI
 
 }  // namespace tlno_amd

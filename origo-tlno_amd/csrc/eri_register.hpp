@@ -111,4 +111,17 @@ void reg_class(ClassKernels (&tab)[6][6]) {
     if constexpr (LA + LB + LC + LD >= 4) k.team = k_eri_team<LA, LB, LC, LD>;
 }
 
+// (la, lb) of every pair class, the f classes after the six of the lane evaluator
+constexpr int kClassLAll[kNumClasses][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1},
+                                            {2, 2}, {3, 0}, {3, 1}, {3, 2}, {3, 3}};
+
+template <int X, int Y>
+void reg_high(HighKernels (&tab)[kNumClasses - kNumLowClasses][kNumClasses]) {
+    constexpr int LA = kClassLAll[X][0], LB = kClassLAll[X][1], LC = kClassLAll[Y][0], LD = kClassLAll[Y][1];
+    HighKernels& k = tab[X - kNumLowClasses][Y];
+    k.team = k_eri_team<LA, LB, LC, LD>;
+    k.store = k_eri_team_store<LA, LB, LC, LD>;
+    k.nout = ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD);
+}
+
 }  // namespace tlno_amd

@@ -49,8 +49,12 @@ struct TeamShape {
     static constexpr int NI = NE * NF;                     // [e|f] intermediates
     // phase B ownership: lane = (bra intermediate e < EP, part); a lane holds
     // the NF intermediates [e|f] of its e and takes every NP-th combination
-    static constexpr int EP = NE <= 8 ? 8 : (NE <= 16 ? 16 : 32);
+    static constexpr int EP = NE <= 8 ? 8 : (NE <= 16 ? 16 : (NE <= 32 ? 32 : 64));
     static constexpr int NP = 64 / EP;
+    // more bra intermediates than lanes (f bra pairs: (ff| has 74): the lanes
+    // take them in rounds of 64, each round evaluating every combination
+    // again, and each round's rows [e|f] wait in LDS ahead of the tables
+    static constexpr int NRND = (NE + 63) / 64;
     static constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
     static constexpr int NOUT = NA * NB * NC * ND;
     static constexpr int OPL = (NOUT + 63) / 64;
@@ -73,8 +77,27 @@ struct TeamShape {
     static constexpr int R0 = NI > NOUT ? NI : NOUT;
     static constexpr int LDS_C = R0 + NX;
     static constexpr int LDS_D = NOUT + NA * NB + NC * ND + NA * NC + NA * ND + NB * NC + NB * ND;
-    static constexpr int LDS = LDS_A > (LDS_C > LDS_D ? LDS_C : LDS_D) ? LDS_A : (LDS_C > LDS_D ? LDS_C : LDS_D);
+    static constexpr int A0 = NRND > 1 ? NI : 0;            // phase A tables start here
+    static constexpr int LDS_CD = LDS_C > LDS_D ? LDS_C : LDS_D;
+    static constexpr int LDS = A0 + LDS_A > LDS_CD ? A0 + LDS_A : LDS_CD;
+    // gfx950 gives a workgroup up to 160 KB of LDS; the largest f class,
+    // (ff|ff), needs 139 KB (one wave per CU)
+    static_assert(LDS * 8 <= 160 * 1024, "team evaluator LDS beyond 160 KB");
 };
+
+// binomial HRR weight C(n, t) X^(n - t) for n <= 4 (the power n - t decides
+// the coefficient: C(n, n-1) = n, C(n, n-2) = n(n-1)/2, C(n, n-3) = 4 for
+// n = 4 and 1 for n = 3); exponents are decoded arithmetically, since a
+// lane-dependent index into a constexpr table would put the table in per-lane
+// scratch memory
+__device__ __forceinline__ double hrr_weight(int n, int t, double X) {
+    const int k = n - t;
+    if (k == 0) return 1.0;
+    if (k == 1) return n * X;
+    if (k == 2) return ((n * (n - 1)) >> 1) * (X * X);
+    if (k == 3) return (n == 4 ? 4.0 : 1.0) * (X * X * X);
+    return (X * X) * (X * X);
+}
 
 // the triples of the list (|f| from LMIN to LMAX, by |f| then descending
 // lex), as compile-time tables for unrolled loops
@@ -112,24 +135,20 @@ __device__ inline void tri_decode(int LMIN, int p, int& x, int& y, int& z) {
     z = L - x - y;
 }
 
+// Phases A-C of one shell quartet, or of slice `slice` of ns of its
+// combinations: leaves the unit-diagonal Cartesian block [a][b][c][d] in
+// lds[0, NOUT) and returns true; returns false (nothing written) when the
+// slice holds no combination.
 template <int LA, int LB, int LC, int LD>
-__device__ __forceinline__ void eri_team_quartet(const JKArgs& a, double* __restrict__ lds, int P, int Q,
-                                                 double* __restrict__ Jm, double* __restrict__ Km,
-                                                 const double* __restrict__ Dm, long M, int slice, int ns) {
+__device__ __forceinline__ bool eri_team_block(const PairData& pd, const DevTables& tb, double fs_min,
+                                               double* __restrict__ lds, int P, int Q, int slice, int ns) {
     using T = TeamShape<LA, LB, LC, LD>;
     const int lane = threadIdx.x & 63;
-    const PairData& pd = a.pd;
 
     // owned intermediates [e|f] for all f: their factors are the rows
     // Gx[ex][*], Gy[ey][*], Gz[ez][*] of a combination's tables
     constexpr TriTab<LC, LC + LD> ft{};
-    const int e_own = lane % T::EP, part = lane / T::EP;
-    int ex, ey, ez;
-    tri_decode(LA, e_own < T::NE ? e_own : 0, ex, ey, ez);
-    const int rx0 = ex * (T::LCD + 1), ry0 = T::GS + ey * (T::LCD + 1), rz0 = 2 * T::GS + ez * (T::LCD + 1);
-    double acc[T::NF];
-#pragma unroll
-    for (int f = 0; f < T::NF; ++f) acc[f] = 0.0;
+    const int part = lane / T::EP;
 
     const double Ax = pd.A[3 * P], Ay = pd.A[3 * P + 1], Az = pd.A[3 * P + 2];
     const double Cx = pd.A[3 * Q], Cy = pd.A[3 * Q + 1], Cz = pd.A[3 * Q + 2];
@@ -142,8 +161,19 @@ __device__ __forceinline__ void eri_team_quartet(const JKArgs& a, double* __rest
     const int per = ((ntot + ns - 1) / ns + T::CB - 1) / T::CB * T::CB;
     const int cbeg = slice * per;
     const int ncomb = min(ntot, cbeg + per);
-    if (cbeg >= ncomb) return;  // the whole block leaves together
+    if (cbeg >= ncomb) return false;  // the whole block leaves together
 
+    double* const tab = lds + T::A0;  // phase A tables (after the rows of earlier rounds)
+    double* ef = lds;                 // NI, [e][f]
+#pragma unroll 1
+    for (int rnd = 0; rnd < T::NRND; ++rnd) {
+    const int e_own = rnd * 64 + lane % T::EP;
+    int ex, ey, ez;
+    tri_decode(LA, e_own < T::NE ? e_own : 0, ex, ey, ez);
+    const int rx0 = ex * (T::LCD + 1), ry0 = T::GS + ey * (T::LCD + 1), rz0 = 2 * T::GS + ez * (T::LCD + 1);
+    double acc[T::NF];
+#pragma unroll
+    for (int f = 0; f < T::NF; ++f) acc[f] = 0.0;
     for (int c0 = cbeg; c0 < ncomb; c0 += T::CB) {
         // ---- phase A: one combination per lane; with 32-combination
         // batches the upper half of the wave evaluates the z tables of the
@@ -164,19 +194,19 @@ __device__ __forceinline__ void eri_team_quartet(const JKArgs& a, double* __rest
             const double PQx = Pp.x - Qp.x, PQy = Pp.y - Qp.y, PQz = Pp.z - Qp.z;
             const double Tv = p * q * ipq * (PQx * PQx + PQy * PQy + PQz * PQz);
             double pref = kTwoPi52 * pd.pw[pb0 + ib] * pd.pw[pk0 + ik] * sipq;
-            if (a.fs_min > 0.0) {  // primitive FASTSCHWARZ screen: the combination adds zero
+            if (fs_min > 0.0) {  // primitive FASTSCHWARZ screen: the combination adds zero
                 const double w2 = pd.pw[pb0 + ib] * pd.pw[pk0 + ik], w4 = (w2 * w2) * (w2 * w2);
-                if (w4 < (a.fs_min * a.fs_min) * (p * q)) pref = 0.0;
+                if (w4 < (fs_min * fs_min) * (p * q)) pref = 0.0;
             }
-            const RootEval<T::NR> re(a.tb, Tv);
+            const RootEval<T::NR> re(tb, Tv);
             double t2, wr;
-            re.get(a.tb, r, t2, wr);
+            re.get(tb, r, t2, wr);
             const double qo = q * ipq, po = p * ipq;
             const double B00 = 0.5 * ipq * t2;
             const double B10 = (0.5 * rcp_nr(p)) * (1.0 - qo * t2);
             const double B01 = (0.5 * rcp_nr(q)) * (1.0 - po * t2);
             const double qt = qo * t2, pt = po * t2;
-            double* g = lds + cl * T::CS;
+            double* g = tab + cl * T::CS;
             // one axis at a time (x, y, z), so a single 2D table is live
 #pragma unroll 1
             for (int ax = ax0; ax < ax1; ++ax) {
@@ -209,7 +239,7 @@ __device__ __forceinline__ void eri_team_quartet(const JKArgs& a, double* __rest
         const int nb = min(T::CB, ncomb - c0);
 #pragma unroll 2
         for (int j = part; j < nb; j += T::NP) {
-            const double* g = lds + j * T::CS;
+            const double* g = tab + j * T::CS;
             double rx[T::LCD + 1], ry[T::LCD + 1], rz[T::LCD + 1];
 #pragma unroll
             for (int t = 0; t <= T::LCD; ++t) {
@@ -223,12 +253,8 @@ __device__ __forceinline__ void eri_team_quartet(const JKArgs& a, double* __rest
         __syncthreads();
     }
 
-    // ---- phase C: HRR from [e|f] to the unit-diagonal Cartesian block, ket
-    // side first ([e|f] -> [e|cd] for every e), then bra side
-    double* ef = lds;             // NI, [e][f]
-    double* xk = lds + T::R0;     // NX, [e][c][d]
-    double* blk = lds;            // NOUT, [a][b][c][d] (over [e|f], read before)
     // sum the parts (lane bits above EP) into part 0, which stores its row
+    // (this round's rows: the tables of a later round sit after all rows)
 #pragma unroll
     for (int f = 0; f < T::NF; ++f) {
         double v = acc[f];
@@ -240,13 +266,16 @@ __device__ __forceinline__ void eri_team_quartet(const JKArgs& a, double* __rest
 #pragma unroll
         for (int f = 0; f < T::NF; ++f) ef[e_own * T::NF + f] = acc[f];
     }
+    }  // rounds
     __syncthreads();
+
+    // ---- phase C: HRR from [e|f] to the unit-diagonal Cartesian block, ket
+    // side first ([e|f] -> [e|cd] for every e), then bra side
+    double* xk = lds + T::R0;     // NX, [e][c][d]
+    double* blk = lds;            // NOUT, [a][b][c][d] (over [e|f], read before)
     const double AB[3] = {pd.AB[3 * P], pd.AB[3 * P + 1], pd.AB[3 * P + 2]};
     const double CD[3] = {pd.AB[3 * Q], pd.AB[3 * Q + 1], pd.AB[3 * Q + 2]};
-    // binomial HRR weight C(n, t) X^(n - t) for n <= 2; exponents are decoded
-    // arithmetically (a lane-dependent index into a constexpr table would put
-    // the table in per-lane scratch memory)
-    auto hw = [](int n, int t, double X) { return t == n ? 1.0 : (n - t == 1 ? n * X : X * X); };
+    auto hw = [](int n, int t, double X) { return hrr_weight(n, t, X); };
     // [e|c d) = sum_u C(d, u) CD^(d - u) [e|c + u]
     for (int i = lane; i < T::NX; i += 64) {
         const int d = i % T::ND, c = (i / T::ND) % T::NC, e = i / (T::ND * T::NC);
@@ -281,8 +310,19 @@ __device__ __forceinline__ void eri_team_quartet(const JKArgs& a, double* __rest
         blk[o] = v * (nang(axc, ayc, azc) * nang(bx, by, bz) * nang(cxc, cyc, czc) * nang(dx, dy, dz));
     }
     __syncthreads();
+    return true;
+}
 
-    // ---- phase D: digestion (J and K element sums from the LDS block)
+// Phase D: digestion of the block in lds[0, NOUT) (J and K element sums),
+// the contract of the lane evaluator (eri_jk_task).
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void eri_team_digest(const JKArgs& a, double* __restrict__ lds, int P, int Q,
+                                                double* __restrict__ Jm, double* __restrict__ Km,
+                                                const double* __restrict__ Dm, long M) {
+    using T = TeamShape<LA, LB, LC, LD>;
+    const int lane = threadIdx.x & 63;
+    const PairData& pd = a.pd;
+    double* blk = lds;
     double s = 1.0;
     if (pd.diag[P]) s *= 0.5;
     if (pd.diag[Q]) s *= 0.5;
@@ -406,6 +446,21 @@ __global__ __launch_bounds__(64) void k_eri_team(JKArgs a) {
                 Q = qb + k;
             }
         }
+        if (a.tquart && (a.sblk || a.sdecay)) {
+            // the routing kernel's density and decay rules, per quartet
+            bool keep = true;
+            if (a.sblk) {
+                const long long ng = a.sng;
+                const long long ga = a.spgA[P], gb = a.spgB[P], gc = a.spgA[Q], gd = a.spgB[Q];
+                const double coul = 4.0 * fmax(a.sblk[ga * ng + gb], a.sblk[gc * ng + gd]);
+                const double exch = fmax(fmax(a.sblk[gc * ng + ga], a.sblk[gd * ng + ga]),
+                                         fmax(a.sblk[gc * ng + gb], a.sblk[gd * ng + gb]));
+                const double fac = a.smode == 1 ? coul : (a.smode == 2 ? exch : fmax(coul, exch));
+                keep = a.sQ[P] * a.sQ[Q] * fac >= a.stau;
+            }
+            if (a.sdecay) keep = keep && a.sdecay[P] * a.sdecay[Q] >= a.sdecay_thr;
+            if (!keep) continue;
+        }
         long M = a.M;
         const double* __restrict__ Dm = a.D;
         double* __restrict__ Jm = a.J;
@@ -423,7 +478,28 @@ __global__ __launch_bounds__(64) void k_eri_team(JKArgs a) {
             atomicAdd(cnt, 1ull);
             atomicAdd(cnt + 1, static_cast<unsigned long long>(a.pd.pn[Q]) * a.pd.pn[P]);
         }
-        eri_team_quartet<LA, LB, LC, LD>(a, lds, P, Q, Jm, Km, Dm, M, slice, ns);
+        if (eri_team_block<LA, LB, LC, LD>(a.pd, a.tb, a.fs_min, lds, P, Q, slice, ns))
+            eri_team_digest<LA, LB, LC, LD>(a, lds, P, Q, Jm, Km, Dm, M);
+    }
+}
+
+// Full unit-diagonal Cartesian blocks [a][b][c][d] of a list of shell
+// quartets (Schwarz diagonal blocks and audit blocks of the classes that
+// only the team evaluator serves): one wave per quartet, striding over the
+// list when the grid is smaller.
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(64) void k_eri_team_store(StoreArgs a) {
+    using T = TeamShape<LA, LB, LC, LD>;
+    __shared__ double lds[T::LDS];
+    const int lane = threadIdx.x & 63;
+    for (long long i = blockIdx.x; i < a.n; i += gridDim.x) {
+        const int2 pq = a.quartets[i];
+        const int P = __builtin_amdgcn_readfirstlane(pq.x);
+        const int Q = __builtin_amdgcn_readfirstlane(pq.y);
+        eri_team_block<LA, LB, LC, LD>(a.pd, a.tb, 0.0, lds, P, Q, 0, 1);
+        double* out = a.out + (a.off ? a.off[i] : i * a.stride);
+        for (int o = lane; o < T::NOUT; o += 64) out[o] = lds[o];
+        __syncthreads();  // the LDS block is reused by the next quartet
     }
 }
 

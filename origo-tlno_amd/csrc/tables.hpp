@@ -31,7 +31,7 @@ constexpr int BOYS_NGRID = 41 * BOYS_STEPS + 1; // T in [0, 41], step 1/32
 constexpr double BOYS_TCUT = 40.0;     // asymptotic + upward recursion beyond
 constexpr int BOYS_TAYLOR = 7;
 
-constexpr int RYS_NMAX = 5;   // shells up to d: at most 5 roots
+constexpr int RYS_NMAX = 9;   // shells up to g: at most 9 roots
 constexpr int RYS_DEG = 11;             // polynomial degree per interval
 constexpr int RYS_NCOEF = RYS_DEG + 1;
 

@@ -20,6 +20,9 @@ test_molecules/README.md:3):
              exponents as water7.
   water100   100 waters laid out like water40 at the same number density
              (about 2500 functions).
+  organic_f  organic30's generator with an f polarization shell (1-2
+             primitives, exponents in [0.3, 6]) on the heavy atoms, a few
+             atoms: the f-shell parity workload.
 
 Bases are drawn once per element (a synthetic basis SET) from the workload's
 seed, so equal elements share contractions as a real basis set does.  Seeds
@@ -180,6 +183,26 @@ def water_cluster(nwat, seed=SEED0 + 4):
     """water40-style cluster of any size at water40's number density."""
     box = 11.0 * (nwat / 40.0) ** (1.0 / 3.0)
     return water40(seed=seed, nwat=nwat, box_A=box)
+
+
+def organic_f(seed=SEED0 + 8, natoms=5, box_A=4.5):
+    """organic30's generator with f shells (the reference's L <= 4 range
+    beyond d): heavy atoms 3s2p1d as organic30 (1-6 primitives, exponents
+    log-uniform in [0.08, 5000]) plus one f polarization shell of 1-2
+    primitives with exponents log-uniform in [0.3, 6] (the range of the f
+    shells of real basis sets, e.g. def2-TZVP's), H 2s1p; elements C/N/O/H
+    with probabilities .4/.2/.2/.2, minimum separation 1.0 A; D = C C^T with
+    C nbf x nbf/5 N(0,1), ||D||_F = 1."""
+    rng = np.random.default_rng(seed)
+    atoms = _random_atoms(rng, natoms, box_A, 1.0, probs=(0.4, 0.2, 0.2, 0.2))
+    eb = _element_bases(rng, ["C", "N", "O", "H"], [0, 0, 0, 1, 1, 2], [0, 0, 1],
+                        (1, 6), (0.08, 5000.0))
+    for el in ("C", "N", "O"):
+        eb[el] = eb[el] + _draw_shells(rng, [3], (1, 2), (0.3, 6.0))
+    b = _build(atoms, eb, f"organic_f{natoms}")
+    D = _density(rng, b.n, max(1, b.n // 5))
+    D /= np.linalg.norm(D)
+    return b, D
 
 
 def distinct_pairs():

@@ -63,6 +63,15 @@ tests) and the reference's outputs:
                    oracle/_ref/libtlno_ref_fp_avx512.so) of water7 at Schwarz
                    0, organic5 at 0 and 1e-12, organic30 at 1e-12
                    (python gen_golden.py fingerprint rewrites only it)
+  fshell.npz       configs.organic_f() (5 atoms, heavy 3s2p1d1f: 125
+                   Cartesian / 105 spherical functions): the reference's
+                   audit Cartesian blocks of one quartet per class pair with
+                   an f shell and 12 shuffled ones, its Rys rules for 6..9
+                   roots, spherical G/J/K at Schwarz 1e-12 and G with density
+                   threshold 1e-7, stream checksum, S/T/V; the oracle's
+                   Cartesian J/K (density D) and spherical J/K (density Dsph)
+                   at Schwarz 1e-12 (python gen_golden.py fshell rewrites
+                   only it)
   density.npz      4-water cluster (96 spherical functions), density
                    configs.spherical_density(96, 9, 20): the reference's
                    density-screened J, K and G (density_threshold 1e-9, 1e-7,
@@ -112,6 +121,8 @@ def main(only=None):
         return batch256()
     if only == "water10":
         return water10()
+    if only == "fshell":
+        return fshell(ref)
     t0 = time.time()
     # ---------------------------------------------------------- Boys / Rys
     xs = np.concatenate([[0.0, 1e-10, 1e-3, 0.5, 5.0, 11.99, 12.0, 12.5, 25.0, 39.99, 40.0, 45.0, 80.0, 200.0],
@@ -180,6 +191,7 @@ def main(only=None):
     batch256()
     water10()
     fingerprint(ref)
+    fshell(ref)
     print(f"golden fixtures written in {time.time() - t0:.1f} s")
 
 
@@ -311,6 +323,55 @@ def fingerprint(ref):
             out[f"{name}_{tag}"] = ref.fingerprint(b, thr)
             print(f"fingerprint {name} at {thr}: {out[f'{name}_{tag}']}", flush=True)
     np.savez_compressed(os.path.join(HERE, "fingerprint.npz"), **out)
+
+
+def fshell(ref):
+    # ---------------------------------------------------------- f shells
+    b, D = configs.organic_f()
+    out = {"basis_sha": basis_hash(b), "D": D}
+    groups = ref.groups(b)
+    Ls = [g[0] for g in groups]
+    byL = {L: [g for g in range(len(groups)) if Ls[g] == L] for L in set(Ls)}
+    rng = np.random.default_rng(61)
+    quads = []
+    # one quartet per class pair with an f shell (bra L >= ket L per pair,
+    # bra pair class >= ket pair class), then shuffled shell orders
+    cls = [(la, lb) for la in range(4) for lb in range(la + 1)]
+    for x in range(len(cls)):
+        for y in range(x + 1):
+            (la, lb), (lc, ld) = cls[x], cls[y]
+            if max(la, lb, lc, ld) == 3:
+                quads.append([int(rng.choice(byL[v])) for v in (la, lb, lc, ld)])
+    for _ in range(12):
+        q = [int(rng.choice(byL[3]))] + [int(rng.integers(len(groups))) for _ in range(3)]
+        quads.append([q[i] for i in rng.permutation(4)])
+    blocks = [ref.cart_block(b, *q, groups=groups).ravel() for q in quads]
+    out["quads"] = np.array(quads, dtype=np.int32)
+    out["block_off"] = np.concatenate([[0], np.cumsum([len(x) for x in blocks])]).astype(np.int64)
+    out["blocks"] = np.concatenate(blocks)
+    # Rys rules of 6..9 roots (the reference's production table route)
+    Ts = np.concatenate([[0.0, 1e-8, 0.5, 7.3, 59.99, 60.0, 64.99, 65.0, 70.99, 71.0, 75.99, 76.0, 90.0, 150.0],
+                         np.random.default_rng(62).uniform(0, 90, 60)])
+    out["Ts"] = Ts
+    for N in range(6, 10):
+        out[f"rys_r{N}"] = np.array([ref.rys(N, T, 0)[0] for T in Ts])
+        out[f"rys_w{N}"] = np.array([ref.rys(N, T, 0)[1] for T in Ts])
+    # Cartesian J/K of the oracle (the reference's direct build is spherical only)
+    ob = O.OracleBasis(b)
+    out["Jc_12"], out["Kc_12"] = ob.jk(D, spherical=False, schwarz=1e-12)
+    # spherical builds: the reference's and the oracle's
+    M = ref.plan_M(b)
+    Ds = sph_density(M, 51, 20)
+    out["Dsph"] = Ds
+    for w in "GJK":
+        out[f"{w}_12"] = ref.direct(b, Ds, w, schwarz=1e-12, threads=1)[0]
+    out["G_d7"] = ref.direct(b, Ds, "G", schwarz=1e-12, density_threshold=1e-7, threads=1)[0]
+    out["Jo_12"], out["Ko_12"] = ob.jk(Ds, spherical=True, schwarz=1e-12)
+    out["checksum_12"] = ref.checksum(b, 1e-12)
+    for sph in (0, 1):
+        S, T, V = ref.onee(b, b.nuclear_charges, b.nuclear_centers, spherical=bool(sph), screening=False)
+        out[f"S{sph}"], out[f"T{sph}"], out[f"V{sph}"] = S, T, V
+    np.savez_compressed(os.path.join(HERE, "fshell.npz"), **out)
 
 
 BATCH256_PICK = tuple(range(0, 256, 16))
