@@ -219,6 +219,65 @@ def water100_run(gate=1e-6, tau=SCREEN_TAU):
     return out
 
 
+def fshell_run(steps=5, warmup=2, natoms=30, box_A=12.0):
+    """f shells: organic_f(30) (organic30's generator and box with an f
+    polarization shell on the heavy atoms; Cartesian, Schwarz 1e-12), J+K
+    builds from a density resident in HBM after the plan autotuner; the class
+    pairs with an f shell run the team evaluator.  Per class pair: serialized
+    kernel time (HIP events), algorithmic flops (tlno_amd.flops) and rate."""
+    lib = _lib.load()
+    b, D = configs.organic_f(natoms=natoms, box_A=box_A)
+    thr = 1e-12
+    plan = DirectPlan(b, ScreeningOptions(integral_threshold=thr), cartesian=True)
+    plan.autotune(schwarz=thr, repeats=1)
+    M = plan.M
+    dD = hipmem.DeviceArray.from_host(D)
+    dJ = hipmem.DeviceArray((M, M))
+    dK = hipmem.DeviceArray((M, M))
+
+    def one():
+        _lib.check(lib.tlno_gpu_build_jk_device(plan.handle, dD.ptr, dJ.ptr, dK.ptr, thr))
+
+    for _ in range(warmup):
+        one()
+    hipmem.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        one()
+    hipmem.synchronize()
+    sec = (time.perf_counter() - t0) / steps
+    sq_all, pq_all = plan.last_build_stats()
+    _lib.check(lib.tlno_gpu_set_timing(plan.handle, 1))
+    for _ in range(3):
+        one()
+    n = len(flops.CLASS_L)
+    sq = (C.c_longlong * (n * n))()
+    pq = (C.c_longlong * (n * n))()
+    ms = (C.c_double * (n * n))()
+    nb = C.c_longlong()
+    _lib.check(lib.tlno_gpu_class_stats_all(plan.handle, sq, pq, ms, C.byref(nb)))
+    _lib.check(lib.tlno_gpu_set_timing(plan.handle, 0))
+    classes = []
+    for X in range(n):
+        for Y in range(X + 1):
+            i = n * X + Y
+            if sq[i] == 0 or max(X, Y) < 6:
+                continue
+            fl = flops.class_flops(X, Y, pq[i], sq[i])
+            t = ms[i] / nb.value * 1e-3
+            classes.append({"class": f"({flops.CLASS_NAME[X]}|{flops.CLASS_NAME[Y]})", "shell_quartets": sq[i],
+                            "prim_quartets": pq[i], "ms": round(ms[i] / nb.value, 4), "gflop": round(fl / 1e9, 3),
+                            "tflops": round(fl / t / 1e12, 2) if t > 0 else 0.0})
+    classes.sort(key=lambda c: -c["ms"])
+    f_sq = sum(c["shell_quartets"] for c in classes)
+    return {"workload": f"organic_f{natoms} ({natoms} atoms in a {box_A:g} A cube, heavy 3s2p1d1f / H 2s1p, "
+                        f"{M} Cartesian functions, Schwarz 1e-12)",
+            "ms_per_build": sec * 1e3, "builds_per_s": 1.0 / sec, "shell_quartets_per_build": sq_all,
+            "prim_quartets_per_build": pq_all, "shell_quartets_per_s": sq_all / sec,
+            "f_share_of_shell_quartets": f_sq / max(1, sq_all),
+            "f_class_serialized_ms": sum(c["ms"] for c in classes), "f_classes": classes}
+
+
 def onee_run(repeats=5):
     """One-electron S, T, V (Cartesian, the reference's default screening:
     group-pair decay 1e-15, nuclear primitive pairs 1e-15 / sum|Z|) on
@@ -399,6 +458,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-water100", action="store_true")
     ap.add_argument("--no-onee", action="store_true")
+    ap.add_argument("--no-fshell", action="store_true")
     ap.add_argument("--no-queue-probe", action="store_true",
                     help="skip the organic30 rerun with HIP's default 4 hardware queues")
     ap.add_argument("--routing-profile", type=int, default=0, metavar="BUILDS",
@@ -478,6 +538,9 @@ def main():
     if not args.no_onee:
         print("bench: one-electron S/T/V", file=sys.stderr, flush=True)
         result["onee"] = onee_run()
+    if not args.no_fshell:
+        print("bench: f-shell builds (organic_f30)", file=sys.stderr, flush=True)
+        result["fshell"] = fshell_run()
     if not args.no_batch:
         print("bench: 256-molecule batch builds", file=sys.stderr, flush=True)
         result["batched"] = batched_run()

@@ -225,6 +225,11 @@ int tlno_gpu_last_build_stats(const tlno_gpu_plan* plan, long long* shell_quarte
 int tlno_gpu_set_timing(tlno_gpu_plan* plan, int enable);
 int tlno_gpu_class_stats(const tlno_gpu_plan* plan, long long* shell_quartets,
                          long long* prim_quartets, double* ms, long long* builds);
+/* The same over all ten pair classes (ss ps pp ds dp dd fs fp fd ff): 100
+ * entries each, index 10*X + Y for bra class X >= ket class Y (the f class
+ * pairs time their team-evaluator launch; other entries 0). */
+int tlno_gpu_class_stats_all(const tlno_gpu_plan* plan, long long* shell_quartets,
+                             long long* prim_quartets, double* ms, long long* builds);
 
 /* Unit-diagonal Cartesian ERI blocks (ga gb|gc gd) for n group quadruples
  * (groups4[4i..4i+3], groups numbered in first-appearance order); block i is

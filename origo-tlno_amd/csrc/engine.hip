@@ -356,6 +356,7 @@ struct TaskSet {
     // (X >= 6, Y <= X) the Schwarz-surviving {P, Q}, heaviest first
     int2* d_hq = nullptr;
     long long hoff[kNumHigh][kNumClasses] = {}, hcnt[kNumHigh][kNumClasses] = {};
+    long long hpq[kNumHigh][kNumClasses] = {};  // their primitive quartets
 };
 
 }  // namespace tlno_amd
@@ -457,6 +458,9 @@ struct tlno_gpu_plan {
     hipEvent_t onee_ev[2] = {nullptr, nullptr};
     double cls_ms[6][6] = {};
     long long timed_builds = 0;
+    // timing mode: the class pairs with an f shell
+    hipEvent_t hev[tlno_amd::kNumClasses - tlno_amd::kNumLowClasses][tlno_amd::kNumClasses][2] = {};
+    double hcls_ms[tlno_amd::kNumClasses - tlno_amd::kNumLowClasses][tlno_amd::kNumClasses] = {};
 
     tlno_amd::PairData pair_data() const {
         tlno_amd::PairData p;
@@ -1797,6 +1801,7 @@ TaskSet& get_tasks(tlno_gpu_plan* pl, double thr) {
                 if (end > k0) {
                     ts.shell_quartets += end - k0;
                     ts.prim_quartets += (pref[end] - pref[k0]) * pl->pn_host[P];
+                    ts.hpq[h][Y] += (pref[end] - pref[k0]) * pl->pn_host[P];
                 }
             }
             std::stable_sort(cls.begin(), cls.end(), [&](const int2& u, const int2& v) {
@@ -2305,8 +2310,11 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
             // timing mode: one stream, class order, events around each class's jobs
             for (std::size_t i = 0; i < njobs; ++i) {
                 Job& j = jobs[i];
-                if (j.high) {  // not part of the per-class timing table
+                if (j.high) {
+                    hipEvent_t* e = pl->hev[j.X - kNumLowClasses][j.Y];
+                    if (e[0]) HIP_OK(hipEventRecord(e[0], st));
                     launch_high(g_high[j.X - kNumLowClasses][j.Y], j.args, st);
+                    if (e[1]) HIP_OK(hipEventRecord(e[1], st));
                     continue;
                 }
                 const bool first = i == 0 || jobs[i - 1].X != j.X || jobs[i - 1].Y != j.Y;
@@ -2366,6 +2374,13 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
                     float ms = 0.0f;
                     HIP_OK(hipEventElapsedTime(&ms, pl->ev[X][Y][0], pl->ev[X][Y][1]));
                     pl->cls_ms[X][Y] += ms;
+                }
+        for (int h = 0; h < kNumHigh; ++h)
+            for (int Y = 0; Y <= h + kNumLowClasses; ++Y)
+                if (ts.hcnt[h][Y] && pl->hev[h][Y][0]) {
+                    float ms = 0.0f;
+                    HIP_OK(hipEventElapsedTime(&ms, pl->hev[h][Y][0], pl->hev[h][Y][1]));
+                    pl->hcls_ms[h][Y] += ms;
                 }
         ++pl->timed_builds;
     }
@@ -3048,6 +3063,10 @@ tlno_gpu_plan::~tlno_gpu_plan() {
             for (int Y = 0; Y < 6; ++Y)
                 for (int e = 0; e < 2; ++e)
                     if (ev[X][Y][e]) (void)hipEventDestroy(ev[X][Y][e]);
+    for (auto& row : hev)
+        for (auto& pr : row)
+            for (auto& e : pr)
+                if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -3102,9 +3121,47 @@ int tlno_gpu_set_timing(tlno_gpu_plan* pl, int enable) {
             for (int X = 0; X < 6; ++X)
                 for (int Y = 0; Y < 6; ++Y)
                     for (int e = 0; e < 2; ++e) HIP_OK(hipEventCreate(&pl->ev[X][Y][e]));
+        if (enable)
+            for (int h = 0; h < kNumHigh; ++h)
+                for (int Y = 0; Y < kNumClasses; ++Y)
+                    for (int e = 0; e < 2; ++e)
+                        if (!pl->hev[h][Y][e]) HIP_OK(hipEventCreate(&pl->hev[h][Y][e]));
         pl->timing = enable ? 1 : 0;
         std::memset(pl->cls_ms, 0, sizeof(pl->cls_ms));
+        std::memset(pl->hcls_ms, 0, sizeof(pl->hcls_ms));
         pl->timed_builds = 0;
+    });
+}
+
+int tlno_gpu_class_stats_all(const tlno_gpu_plan* pl, long long* shellq, long long* primq, double* ms,
+                             long long* builds) {
+    return guarded([&] {
+        if (!pl) throw std::runtime_error("jk_direct: null J/K plan backend.");
+        auto it = pl->tasks.find(pl->last_thr);
+        const bool have = it != pl->tasks.end();
+        for (int X = 0; X < kNumClasses; ++X)
+            for (int Y = 0; Y < kNumClasses; ++Y) {
+                const int i = kNumClasses * X + Y;
+                long long sq = 0, pq = 0;
+                double m = 0.0;
+                if (X < kNumLowClasses && Y < kNumLowClasses) {
+                    if (have) {
+                        sq = it->second.cls_sq[X][Y];
+                        pq = it->second.cls_pq[X][Y];
+                    }
+                    m = pl->cls_ms[X][Y];
+                } else if (X >= kNumLowClasses && Y <= X) {
+                    if (have) {
+                        sq = it->second.hcnt[X - kNumLowClasses][Y];
+                        pq = it->second.hpq[X - kNumLowClasses][Y];
+                    }
+                    m = pl->hcls_ms[X - kNumLowClasses][Y];
+                }
+                if (shellq) shellq[i] = sq;
+                if (primq) primq[i] = pq;
+                if (ms) ms[i] = m;
+            }
+        if (builds) *builds = pl->timed_builds;
     });
 }
 

@@ -40,6 +40,7 @@ SIGNATURES = {
     "tlno_gpu_set_primitive_screens": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
     "tlno_gpu_last_mixed_stats": (C.c_int, [C.c_void_p, LL, LL, D]),
     "tlno_gpu_class_stats": (C.c_int, [C.c_void_p, LL, LL, D, LL]),
+    "tlno_gpu_class_stats_all": (C.c_int, [C.c_void_p, LL, LL, D, LL]),
     "tlno_gpu_eri_cart_blocks": (C.c_int, [C.c_void_p, C.c_int, I, LL, D]),
     "tlno_gpu_boys": (C.c_int, [C.c_int, D, C.c_int, D]),
     "tlno_gpu_rys": (C.c_int, [C.c_int, C.c_int, D, D, D]),

@@ -117,7 +117,7 @@ def quartet_features(plan, P, Q) -> np.ndarray:
 def pair_classes(plan) -> np.ndarray:
     _, _, _, cb = plan.pairs()
     cls = np.zeros(plan.profile.pair_groups, np.int32)
-    for c in range(6):
+    for c in range(len(cb) - 1):
         cls[cb[c]:cb[c + 1]] = c
     return cls
 
