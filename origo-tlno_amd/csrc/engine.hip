@@ -765,7 +765,7 @@ void launch_jk(const ClassKernels& k, bool fp32, JKArgs& args, long long nwaves,
 }
 
 // team kernel of a class pair with an f shell over its quartet list (one
-// 64-thread block per quartet and slice; capped grids stride)
+// workgroup of k.threads per quartet and slice; capped grids stride)
 void launch_high(const HighKernels& k, JKArgs& args, hipStream_t st) {
     void* params[] = {&args};
     const long long ns = std::max(1, args.team_slices);
@@ -774,7 +774,7 @@ void launch_high(const HighKernels& k, JKArgs& args, hipStream_t st) {
     const long long cap = max_grid_waves();
     args.stride = work > cap ? static_cast<int>(cap) : 0;
     HIP_OK(hipLaunchKernel(reinterpret_cast<const void*>(k.team), dim3(static_cast<unsigned>(std::min(work, cap))),
-                           dim3(64), params, 0, st));
+                           dim3(k.threads), params, 0, st));
     args.stride = 0;
 }
 
@@ -1145,11 +1145,12 @@ void run_store(tlno_gpu_plan* pl, int X, int Y, const std::vector<int2>& qs, con
     a.n = static_cast<int>(qs.size());
     a.out = d_out;
     if (X >= kNumLowClasses) {
-        // classes with an f shell: team-evaluator blocks, one wave per quartet
+        // classes with an f shell: team-evaluator blocks, one workgroup per quartet
         void* params[] = {&a};
         const long long nb = std::min<long long>(a.n, max_grid_waves());
-        HIP_OK(hipLaunchKernel(reinterpret_cast<const void*>(g_high[X - kNumLowClasses][Y].store),
-                               dim3(static_cast<unsigned>(nb)), dim3(64), params, 0, pl->stream));
+        const HighKernels& hk = g_high[X - kNumLowClasses][Y];
+        HIP_OK(hipLaunchKernel(reinterpret_cast<const void*>(hk.store), dim3(static_cast<unsigned>(nb)),
+                               dim3(hk.threads), params, 0, pl->stream));
     } else {
         launch_store(g_kernels[X][Y], a, pl->stream);
     }

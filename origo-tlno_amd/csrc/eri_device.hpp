@@ -882,6 +882,7 @@ struct HighKernels {
     JKKernel team = nullptr;
     StoreKernel store = nullptr;
     int nout = 0;
+    int threads = 64;  // threads per quartet (workgroup size): 64 or 256
 };
 // pair classes: ss ps pp ds dp dd (the lane and team evaluators), then
 // fs fp fd ff (team evaluator only)

@@ -108,7 +108,10 @@ void reg_class(ClassKernels (&tab)[6][6]) {
     reg_groups<LA, LB, LC, LD>(k, std::make_integer_sequence<int, ngroup>{});
     reg_store<LA, LB, LC, LD>(k, std::make_integer_sequence<int, CP::nchunk>{});
     k.nout = CP::nc;
-    if constexpr (LA + LB + LC + LD >= 4) k.team = k_eri_team<LA, LB, LC, LD>;
+    if constexpr (LA + LB + LC + LD >= 4) {
+        static_assert(TeamShape<LA, LB, LC, LD>::NT == 64, "classes up to d run one wave per quartet");
+        k.team = k_eri_team<LA, LB, LC, LD>;
+    }
 }
 
 // (la, lb) of every pair class, the f classes after the six of the lane evaluator
@@ -122,6 +125,7 @@ void reg_high(HighKernels (&tab)[kNumClasses - kNumLowClasses][kNumClasses]) {
     k.team = k_eri_team<LA, LB, LC, LD>;
     k.store = k_eri_team_store<LA, LB, LC, LD>;
     k.nout = ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD);
+    k.threads = TeamShape<LA, LB, LC, LD>::NT;
 }
 
 }  // namespace tlno_amd

@@ -1,5 +1,6 @@
-// Team evaluator for high angular momentum classes: one wave per shell
-// quartet, with the horizontal recurrence moved after the contraction.
+// Team evaluator for high angular momentum classes: one wave (four for the
+// largest f classes, TeamShape::TW) per shell quartet, with the horizontal
+// recurrence moved after the contraction.
 //
 // For bra shells (LA, LB) and ket shells (LC, LD) the contracted quartet is
 //   (ab|cd) = HRR_bra HRR_ket [e|f],   [e|f] = sum_prim sum_root Gx Gy (w Gz),
@@ -47,16 +48,26 @@ struct TeamShape {
     static constexpr int NE = ncart_cum(LAB) - (LA > 0 ? ncart_cum(LA - 1) : 0);
     static constexpr int NF = ncart_cum(LCD) - (LC > 0 ? ncart_cum(LC - 1) : 0);
     static constexpr int NI = NE * NF;                     // [e|f] intermediates
-    // phase B ownership: lane = (bra intermediate e < EP, part); a lane holds
-    // the NF intermediates [e|f] of its e and takes every NP-th combination
-    static constexpr int EP = NE <= 8 ? 8 : (NE <= 16 ? 16 : (NE <= 32 ? 32 : 64));
-    static constexpr int NP = 64 / EP;
-    // more bra intermediates than lanes (f bra pairs: (ff| has 74): the lanes
-    // take them in rounds of 64, each round evaluating every combination
-    // again, and each round's rows [e|f] wait in LDS ahead of the tables
-    static constexpr int NRND = (NE + 63) / 64;
     static constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
     static constexpr int NOUT = NA * NB * NC * ND;
+    // waves per shell quartet: 4 for the f classes whose bra intermediates
+    // outnumber a wave's lanes ((ff| has 74) or whose block exceeds 3,000
+    // values -- their HRR and digestion, over up to 10,000 values, dominate
+    // (f shells carry few primitives), and their LDS leaves one workgroup per
+    // CU (the first wave alone evaluates the roots and tables, so classes of
+    // 1,800-2,200 values with long contractions, (fd|fp) and (fd|dd), measured
+    // 35-50 % slower on four waves); one wave for every class up to d
+    static constexpr int TW = (NE > 64 || NOUT > 3000) ? 4 : 1;
+    static constexpr int NT = 64 * TW;                     // threads per quartet
+    // phase B ownership: thread = (bra intermediate e < EP, part); a thread
+    // holds the NF intermediates [e|f] of its e and takes every NP-th
+    // combination (four-wave quartets: one thread per e, every combination)
+    static constexpr int EP = TW > 1 ? NT : (NE <= 8 ? 8 : (NE <= 16 ? 16 : (NE <= 32 ? 32 : 64)));
+    static constexpr int NP = TW > 1 ? 1 : 64 / EP;
+    // more bra intermediates than threads: rounds of NT, each round evaluating
+    // every combination again, each round's rows [e|f] waiting in LDS ahead of
+    // the tables (not reached for L <= 3: (ff| has 74 on four waves)
+    static constexpr int NRND = (NE + NT - 1) / NT;
     static constexpr int OPL = (NOUT + 63) / 64;
     static constexpr int GS = (LAB + 1) * (LCD + 1);       // one axis table
     // one combination's three tables, padded to an odd number of doubles:
@@ -143,12 +154,13 @@ template <int LA, int LB, int LC, int LD>
 __device__ __forceinline__ bool eri_team_block(const PairData& pd, const DevTables& tb, double fs_min,
                                                double* __restrict__ lds, int P, int Q, int slice, int ns) {
     using T = TeamShape<LA, LB, LC, LD>;
-    const int lane = threadIdx.x & 63;
+    const int tid = threadIdx.x;  // 0 .. NT - 1
+    const int lane = tid & 63;
 
     // owned intermediates [e|f] for all f: their factors are the rows
     // Gx[ex][*], Gy[ey][*], Gz[ez][*] of a combination's tables
     constexpr TriTab<LC, LC + LD> ft{};
-    const int part = lane / T::EP;
+    const int part = tid / T::EP;
 
     const double Ax = pd.A[3 * P], Ay = pd.A[3 * P + 1], Az = pd.A[3 * P + 2];
     const double Cx = pd.A[3 * Q], Cy = pd.A[3 * Q + 1], Cz = pd.A[3 * Q + 2];
@@ -167,7 +179,7 @@ __device__ __forceinline__ bool eri_team_block(const PairData& pd, const DevTabl
     double* ef = lds;                 // NI, [e][f]
 #pragma unroll 1
     for (int rnd = 0; rnd < T::NRND; ++rnd) {
-    const int e_own = rnd * 64 + lane % T::EP;
+    const int e_own = rnd * T::NT + tid % T::EP;
     int ex, ey, ez;
     tri_decode(LA, e_own < T::NE ? e_own : 0, ex, ey, ez);
     const int rx0 = ex * (T::LCD + 1), ry0 = T::GS + ey * (T::LCD + 1), rz0 = 2 * T::GS + ez * (T::LCD + 1);
@@ -175,14 +187,14 @@ __device__ __forceinline__ bool eri_team_block(const PairData& pd, const DevTabl
 #pragma unroll
     for (int f = 0; f < T::NF; ++f) acc[f] = 0.0;
     for (int c0 = cbeg; c0 < ncomb; c0 += T::CB) {
-        // ---- phase A: one combination per lane; with 32-combination
-        // batches the upper half of the wave evaluates the z tables of the
-        // lower half's combinations (each half repeats the roots)
+        // ---- phase A: one combination per lane of the first wave; with
+        // 32-combination batches the upper half of the wave evaluates the z
+        // tables of the lower half's combinations (each half repeats the roots)
         const int cl = T::CB == 64 ? lane : (lane & 31);
         const int ax0 = T::CB == 64 ? 0 : (lane < 32 ? 0 : 2);
         const int ax1 = T::CB == 64 ? 3 : (lane < 32 ? 2 : 3);
         const int c = c0 + cl;
-        if (c < ncomb) {
+        if (c < ncomb && tid < 64) {
             const int r = c % T::NR;
             const int pq = c / T::NR;
             const int ib = pq % pbn, ik = pq / pbn;
@@ -277,7 +289,7 @@ __device__ __forceinline__ bool eri_team_block(const PairData& pd, const DevTabl
     const double CD[3] = {pd.AB[3 * Q], pd.AB[3 * Q + 1], pd.AB[3 * Q + 2]};
     auto hw = [](int n, int t, double X) { return hrr_weight(n, t, X); };
     // [e|c d) = sum_u C(d, u) CD^(d - u) [e|c + u]
-    for (int i = lane; i < T::NX; i += 64) {
+    for (int i = tid; i < T::NX; i += T::NT) {
         const int d = i % T::ND, c = (i / T::ND) % T::NC, e = i / (T::ND * T::NC);
         int cxc, cyc, czc, dx, dy, dz;
         tri_decode(LC, c, cxc, cyc, czc);
@@ -294,7 +306,7 @@ __device__ __forceinline__ bool eri_team_block(const PairData& pd, const DevTabl
     __syncthreads();
     // (a b| = sum_t C(b, t) AB^(b - t) (a + t|, then the unit-diagonal norms
     auto nang = [](int x, int y, int z) { return inv_sqrt_small(dfact2m1(x) * dfact2m1(y) * dfact2m1(z)); };
-    for (int o = lane; o < T::NOUT; o += 64) {
+    for (int o = tid; o < T::NOUT; o += T::NT) {
         const int cd_ = o % (T::NC * T::ND), b = (o / (T::NC * T::ND)) % T::NB, a0 = o / (T::NC * T::ND * T::NB);
         int axc, ayc, azc, bx, by, bz, cxc, cyc, czc, dx, dy, dz;
         tri_decode(LA, a0, axc, ayc, azc);
@@ -320,7 +332,7 @@ __device__ __forceinline__ void eri_team_digest(const JKArgs& a, double* __restr
                                                 double* __restrict__ Jm, double* __restrict__ Km,
                                                 const double* __restrict__ Dm, long M) {
     using T = TeamShape<LA, LB, LC, LD>;
-    const int lane = threadIdx.x & 63;
+    const int tid = threadIdx.x;
     const PairData& pd = a.pd;
     double* blk = lds;
     double s = 1.0;
@@ -337,17 +349,17 @@ __device__ __forceinline__ void eri_team_digest(const JKArgs& a, double* __restr
     double* dad = dac + NA * NC;
     double* dbc = dad + NA * ND;
     double* dbd = dbc + NB * NC;
-    for (int t = lane; t < NA * NB; t += 64) dab[t] = Dm[(A0 + t / NB) * M + B0 + t % NB];
-    for (int t = lane; t < NC * ND; t += 64) dcd[t] = Dm[(C0 + t / ND) * M + D0 + t % ND];
-    for (int t = lane; t < NA * NC; t += 64) dac[t] = Dm[(A0 + t / NC) * M + C0 + t % NC];
-    for (int t = lane; t < NA * ND; t += 64) dad[t] = Dm[(A0 + t / ND) * M + D0 + t % ND];
-    for (int t = lane; t < NB * NC; t += 64) dbc[t] = Dm[(B0 + t / NC) * M + C0 + t % NC];
-    for (int t = lane; t < NB * ND; t += 64) dbd[t] = Dm[(B0 + t / ND) * M + D0 + t % ND];
+    for (int t = tid; t < NA * NB; t += T::NT) dab[t] = Dm[(A0 + t / NB) * M + B0 + t % NB];
+    for (int t = tid; t < NC * ND; t += T::NT) dcd[t] = Dm[(C0 + t / ND) * M + D0 + t % ND];
+    for (int t = tid; t < NA * NC; t += T::NT) dac[t] = Dm[(A0 + t / NC) * M + C0 + t % NC];
+    for (int t = tid; t < NA * ND; t += T::NT) dad[t] = Dm[(A0 + t / ND) * M + D0 + t % ND];
+    for (int t = tid; t < NB * NC; t += T::NT) dbc[t] = Dm[(B0 + t / NC) * M + C0 + t % NC];
+    for (int t = tid; t < NB * ND; t += T::NT) dbd[t] = Dm[(B0 + t / ND) * M + D0 + t % ND];
     __syncthreads();
     auto B = [&](int ai, int bi, int ci, int di) { return blk[((ai * NB + bi) * NC + ci) * ND + di]; };
     if (a.do_j) {
         // J[a][b] += 2 s sum_cd (ab|cd) D[c][d];  J[c][d] += 2 s sum_ab (ab|cd) D[a][b]
-        for (int t = lane; t < NA * NB + NC * ND; t += 64) {
+        for (int t = tid; t < NA * NB + NC * ND; t += T::NT) {
             double v = 0.0;
             if (t < NA * NB) {
                 const int ai = t / NB, bi = t % NB;
@@ -368,7 +380,7 @@ __device__ __forceinline__ void eri_team_digest(const JKArgs& a, double* __restr
         // K[a][c] += s sum_bd v D[b][d]; K[a][d] += s sum_bc v D[b][c];
         // K[b][c] += s sum_ad v D[a][d]; K[b][d] += s sum_ac v D[a][c]
         constexpr int n0 = NA * NC, n1 = NA * ND, n2 = NB * NC, n3 = NB * ND;
-        for (int t = lane; t < n0 + n1 + n2 + n3; t += 64) {
+        for (int t = tid; t < n0 + n1 + n2 + n3; t += T::NT) {
             double v = 0.0;
             if (t < n0) {
                 const int ai = t / NC, ci = t % NC;
@@ -406,10 +418,9 @@ __device__ __forceinline__ void eri_team_digest(const JKArgs& a, double* __restr
 // Same task formats, counters, molecule offsets and persistent stride (over
 // work items) as the lane evaluator; FP64 only.
 template <int LA, int LB, int LC, int LD>
-__global__ __launch_bounds__(64) void k_eri_team(JKArgs a) {
+__global__ __launch_bounds__((TeamShape<LA, LB, LC, LD>::NT)) void k_eri_team(JKArgs a) {
     using T = TeamShape<LA, LB, LC, LD>;
     __shared__ double lds[T::LDS];
-    const int lane = threadIdx.x & 63;
     const long long w0 = blockIdx.x;
     const int ns = a.team_slices > 1 ? a.team_slices : 1;
     const long long nwork = (a.tquart ? a.ntquart : static_cast<long long>(a.ntasks) * 64) * ns;
@@ -473,7 +484,7 @@ __global__ __launch_bounds__(64) void k_eri_team(JKArgs a) {
             Jm += base;
             Km += base;
         }
-        if (a.counter && lane == 0 && slice == 0) {
+        if (a.counter && threadIdx.x == 0 && slice == 0) {
             unsigned long long* cnt = a.counter + 4 * (item % kCounterSlots);
             atomicAdd(cnt, 1ull);
             atomicAdd(cnt + 1, static_cast<unsigned long long>(a.pd.pn[Q]) * a.pd.pn[P]);
@@ -488,17 +499,16 @@ __global__ __launch_bounds__(64) void k_eri_team(JKArgs a) {
 // only the team evaluator serves): one wave per quartet, striding over the
 // list when the grid is smaller.
 template <int LA, int LB, int LC, int LD>
-__global__ __launch_bounds__(64) void k_eri_team_store(StoreArgs a) {
+__global__ __launch_bounds__((TeamShape<LA, LB, LC, LD>::NT)) void k_eri_team_store(StoreArgs a) {
     using T = TeamShape<LA, LB, LC, LD>;
     __shared__ double lds[T::LDS];
-    const int lane = threadIdx.x & 63;
     for (long long i = blockIdx.x; i < a.n; i += gridDim.x) {
         const int2 pq = a.quartets[i];
         const int P = __builtin_amdgcn_readfirstlane(pq.x);
         const int Q = __builtin_amdgcn_readfirstlane(pq.y);
         eri_team_block<LA, LB, LC, LD>(a.pd, a.tb, 0.0, lds, P, Q, 0, 1);
         double* out = a.out + (a.off ? a.off[i] : i * a.stride);
-        for (int o = lane; o < T::NOUT; o += 64) out[o] = lds[o];
+        for (int o = threadIdx.x; o < T::NOUT; o += T::NT) out[o] = lds[o];
         __syncthreads();  // the LDS block is reused by the next quartet
     }
 }
