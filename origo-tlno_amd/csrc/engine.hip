@@ -104,6 +104,7 @@ struct DeviceCtx {
     bool ready = false;
     double* scratch = nullptr;  // this device's copy of the static scratch array
     double* boys = nullptr;
+    double* boys_lds = nullptr;
     double* rys = nullptr;
     DevTables tb{};
     // side streams of concurrent builds; a build holds side_mu from its fork
@@ -178,10 +179,13 @@ DeviceCtx& device_ctx(int dev) {
     const HostTables& t = host_tables();
     HIP_OK(hipMalloc(&c.boys, t.boys.size() * sizeof(double)));
     HIP_OK(hipMemcpy(c.boys, t.boys.data(), t.boys.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_OK(hipMalloc(&c.boys_lds, t.boys_lds.size() * sizeof(double)));
+    HIP_OK(hipMemcpy(c.boys_lds, t.boys_lds.data(), t.boys_lds.size() * sizeof(double), hipMemcpyHostToDevice));
     HIP_OK(hipMalloc(&c.rys, t.rys.size() * sizeof(double)));
     HIP_OK(hipMemcpy(c.rys, t.rys.data(), t.rys.size() * sizeof(double), hipMemcpyHostToDevice));
     HIP_OK(hipGetSymbolAddress(reinterpret_cast<void**>(&c.scratch), HIP_SYMBOL(scratch)));
     c.tb.boys = c.boys;
+    c.tb.boys_lds = c.boys_lds;
     c.tb.rys = c.rys;
     for (int i = 0; i < RYS_NMAX + 2; ++i) c.tb.rys_off[i] = t.rys_off[i];
     std::memcpy(c.tb.herm_r2, t.herm_r2, sizeof(c.tb.herm_r2));

@@ -80,6 +80,7 @@ __host__ __device__ constexpr CartT<L> make_cart() {
 // ---------------------------------------------------------------- tables ---
 struct DevTables {
     const double* boys;   // [BOYS_NGRID][BOYS_MMAX+1]
+    const double* boys_lds;  // [BOYS_LDS_NGRID][BOYS_LDS_M], the copy kernels stage in LDS
     const double* rys;    // polynomial pieces, see tables.hpp
     int rys_off[RYS_NMAX + 2];
     double herm_r2[RYS_NMAX + 1][RYS_NMAX];
@@ -106,11 +107,24 @@ __device__ __forceinline__ double rsqrt_nr(double x) {
     return y;
 }
 
-// F_0..F_MMAX at T
+// F_0..F_MMAX at T.  sb: the coarse table staged in LDS by the calling
+// kernel (nullptr: the fine table in global memory)
 template <int MMAX>
-__device__ __forceinline__ void boys_all(const DevTables& tb, double T, double* F) {
+__device__ __forceinline__ void boys_all(const DevTables& tb, double T, double* F, const double* sb = nullptr) {
     static_assert(MMAX + BOYS_TAYLOR - 1 <= BOYS_MMAX, "Boys order beyond the tabulated orders");
-    if (T < BOYS_TCUT) {
+    constexpr bool kLdsOrders = MMAX + BOYS_LDS_TAYLOR - 1 < BOYS_LDS_M;  // the LDS copy holds orders 0..11
+    if (kLdsOrders && sb && T < BOYS_TCUT) {
+        const int k = max(static_cast<int>(T * BOYS_LDS_STEPS + 0.5), 0);
+        const double dd = k * (1.0 / BOYS_LDS_STEPS) - T;
+        const double* row = sb + k * BOYS_LDS_M;
+#pragma unroll
+        for (int m = 0; m <= (kLdsOrders ? MMAX : 0); ++m) {
+            double v = row[m + BOYS_LDS_TAYLOR - 1];
+#pragma unroll
+            for (int j = BOYS_LDS_TAYLOR - 1; j >= 1; --j) v = row[m + j - 1] + v * dd * (1.0 / j);
+            F[m] = v;
+        }
+    } else if (T < BOYS_TCUT) {
         // T >= 0 for every integral argument; the clamp keeps a negative
         // argument from a caller inside the table
         const int k = max(static_cast<int>(T * BOYS_STEPS + 0.5), 0);
@@ -141,10 +155,11 @@ __device__ __forceinline__ void boys_all(const DevTables& tb, double T, double* 
 }
 
 template <int N>
-__device__ __forceinline__ void rys_roots(const DevTables& tb, double T, double* r, double* w) {
+__device__ __forceinline__ void rys_roots(const DevTables& tb, double T, double* r, double* w,
+                                          const double* sb = nullptr) {
     if constexpr (N == 1) {
         double F[2];
-        boys_all<1>(tb, T, F);
+        boys_all<1>(tb, T, F, sb);
         r[0] = F[1] * rcp_nr(F[0]);
         w[0] = F[0];
     } else if constexpr (N == 2) {
@@ -155,7 +170,7 @@ __device__ __forceinline__ void rys_roots(const DevTables& tb, double T, double*
             // values per T instead of 48 interpolation coefficients; the
             // gathers, not the arithmetic, bound the classes that use it)
             double F[4];
-            boys_all<3>(tb, T, F);
+            boys_all<3>(tb, T, F, sb);
             const double id = rcp_nr(F[0] * F[2] - F[1] * F[1]);
             const double c1 = (F[1] * F[2] - F[0] * F[3]) * id;
             const double c0 = (F[1] * F[3] - F[2] * F[2]) * id;
@@ -355,7 +370,7 @@ template <typename R, int LA, int LB, int LC, int LD, int AC, int AN, int CC, in
 __device__ __forceinline__ void quartet(const DevTables& tb, const PairData& pd, int P, int Q, int ib0, int ib1,
                                         int j0, int js, bool uniform,
                                         R (&acc)[Shape<LA, LB, LC, LD, AC, AN, CC, CN>::NOUT],
-                                        double fs_min = 0.0) {
+                                        double fs_min = 0.0, const double* sb = nullptr) {
     using S = Shape<LA, LB, LC, LD, AC, AN, CC, CN>;
     constexpr CartT<LA> ca = make_cart<LA>();
     constexpr CartT<LB> cb = make_cart<LB>();
@@ -392,7 +407,7 @@ __device__ __forceinline__ void quartet(const DevTables& tb, const PairData& pd,
         const double T = p * q * ipq * (PQx * PQx + PQy * PQy + PQz * PQz);
         const double pref = FS ? kTwoPi52 * wab * wcd * sipq * keep : kTwoPi52 * wab * wcd * sipq;
         double rt[S::NR], wt[S::NR];
-        rys_roots<S::NR>(tb, T, rt, wt);
+        rys_roots<S::NR>(tb, T, rt, wt, sb);
         const R qo = R(q * ipq), po = R(p * ipq);
         const R hpq = R(0.5 * ipq), i2p = R(0.5 * rcp_nr(p)), i2q = R(0.5 * rcp_nr(q));
         const R PAx = R(Pp.x - Ax), PAy = R(Pp.y - Ay), PAz = R(Pp.z - Az);
@@ -592,7 +607,7 @@ __device__ __forceinline__ int ket_stride(int nq) {
 // takes ket l % ks of the task (ks = ket_stride(kets)) and share l / ks of
 // the 64 / ks shares of its primitive quartets (see quartet).
 template <typename R, int LA, int LB, int LC, int LD, int AC, int AN, int CC, int CN, bool FS = false>
-__device__ __forceinline__ void eri_jk_task(const JKArgs& a, const int wave) {
+__device__ __forceinline__ void eri_jk_task(const JKArgs& a, const int wave, const double* sb) {
     using S = Shape<LA, LB, LC, LD, AC, AN, CC, CN>;
     const int lane = threadIdx.x & 63;
     const int task = a.qlist && a.split > 0 ? wave / a.split : wave;
@@ -665,7 +680,7 @@ __device__ __forceinline__ void eri_jk_task(const JKArgs& a, const int wave) {
 
     R acc[S::NOUT];
     quartet<R, LA, LB, LC, LD, AC, AN, CC, CN, FS>(a.tb, a.pd, P, Q, ib0, active ? ib1 : ib0, sub, ibs, true, acc,
-                                                   a.fs_min);
+                                                   a.fs_min, sb);
     if (ibs > 1) {
         // the ibs lanes of a ket (lane bits above ks) hold parts of one block:
         // sum them into the first, which alone digests
@@ -827,7 +842,7 @@ __device__ __forceinline__ void eri_jk_task(const JKArgs& a, const int wave) {
 // FS: the instance with the primitive FASTSCHWARZ screen (a kernel of its
 // own, so the unscreened kernels keep their registers and occupancy).
 template <typename R, int LA, int LB, int LC, int LD, int AC, int AN, int CC, int CN, bool FS = false>
-__device__ __forceinline__ void eri_jk_body(const JKArgs& a) {
+__device__ __forceinline__ void eri_jk_body(const JKArgs& a, const double* sb) {
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     // persistent: the grid is whole blocks of four waves, so a cap that is
     // not a multiple of four launches waves beyond the stride; they leave
@@ -835,7 +850,7 @@ __device__ __forceinline__ void eri_jk_body(const JKArgs& a) {
     if (a.stride > 0 && wave >= a.stride) return;
     const int end = a.stride > 0 ? a.nwork : wave + 1;
     const int step = a.stride > 0 ? a.stride : 1;
-    for (int w = wave; w < end; w += step) eri_jk_task<R, LA, LB, LC, LD, AC, AN, CC, CN, FS>(a, w);
+    for (int w = wave; w < end; w += step) eri_jk_task<R, LA, LB, LC, LD, AC, AN, CC, CN, FS>(a, w, sb);
 }
 
 // Writes the lane's slice of the unit-diagonal Cartesian block of (P|Q) into

@@ -55,11 +55,23 @@ struct ChunkPolicy {
     static constexpr int cc(int i) { return mode == 2 ? (i % ncg) * cn : -1; }
 };
 
+// Classes with one or two Rys roots take their roots from the Boys moments
+// F_0..F_3, gathered per primitive quartet: the block stages the coarse Boys
+// table (31.6 KB) in LDS once, and the gathers go to LDS instead of through
+// the texture path, which bounds these classes.
 template <typename R, int LA, int LB, int LC, int LD, bool FS, int C0, int... J>
 __global__ __launch_bounds__(256) void k_eri_jk(JKArgs a) {
     using CP = ChunkPolicy<LA, LB, LC, LD>;
+    constexpr bool kLdsBoys = (LA + LB + LC + LD) / 2 + 1 <= 2;
+    __shared__ double sboys[kLdsBoys ? BOYS_LDS_NGRID * BOYS_LDS_M : 1];
+    const double* sb = nullptr;
+    if constexpr (kLdsBoys) {
+        for (int i = threadIdx.x; i < BOYS_LDS_NGRID * BOYS_LDS_M; i += blockDim.x) sboys[i] = a.tb.boys_lds[i];
+        __syncthreads();
+        sb = sboys;
+    }
     const int ch = C0 + static_cast<int>(blockIdx.y);
-    ((ch == C0 + J ? (eri_jk_body<R, LA, LB, LC, LD, CP::ac(C0 + J), CP::an(C0 + J), CP::cc(C0 + J), CP::cn, FS>(a), 0)
+    ((ch == C0 + J ? (eri_jk_body<R, LA, LB, LC, LD, CP::ac(C0 + J), CP::an(C0 + J), CP::cc(C0 + J), CP::cn, FS>(a, sb), 0)
                    : 0),
      ...);
 }

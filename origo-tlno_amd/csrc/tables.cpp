@@ -187,6 +187,10 @@ HostTables* build_tables() {
     for (int k = 0; k < BOYS_NGRID; ++k)
         for (int m = 0; m <= BOYS_MMAX; ++m)
             t->boys[static_cast<std::size_t>(k) * (BOYS_MMAX + 1) + m] = double(boys_ld(m, ld(k) / BOYS_STEPS));
+    t->boys_lds.assign(static_cast<std::size_t>(BOYS_LDS_NGRID) * BOYS_LDS_M, 0.0);
+    for (int k = 0; k < BOYS_LDS_NGRID; ++k)
+        for (int m = 0; m < BOYS_LDS_M; ++m)
+            t->boys_lds[static_cast<std::size_t>(k) * BOYS_LDS_M + m] = double(boys_ld(m, ld(k) / BOYS_LDS_STEPS));
     // Rys: Chebyshev interpolation at D+1 nodes per unit interval, converted
     // to monomials in u = T - (k + 1/2), all in long double.
     int off = 0;

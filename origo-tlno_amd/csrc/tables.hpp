@@ -31,6 +31,14 @@ constexpr int BOYS_NGRID = 41 * BOYS_STEPS + 1; // T in [0, 41], step 1/32
 constexpr double BOYS_TCUT = 40.0;     // asymptotic + upward recursion beyond
 constexpr int BOYS_TAYLOR = 7;
 
+// The coarse copy the lane kernels with one or two roots stage in LDS:
+// T_k = k/8, orders 0..11, 9-term Taylor expansion (|T - T_k| <= 1/16,
+// remainder <= 16^-9 / 9! < 2e-16 relative) for F_0..F_3; 31.6 KB.
+constexpr int BOYS_LDS_STEPS = 8;
+constexpr int BOYS_LDS_NGRID = 41 * BOYS_LDS_STEPS + 1;
+constexpr int BOYS_LDS_M = 12;
+constexpr int BOYS_LDS_TAYLOR = 9;
+
 constexpr int RYS_NMAX = 9;   // shells up to g: at most 9 roots
 constexpr int RYS_DEG = 11;             // polynomial degree per interval
 constexpr int RYS_NCOEF = RYS_DEG + 1;
@@ -43,6 +51,7 @@ inline int rys_ttab(int N) {
 
 struct HostTables {
     std::vector<double> boys;        // [BOYS_NGRID][BOYS_MMAX+1]
+    std::vector<double> boys_lds;    // [BOYS_LDS_NGRID][BOYS_LDS_M]
     std::vector<double> rys;         // concatenated per N: [nint][2N][RYS_NCOEF]
     int rys_off[RYS_NMAX + 2] = {0}; // offset of N's block in rys
     double herm_r2[RYS_NMAX + 1][RYS_NMAX] = {};
