@@ -219,14 +219,16 @@ def water100_run(gate=1e-6, tau=SCREEN_TAU):
     return out
 
 
-def fshell_run(steps=5, warmup=2, natoms=30, box_A=12.0):
-    """f shells: organic_f(30) (organic30's generator and box with an f
-    polarization shell on the heavy atoms; Cartesian, Schwarz 1e-12), J+K
-    builds from a density resident in HBM after the plan autotuner; the class
-    pairs with an f shell run the team evaluator.  Per class pair: serialized
-    kernel time (HIP events), algorithmic flops (tlno_amd.flops) and rate."""
+def high_l_run(kind="f", steps=5, warmup=2, natoms=30, box_A=12.0):
+    """f or g shells: organic_f(30) / organic_g(30) (organic30's generator and
+    box with an f, or an f and a g, polarization shell on the heavy atoms;
+    Cartesian, Schwarz 1e-12), J+K builds from a density resident in HBM after
+    the plan autotuner; the class pairs with an f or g shell run the team
+    evaluator.  Per class pair: serialized kernel time (HIP events),
+    algorithmic flops (tlno_amd.flops) and rate."""
     lib = _lib.load()
-    b, D = configs.organic_f(natoms=natoms, box_A=box_A)
+    make, shells = {"f": (configs.organic_f, "3s2p1d1f"), "g": (configs.organic_g, "3s2p1d1f1g")}[kind]
+    b, D = make(natoms=natoms, box_A=box_A)
     thr = 1e-12
     plan = DirectPlan(b, ScreeningOptions(integral_threshold=thr), cartesian=True)
     plan.autotune(schwarz=thr, repeats=1)
@@ -269,13 +271,13 @@ def fshell_run(steps=5, warmup=2, natoms=30, box_A=12.0):
                             "prim_quartets": pq[i], "ms": round(ms[i] / nb.value, 4), "gflop": round(fl / 1e9, 3),
                             "tflops": round(fl / t / 1e12, 2) if t > 0 else 0.0})
     classes.sort(key=lambda c: -c["ms"])
-    f_sq = sum(c["shell_quartets"] for c in classes)
-    return {"workload": f"organic_f{natoms} ({natoms} atoms in a {box_A:g} A cube, heavy 3s2p1d1f / H 2s1p, "
+    hi_sq = sum(c["shell_quartets"] for c in classes)
+    return {"workload": f"organic_{kind}{natoms} ({natoms} atoms in a {box_A:g} A cube, heavy {shells} / H 2s1p, "
                         f"{M} Cartesian functions, Schwarz 1e-12)",
             "ms_per_build": sec * 1e3, "builds_per_s": 1.0 / sec, "shell_quartets_per_build": sq_all,
             "prim_quartets_per_build": pq_all, "shell_quartets_per_s": sq_all / sec,
-            "f_share_of_shell_quartets": f_sq / max(1, sq_all),
-            "f_class_serialized_ms": sum(c["ms"] for c in classes), "f_classes": classes}
+            "high_l_share_of_shell_quartets": hi_sq / max(1, sq_all),
+            "high_l_class_serialized_ms": sum(c["ms"] for c in classes), "high_l_classes": classes}
 
 
 def onee_run(repeats=5):
@@ -459,6 +461,7 @@ def main():
     ap.add_argument("--no-water100", action="store_true")
     ap.add_argument("--no-onee", action="store_true")
     ap.add_argument("--no-fshell", action="store_true")
+    ap.add_argument("--no-gshell", action="store_true")
     ap.add_argument("--no-queue-probe", action="store_true",
                     help="skip the organic30 rerun with HIP's default 4 hardware queues")
     ap.add_argument("--routing-profile", type=int, default=0, metavar="BUILDS",
@@ -540,7 +543,10 @@ def main():
         result["onee"] = onee_run()
     if not args.no_fshell:
         print("bench: f-shell builds (organic_f30)", file=sys.stderr, flush=True)
-        result["fshell"] = fshell_run()
+        result["fshell"] = high_l_run("f")
+    if not args.no_gshell:
+        print("bench: g-shell builds (organic_g30)", file=sys.stderr, flush=True)
+        result["gshell"] = high_l_run("g")
     if not args.no_batch:
         print("bench: 256-molecule batch builds", file=sys.stderr, flush=True)
         result["batched"] = batched_run()

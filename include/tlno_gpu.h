@@ -62,10 +62,10 @@ int tlno_gpu_plan_info(const tlno_gpu_plan* plan, int* M, int* n_groups,
 int tlno_gpu_plan_groups(const tlno_gpu_plan* plan, int max_groups, int* L, int* comps);
 
 /* Shell pairs of the plan in the plan's order (class ss, ps, pp, ds, dp, dd,
- * fs, fp, fd, ff, then descending Schwarz bound): bra/ket-side group of each
- * pair (higher L first), Schwarz bound Q (sqrt max|(ab|ab)| in the plan's
- * basis), and the first pair index of each class, class_begin[0..10]
- * (class_begin[10] = number of pairs).  Any pointer may be
+ * fs, fp, fd, ff, gs, gp, gd, gf, gg, then descending Schwarz bound):
+ * bra/ket-side group of each pair (higher L first), Schwarz bound Q (sqrt
+ * max|(ab|ab)| in the plan's basis), and the first pair index of each class,
+ * class_begin[0..15] (class_begin[15] = number of pairs).  Any pointer may be
  * NULL.  Counterpart of the reference's pair-group list and pair_group_max
  * (rys_engine.ipp:2436-2460, 16164-16258). */
 int tlno_gpu_plan_pairs(const tlno_gpu_plan* plan, int* group_a, int* group_b, double* Q,
@@ -225,9 +225,10 @@ int tlno_gpu_last_build_stats(const tlno_gpu_plan* plan, long long* shell_quarte
 int tlno_gpu_set_timing(tlno_gpu_plan* plan, int enable);
 int tlno_gpu_class_stats(const tlno_gpu_plan* plan, long long* shell_quartets,
                          long long* prim_quartets, double* ms, long long* builds);
-/* The same over all ten pair classes (ss ps pp ds dp dd fs fp fd ff): 100
- * entries each, index 10*X + Y for bra class X >= ket class Y (the f class
- * pairs time their team-evaluator launch; other entries 0). */
+/* The same over all fifteen pair classes (ss ps pp ds dp dd fs fp fd ff gs
+ * gp gd gf gg): 225 entries each, index 15*X + Y for bra class X >= ket class
+ * Y (the f and g class pairs time their team-evaluator launch; other
+ * entries 0). */
 int tlno_gpu_class_stats_all(const tlno_gpu_plan* plan, long long* shell_quartets,
                              long long* prim_quartets, double* ms, long long* builds);
 

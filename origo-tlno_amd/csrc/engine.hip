@@ -58,12 +58,12 @@ thread_local std::string g_err;
 constexpr int kSideStreams = tlno_amd_side_streams;
 
 // total angular momentum and Cartesian components of the pair classes ss,
-// ps, pp, ds, dp, dd (lane and team evaluators) and fs, fp, fd, ff (team
-// evaluator only, their pairs after all others in plan order)
-constexpr int kClassLAB[kNumClasses] = {0, 1, 2, 2, 3, 4, 3, 4, 5, 6};
-constexpr int kClassNcart[kNumClasses] = {1, 3, 9, 6, 18, 36, 10, 30, 60, 100};
-constexpr int kClassL[kNumClasses][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1},
-                                         {2, 2}, {3, 0}, {3, 1}, {3, 2}, {3, 3}};
+// ps, pp, ds, dp, dd (lane and team evaluators) and fs, fp, fd, ff, gs, gp,
+// gd, gf, gg (team evaluator only, their pairs after all others in plan order)
+constexpr int kClassLAB[kNumClasses] = {0, 1, 2, 2, 3, 4, 3, 4, 5, 6, 4, 5, 6, 7, 8};
+constexpr int kClassNcart[kNumClasses] = {1, 3, 9, 6, 18, 36, 10, 30, 60, 100, 15, 45, 90, 150, 225};
+constexpr int kClassL[kNumClasses][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}, {3, 0}, {3, 1},
+                                         {3, 2}, {3, 3}, {4, 0}, {4, 1}, {4, 2}, {4, 3}, {4, 4}};
 constexpr int kNumHigh = kNumClasses - kNumLowClasses;
 constexpr int kMaxComp = ncart(kMaxShellL);  // Cartesian components of the largest shell
 
@@ -106,6 +106,7 @@ struct DeviceCtx {
     double* boys = nullptr;
     double* boys_lds = nullptr;
     double* rys = nullptr;
+    double* c2s = nullptr;  // spherical projection table (C2S::c)
     DevTables tb{};
     // side streams of concurrent builds; a build holds side_mu from its fork
     // to its join, so the fork/join events are never re-recorded mid-build
@@ -143,6 +144,13 @@ void ensure_side_streams(DeviceCtx& c) {
     c.nside = n;
 }
 
+// Cartesian -> spherical projection of each shell L (rows s, columns the
+// local Cartesian components), built by c2s_table() below
+struct C2S {
+    double c[kMaxShellL + 1][2 * kMaxShellL + 1][kMaxComp] = {};
+};
+const C2S& c2s_table();
+
 ClassKernels g_kernels[6][6];
 HighKernels g_high[kNumHigh][kNumClasses];
 std::once_flag g_kernels_once;
@@ -167,6 +175,22 @@ void register_all_kernels() {
     register_kernels_hi9a(g_high);
     register_kernels_hi9b(g_high);
     register_kernels_hi9c(g_high);
+    register_kernels_hi10(g_high);
+    register_kernels_hi11a(g_high);
+    register_kernels_hi11b(g_high);
+    register_kernels_hi12a(g_high);
+    register_kernels_hi12b(g_high);
+    register_kernels_hi12c(g_high);
+    register_kernels_hi13a(g_high);
+    register_kernels_hi13b(g_high);
+    register_kernels_hi13c(g_high);
+    register_kernels_hi13d(g_high);
+    register_kernels_hi14a(g_high);
+    register_kernels_hi14b(g_high);
+    register_kernels_hi14c(g_high);
+    register_kernels_hi14d(g_high);
+    register_kernels_hi14e(g_high);
+    register_kernels_hi14f(g_high);
 }
 
 DeviceCtx& device_ctx(int dev) {
@@ -184,6 +208,8 @@ DeviceCtx& device_ctx(int dev) {
     HIP_OK(hipMalloc(&c.rys, t.rys.size() * sizeof(double)));
     HIP_OK(hipMemcpy(c.rys, t.rys.data(), t.rys.size() * sizeof(double), hipMemcpyHostToDevice));
     HIP_OK(hipGetSymbolAddress(reinterpret_cast<void**>(&c.scratch), HIP_SYMBOL(scratch)));
+    HIP_OK(hipMalloc(&c.c2s, sizeof(C2S::c)));
+    HIP_OK(hipMemcpy(c.c2s, c2s_table().c, sizeof(C2S::c), hipMemcpyHostToDevice));
     c.tb.boys = c.boys;
     c.tb.boys_lds = c.boys_lds;
     c.tb.rys = c.rys;
@@ -199,9 +225,6 @@ DeviceCtx& device_ctx(int dev) {
 // solid-harmonic recursion (Helgaker/Jorgensen/Olsen eqs. 6.4.70-6.4.73),
 // normalized in the unit-diagonal metric.  Row s, column = local Cartesian
 // component (descending lex).
-struct C2S {
-    double c[kMaxShellL + 1][2 * kMaxShellL + 1][kMaxComp] = {};
-};
 
 double dfact(int n) {
     double v = 1.0;
@@ -415,6 +438,11 @@ struct tlno_gpu_plan {
     unsigned long long* d_clsmax = nullptr;  // [kNumClasses]
     double* d_pairfac = nullptr;             // [npair]
     int* d_cut = nullptr;                    // [6 npair]
+    // global scratch of the g classes whose HRR and digestion outgrow LDS:
+    // gslots workgroup slots of the largest such class's working set
+    double* d_gwork = nullptr;
+    long long gwork_doubles = 0;
+    int gslots = 0;
     unsigned long long* d_counter = nullptr; // shell quartets evaluated by the last build
     long long last_evaluated = 0, last_evaluated_prim = 0;
     double last_maxD = 0.0;
@@ -496,7 +524,8 @@ bool finite(double v) { return std::isfinite(v); }
 
 int class_of(int LA, int LB) {
     // LA >= LB
-    static const int t[4][4] = {{0, -1, -1, -1}, {1, 2, -1, -1}, {3, 4, 5, -1}, {6, 7, 8, 9}};
+    static const int t[5][5] = {{0, -1, -1, -1, -1}, {1, 2, -1, -1, -1}, {3, 4, 5, -1, -1}, {6, 7, 8, 9, -1},
+                                {10, 11, 12, 13, 14}};
     return t[LA][LB];
 }
 
@@ -527,7 +556,7 @@ std::vector<GroupH> build_groups(int n, const double* centers, const int* ang, c
         const int L = a[0] + a[1] + a[2];
         if (L > kMaxShellL)
             throw std::runtime_error(
-                "eri_ao_tensor_hgp: this build supports shell L=0..3 (s, p, d, f).");
+                "eri_ao_tensor_hgp: this build supports shell L=0..4 (s, p, d, f, g).");
         const double na = 1.0 / std::sqrt(dfact(a[0]) * dfact(a[1]) * dfact(a[2]));
         int gi = -1;
         for (int g = 0; g < static_cast<int>(groups.size()) && gi < 0; ++g) {
@@ -574,8 +603,13 @@ std::vector<GroupH> build_groups(int n, const double* centers, const int* ang, c
 }
 
 // ------------------------------------------------------ transform kernels --
+// The projection table on the device ([L][s][component], the layout of
+// C2S::c), passed by pointer: for shells up to g it outgrows a kernel argument.
 struct C2SArgs {
-    double c[kMaxShellL + 1][2 * kMaxShellL + 1][kMaxComp];
+    const double* c;
+    __device__ double at(int L, int s, int comp) const {
+        return c[(L * (2 * kMaxShellL + 1) + s) * kMaxComp + comp];
+    }
 };
 
 // D_int[i][j] = D_user[perm[i]][perm[j]]  (Cartesian mode)
@@ -597,8 +631,8 @@ __global__ void k_prep_sph(const double* __restrict__ Dsph, double* __restrict__
     double v = 0.0;
     for (int s = 0; s < 2 * Li + 1; ++s) {
         double t2 = 0.0;
-        for (int t = 0; t < 2 * Lj + 1; ++t) t2 += row[(long)s * Ms + t] * c2s.c[Lj][t][lj];
-        v += c2s.c[Li][s][li] * t2;
+        for (int t = 0; t < 2 * Lj + 1; ++t) t2 += row[(long)s * Ms + t] * c2s.at(Lj, t, lj);
+        v += c2s.at(Li, s, li) * t2;
     }
     Dint[idx] = v;
 }
@@ -621,12 +655,12 @@ __global__ void k_fin_sph(const double* __restrict__ Xacc, double* __restrict__ 
     const int iu = sib[u], iv = sib[v];
     double acc = 0.0;
     for (int a = 0; a < ncart(Lu); ++a) {
-        const double ca = c2s.c[Lu][su][a];
+        const double ca = c2s.at(Lu, su, a);
         if (ca == 0.0) continue;
         double t = 0.0;
         for (int b = 0; b < ncart(Lv); ++b) {
             const long ij = (long)(iu + a) * Mc + (iv + b), ji = (long)(iv + b) * Mc + (iu + a);
-            t += c2s.c[Lv][sv][b] * (Xacc[ij] + Xacc[ji]);
+            t += c2s.at(Lv, sv, b) * (Xacc[ij] + Xacc[ji]);
         }
         acc += ca * t;
     }
@@ -768,14 +802,43 @@ void launch_jk(const ClassKernels& k, bool fp32, JKArgs& args, long long nwaves,
     args.nwork = saved_nwork;
 }
 
-// team kernel of a class pair with an f shell over its quartet list (one
-// workgroup of k.threads per quartet and slice; capped grids stride)
-void launch_high(const HighKernels& k, JKArgs& args, hipStream_t st) {
+// Scratch for the g classes whose HRR and digestion work in global memory
+// (HighKernels::gwork doubles per workgroup): one slot per CU, allocated on
+// first use and kept with the plan.
+void ensure_gwork(tlno_gpu_plan* pl, long long per_slot) {
+    if (per_slot <= 0) return;
+    if (pl->gslots == 0) {
+        int ncu = 0;
+        HIP_OK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, pl->device));
+        pl->gslots = std::max(1, ncu);
+    }
+    const long long need = per_slot * pl->gslots;
+    if (need <= pl->gwork_doubles) return;
+    require_alloc("global scratch of the g classes");
+    if (pl->d_gwork) HIP_OK(hipFree(pl->d_gwork));
+    pl->d_gwork = nullptr;
+    pl->gwork_doubles = 0;
+    HIP_OK(hipMalloc(&pl->d_gwork, static_cast<std::size_t>(need) * sizeof(double)));
+    pl->gwork_doubles = need;
+}
+
+// team kernel of a class pair with an f or g shell over its quartet list
+// (one workgroup of k.threads per quartet and slice; capped grids stride).
+// Classes whose HRR and digestion work in global memory take one scratch
+// slot per workgroup: the grid is capped at gslots workgroups striding over
+// the quartets, and the caller runs such launches one after another (they
+// share the scratch).
+void launch_high(const HighKernels& k, JKArgs& args, hipStream_t st, double* gwork, int gslots) {
     void* params[] = {&args};
     const long long ns = std::max(1, args.team_slices);
     const long long work = args.ntquart * ns;
     if (work <= 0) return;
-    const long long cap = max_grid_waves();
+    long long cap = max_grid_waves();
+    if (k.gwork > 0) {
+        if (!gwork || gslots <= 0) throw std::runtime_error("tlno_gpu: g-class scratch not prepared");
+        cap = std::min<long long>(cap, gslots);
+        args.gwork = gwork;
+    }
     args.stride = work > cap ? static_cast<int>(cap) : 0;
     HIP_OK(hipLaunchKernel(reinterpret_cast<const void*>(k.team), dim3(static_cast<unsigned>(std::min(work, cap))),
                            dim3(k.threads), params, 0, st));
@@ -1149,10 +1212,16 @@ void run_store(tlno_gpu_plan* pl, int X, int Y, const std::vector<int2>& qs, con
     a.n = static_cast<int>(qs.size());
     a.out = d_out;
     if (X >= kNumLowClasses) {
-        // classes with an f shell: team-evaluator blocks, one workgroup per quartet
+        // classes with an f or g shell: team-evaluator blocks, one workgroup
+        // per quartet (a scratch slot each where the working set is global)
         void* params[] = {&a};
-        const long long nb = std::min<long long>(a.n, max_grid_waves());
         const HighKernels& hk = g_high[X - kNumLowClasses][Y];
+        long long nb = std::min<long long>(a.n, max_grid_waves());
+        if (hk.gwork > 0) {
+            ensure_gwork(pl, hk.gwork);
+            a.gwork = pl->d_gwork;
+            nb = std::min<long long>(nb, pl->gslots);
+        }
         HIP_OK(hipLaunchKernel(reinterpret_cast<const void*>(hk.store), dim3(static_cast<unsigned>(nb)),
                                dim3(hk.threads), params, 0, pl->stream));
     } else {
@@ -2000,8 +2069,7 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
     const hipStream_t st = pl->stream;
     const long mc2 = static_cast<long>(pl->Mcart) * pl->Mcart;
     const long mo2 = static_cast<long>(pl->Mout) * pl->Mout;
-    C2SArgs c2s;
-    std::memcpy(c2s.c, c2s_table().c, sizeof(c2s.c));
+    const C2SArgs c2s{ctx.c2s};
     if (pl->spherical)
         k_prep_sph<<<grid_for(mc2, 256), 256, 0, st>>>(dD, bb.Dint, pl->d_intL, pl->d_intloc, pl->d_intsb,
                                                        pl->Mcart, pl->Msph, c2s);
@@ -2292,6 +2360,7 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
                 }
                 if (njobs == kMaxJobs) throw std::runtime_error("tlno_gpu: too many class launches");
                 const double cost = 1e-6 * static_cast<double>(n) * kClassNcart[X] * kClassNcart[Y];
+                ensure_gwork(pl, g_high[h][Y].gwork);
                 jobs[njobs++] = Job{X, Y, 1, false, ah, n, cost, true};
             }
         if (concurrent) {
@@ -2305,9 +2374,12 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
                 int best = 0;
                 for (int q = 1; q < ctx.nside; ++q)
                     if (load[q] < load[best]) best = q;
+                // the g classes with global working sets share the plan's
+                // scratch: all on the first side stream, one after another
+                if (j.high && g_high[j.X - kNumLowClasses][j.Y].gwork > 0) best = 0;
                 load[best] += j.cost;
                 if (j.high)
-                    launch_high(g_high[j.X - kNumLowClasses][j.Y], j.args, ctx.side[best]);
+                    launch_high(g_high[j.X - kNumLowClasses][j.Y], j.args, ctx.side[best], pl->d_gwork, pl->gslots);
                 else
                     launch_jk(g_kernels[j.X][j.Y], j.fp32, j.args, j.nwaves, ctx.side[best], j.team_min);
             }
@@ -2318,7 +2390,7 @@ void jk_device(tlno_gpu_plan* pl, const double* dD, double* dJ, double* dK, doub
                 if (j.high) {
                     hipEvent_t* e = pl->hev[j.X - kNumLowClasses][j.Y];
                     if (e[0]) HIP_OK(hipEventRecord(e[0], st));
-                    launch_high(g_high[j.X - kNumLowClasses][j.Y], j.args, st);
+                    launch_high(g_high[j.X - kNumLowClasses][j.Y], j.args, st, pl->d_gwork, pl->gslots);
                     if (e[1]) HIP_OK(hipEventRecord(e[1], st));
                     continue;
                 }
@@ -3045,7 +3117,7 @@ tlno_gpu_plan::~tlno_gpu_plan() {
     f(d_int2basis); f(d_intL); f(d_intloc); f(d_intsb); f(d_sphL); f(d_sphloc); f(d_sphib);
     f(dDin); f(dDint); f(dJacc); f(dKacc); f(dJout); f(dKout); f(d_scalar);
     f(d_row_group); f(d_pgA); f(d_pgB); f(d_pcls); f(d_clsb); f(d_Q); f(d_gmax); f(d_clsmax); f(d_pairfac);
-    f(d_cut); f(d_counter);
+    f(d_cut); f(d_counter); f(d_gwork);
     f(d_pf0); f(d_pf1); f(d_pcen); f(d_grows); f(d_blk); f(d_bsum); f(d_frag); f(d_mvec); f(d_mnorm);
     f(d_qlist); f(d_t64); f(d_t32); f(d_rstats); f(d_cand);
     f(d_onee_list); f(d_onee_X); f(d_onee_O); f(d_onee_nuc); f(d_decay);
@@ -3621,8 +3693,7 @@ int tlno_gpu_onee_screened(tlno_gpu_plan* pl, int n_nuc, const double* Z, const 
         launch_onee(pl->pair_data(), ctx.tb, pl->dpL, pl->dpexp, dR, dZ, n_nuc, pl->d_onee_list, pl->onee_cb, vthr,
                     pl->Mcart, dX, dX + mc2, dX + 2 * mc2, st);
         HIP_OK(hipEventRecord(pl->onee_ev[1], st));
-        C2SArgs c2s;
-        std::memcpy(c2s.c, c2s_table().c, sizeof(c2s.c));
+        const C2SArgs c2s{ctx.c2s};
         for (int m = 0; m < 3; ++m) {
             if (pl->spherical)
                 k_fin_sph<<<grid_for(mo2, 256), 256, 0, st>>>(dX + m * mc2, dO + m * mo2, pl->d_sphL, pl->d_sphloc,

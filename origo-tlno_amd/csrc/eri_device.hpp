@@ -571,6 +571,9 @@ struct JKArgs {
     double stau;
     const double* sdecay;
     double sdecay_thr;
+    // team evaluator classes whose HRR and digestion work in global memory
+    // (TeamShape::GWORK): one scratch slot per workgroup, gridDim.x <= slots
+    double* gwork;
 };
 
 // Prefactor of the FASTSCHWARZ primitive-pair estimate, sqrt(2) pi^{5/2}
@@ -585,6 +588,7 @@ struct StoreArgs {
     long long stride;
     int n;
     double* out;
+    double* gwork;          // team classes with global working sets: one slot per workgroup
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -897,13 +901,14 @@ struct HighKernels {
     JKKernel team = nullptr;
     StoreKernel store = nullptr;
     int nout = 0;
-    int threads = 64;  // threads per quartet (workgroup size): 64 or 256
+    int threads = 64;  // threads per quartet (workgroup size): 64 to 384
+    long long gwork = 0;  // doubles of global scratch per workgroup (0: LDS only)
 };
 // pair classes: ss ps pp ds dp dd (the lane and team evaluators), then
-// fs fp fd ff (team evaluator only)
+// fs fp fd ff, gs gp gd gf gg (team evaluator only)
 constexpr int kNumLowClasses = 6;
-constexpr int kNumClasses = 10;
-constexpr int kMaxShellL = 3;
+constexpr int kNumClasses = 15;
+constexpr int kMaxShellL = 4;
 
 struct ClassKernels {
     int nchunk = 0;
@@ -926,7 +931,8 @@ TLNO_AMD_REG(5)
 TLNO_AMD_REG(5b)
 TLNO_AMD_REG(5c)
 #undef TLNO_AMD_REG
-// pair classes with an f shell: 6 fs, 7 fp, 8 fd, 9 ff (team evaluator only)
+// pair classes with an f or g shell: 6 fs, 7 fp, 8 fd, 9 ff, 10 gs, 11 gp,
+// 12 gd, 13 gf, 14 gg (team evaluator only)
 #define TLNO_AMD_REG_HI(tag) void register_kernels_##tag(HighKernels (&tab)[kNumClasses - kNumLowClasses][kNumClasses]);
 TLNO_AMD_REG_HI(hi6)
 TLNO_AMD_REG_HI(hi7)
@@ -935,6 +941,22 @@ TLNO_AMD_REG_HI(hi8b)
 TLNO_AMD_REG_HI(hi9a)
 TLNO_AMD_REG_HI(hi9b)
 TLNO_AMD_REG_HI(hi9c)
+TLNO_AMD_REG_HI(hi10)
+TLNO_AMD_REG_HI(hi11a)
+TLNO_AMD_REG_HI(hi11b)
+TLNO_AMD_REG_HI(hi12a)
+TLNO_AMD_REG_HI(hi12b)
+TLNO_AMD_REG_HI(hi12c)
+TLNO_AMD_REG_HI(hi13a)
+TLNO_AMD_REG_HI(hi13b)
+TLNO_AMD_REG_HI(hi13c)
+TLNO_AMD_REG_HI(hi13d)
+TLNO_AMD_REG_HI(hi14a)
+TLNO_AMD_REG_HI(hi14b)
+TLNO_AMD_REG_HI(hi14c)
+TLNO_AMD_REG_HI(hi14d)
+TLNO_AMD_REG_HI(hi14e)
+TLNO_AMD_REG_HI(hi14f)
 #undef TLNO_AMD_REG_HI
 
 }  // namespace tlno_amd

@@ -127,8 +127,8 @@ void reg_class(ClassKernels (&tab)[6][6]) {
 }
 
 // (la, lb) of every pair class, the f classes after the six of the lane evaluator
-constexpr int kClassLAll[kNumClasses][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1},
-                                            {2, 2}, {3, 0}, {3, 1}, {3, 2}, {3, 3}};
+constexpr int kClassLAll[kNumClasses][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}, {3, 0}, {3, 1},
+                                            {3, 2}, {3, 3}, {4, 0}, {4, 1}, {4, 2}, {4, 3}, {4, 4}};
 
 template <int X, int Y>
 void reg_high(HighKernels (&tab)[kNumClasses - kNumLowClasses][kNumClasses]) {
@@ -138,6 +138,7 @@ void reg_high(HighKernels (&tab)[kNumClasses - kNumLowClasses][kNumClasses]) {
     k.store = k_eri_team_store<LA, LB, LC, LD>;
     k.nout = ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD);
     k.threads = TeamShape<LA, LB, LC, LD>::NT;
+    k.gwork = TeamShape<LA, LB, LC, LD>::WORK;
 }
 
 }  // namespace tlno_amd

@@ -1,6 +1,6 @@
-// Team evaluator for high angular momentum classes: one wave (four for the
-// largest f classes, TeamShape::TW) per shell quartet, with the horizontal
-// recurrence moved after the contraction.
+// Team evaluator for high angular momentum classes: one wave (four or six for
+// the largest f and g classes, TeamShape::TW) per shell quartet, with the
+// horizontal recurrence moved after the contraction.
 //
 // For bra shells (LA, LB) and ket shells (LC, LD) the contracted quartet is
 //   (ab|cd) = HRR_bra HRR_ket [e|f],   [e|f] = sum_prim sum_root Gx Gy (w Gz),
@@ -15,7 +15,8 @@
 //            table rows of e are one read each per combination) and takes
 //            every NP-th combination of the batch; the NP parts are summed
 //            with cross-lane shuffles after the contraction;
-//   phase C  [e|f] goes to LDS, each lane expands its slice of the outputs by
+//   phase C  [e|f] goes to LDS (to a per-workgroup global slot for the g
+//            classes whose working set exceeds 144 KB), each lane expands its slice of the outputs by
 //            the binomial form of the HRR, (a, b| = sum_{t <= b} C(b, t) AB^(b-t) [a+t|
 //            (per axis), the same on the ket;
 //   phase D  the block sits in LDS; lanes reduce J and K blocks with the
@@ -50,24 +51,35 @@ struct TeamShape {
     static constexpr int NI = NE * NF;                     // [e|f] intermediates
     static constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
     static constexpr int NOUT = NA * NB * NC * ND;
-    // waves per shell quartet: 4 for the f classes whose bra intermediates
-    // outnumber a wave's lanes ((ff| has 74) or whose block exceeds 3,000
-    // values -- their HRR and digestion, over up to 10,000 values, dominate
-    // (f shells carry few primitives), and their LDS leaves one workgroup per
-    // CU (the first wave alone evaluates the roots and tables, so classes of
-    // 1,800-2,200 values with long contractions, (fd|fp) and (fd|dd), measured
-    // 35-50 % slower on four waves); one wave for every class up to d
-    static constexpr int TW = (NE > 64 || NOUT > 3000) ? 4 : 1;
+    // ket intermediates one thread holds through the contraction: up to 80
+    // in registers; the g kets (gf| and (gg| (100, 145) split into two
+    // f-chunks held by different waves
+    static constexpr int NCH = (NF + 79) / 80;
+    static constexpr int FCH = (NF + NCH - 1) / NCH;
+    static_assert(NCH <= 2, "at most two f-chunks");
+    static constexpr int EPAD = 64 * ((NE + 63) / 64);
+    // waves per shell quartet: one thread per (bra intermediate, f-chunk) for
+    // the f and g classes whose bra intermediates outnumber a wave's lanes
+    // ((ff| has 74, (gg| 145), whose kets need f-chunks, or whose block
+    // exceeds 3,000 values -- their HRR and digestion, over up to 50,625
+    // values, dominate (f and g shells carry few primitives), and their LDS
+    // leaves one workgroup per CU; at least four waves there (the first wave
+    // alone evaluates the roots and tables, so classes of 1,800-2,200 values
+    // with long contractions, (fd|fp) and (fd|dd), measured 35-50 % slower on
+    // four waves and keep one); one wave for every class up to d
+    static constexpr int TW = (NE > 64 || NOUT > 3000 || NCH > 1)
+                                  ? (EPAD / 64 * NCH > 4 ? EPAD / 64 * NCH : 4) : 1;
     static constexpr int NT = 64 * TW;                     // threads per quartet
     // phase B ownership: thread = (bra intermediate e < EP, part); a thread
-    // holds the NF intermediates [e|f] of its e and takes every NP-th
-    // combination (four-wave quartets: one thread per e, every combination)
-    static constexpr int EP = TW > 1 ? NT : (NE <= 8 ? 8 : (NE <= 16 ? 16 : (NE <= 32 ? 32 : 64)));
+    // holds its f-chunk of the intermediates [e|f] of its e and takes every
+    // NP-th combination (several waves: thread = (f-chunk, e), every
+    // combination)
+    static constexpr int EP = TW > 1 ? EPAD : (NE <= 8 ? 8 : (NE <= 16 ? 16 : (NE <= 32 ? 32 : 64)));
     static constexpr int NP = TW > 1 ? 1 : 64 / EP;
-    // more bra intermediates than threads: rounds of NT, each round evaluating
+    // more bra intermediates than threads: rounds, each round evaluating
     // every combination again, each round's rows [e|f] waiting in LDS ahead of
-    // the tables (not reached for L <= 3: (ff| has 74 on four waves)
-    static constexpr int NRND = (NE + NT - 1) / NT;
+    // the tables (not reached: one wave takes at most 64, several waves EPAD)
+    static constexpr int NRND = TW > 1 ? 1 : (NE + 63) / 64;
     static constexpr int OPL = (NOUT + 63) / 64;
     static constexpr int GS = (LAB + 1) * (LCD + 1);       // one axis table
     // one combination's three tables, padded to an odd number of doubles:
@@ -90,11 +102,15 @@ struct TeamShape {
     static constexpr int LDS_D = NOUT + NA * NB + NC * ND + NA * NC + NA * ND + NB * NC + NB * ND;
     static constexpr int A0 = NRND > 1 ? NI : 0;            // phase A tables start here
     static constexpr int LDS_CD = LDS_C > LDS_D ? LDS_C : LDS_D;
-    static constexpr int LDS = A0 + LDS_A > LDS_CD ? A0 + LDS_A : LDS_CD;
-    // gfx950 gives a workgroup up to 160 KB of LDS; the largest f class,
-    // (ff|ff), needs 139 KB (one wave per CU)
+    // phases C and D work in LDS while their working set fits (gfx950 gives a
+    // workgroup up to 160 KB; (ff|ff) needs 139 KB), else in a global scratch
+    // slot of the workgroup (the large g classes: (gg|gg) needs 843 KB)
+    static constexpr bool GWORK = LDS_CD * 8 > 144 * 1024;
+    static constexpr int WORK = GWORK ? LDS_CD : 0;        // doubles per scratch slot
+    static constexpr int LDS = GWORK ? A0 + LDS_A : (A0 + LDS_A > LDS_CD ? A0 + LDS_A : LDS_CD);
     static_assert(LDS * 8 <= 160 * 1024, "team evaluator LDS beyond 160 KB");
 };
+
 
 // binomial HRR weight C(n, t) X^(n - t) for n <= 4 (the power n - t decides
 // the coefficient: C(n, n-1) = n, C(n, n-2) = n(n-1)/2, C(n, n-3) = 4 for
@@ -129,6 +145,15 @@ struct TriTab {
     }
 };
 
+// acc[f - F0] += Gx[ex][fx] Gy[ey][fy] Gz[ez][fz] for the ket intermediates
+// f in [F0, F1) (compile-time triples, register accumulators)
+template <int LC, int LD, int F0, int F1>
+__device__ __forceinline__ void team_contract(const double* rx, const double* ry, const double* rz, double* acc) {
+    constexpr TriTab<LC, LC + LD> ft{};
+#pragma unroll
+    for (int f = F0; f < F1; ++f) acc[f - F0] = fma(rx[ft.x[f]] * ry[ft.y[f]], rz[ft.z[f]], acc[f - F0]);
+}
+
 // triple of list position p (|e| from LMIN), decoded arithmetically
 __device__ inline void tri_decode(int LMIN, int p, int& x, int& y, int& z) {
     int L = LMIN, base = 0;
@@ -152,15 +177,17 @@ __device__ inline void tri_decode(int LMIN, int p, int& x, int& y, int& z) {
 // slice holds no combination.
 template <int LA, int LB, int LC, int LD>
 __device__ __forceinline__ bool eri_team_block(const PairData& pd, const DevTables& tb, double fs_min,
-                                               double* __restrict__ lds, int P, int Q, int slice, int ns) {
+                                               double* __restrict__ lds, double* __restrict__ work, int P, int Q,
+                                               int slice, int ns) {
     using T = TeamShape<LA, LB, LC, LD>;
     const int tid = threadIdx.x;  // 0 .. NT - 1
     const int lane = tid & 63;
 
-    // owned intermediates [e|f] for all f: their factors are the rows
-    // Gx[ex][*], Gy[ey][*], Gz[ez][*] of a combination's tables
-    constexpr TriTab<LC, LC + LD> ft{};
-    const int part = tid / T::EP;
+    // owned intermediates [e|f] for the f of the thread's chunk: their
+    // factors are the rows Gx[ex][*], Gy[ey][*], Gz[ez][*] of a combination's
+    // tables; the f-chunk is uniform over a wave (EP is a multiple of 64)
+    const int part = T::TW > 1 ? 0 : tid / T::EP;
+    const int chunk = T::TW > 1 ? tid / T::EP : 0;
 
     const double Ax = pd.A[3 * P], Ay = pd.A[3 * P + 1], Az = pd.A[3 * P + 2];
     const double Cx = pd.A[3 * Q], Cy = pd.A[3 * Q + 1], Cz = pd.A[3 * Q + 2];
@@ -176,16 +203,16 @@ __device__ __forceinline__ bool eri_team_block(const PairData& pd, const DevTabl
     if (cbeg >= ncomb) return false;  // the whole block leaves together
 
     double* const tab = lds + T::A0;  // phase A tables (after the rows of earlier rounds)
-    double* ef = lds;                 // NI, [e][f]
+    double* ef = work;                // NI, [e][f]
 #pragma unroll 1
     for (int rnd = 0; rnd < T::NRND; ++rnd) {
     const int e_own = rnd * T::NT + tid % T::EP;
     int ex, ey, ez;
     tri_decode(LA, e_own < T::NE ? e_own : 0, ex, ey, ez);
     const int rx0 = ex * (T::LCD + 1), ry0 = T::GS + ey * (T::LCD + 1), rz0 = 2 * T::GS + ez * (T::LCD + 1);
-    double acc[T::NF];
+    double acc[T::FCH];
 #pragma unroll
-    for (int f = 0; f < T::NF; ++f) acc[f] = 0.0;
+    for (int f = 0; f < T::FCH; ++f) acc[f] = 0.0;
     for (int c0 = cbeg; c0 < ncomb; c0 += T::CB) {
         // ---- phase A: one combination per lane of the first wave; with
         // 32-combination batches the upper half of the wave evaluates the z
@@ -248,7 +275,7 @@ __device__ __forceinline__ bool eri_team_block(const PairData& pd, const DevTabl
         __syncthreads();
         // ---- phase B: accumulate the owned intermediates over this part's
         // combinations of the batch (three LDS rows per combination)
-        const int nb = min(T::CB, ncomb - c0);
+        const int nb = (T::TW == 1 || chunk < T::NCH) ? min(T::CB, ncomb - c0) : 0;
 #pragma unroll 2
         for (int j = part; j < nb; j += T::NP) {
             const double* g = tab + j * T::CS;
@@ -259,8 +286,12 @@ __device__ __forceinline__ bool eri_team_block(const PairData& pd, const DevTabl
                 ry[t] = g[ry0 + t];
                 rz[t] = g[rz0 + t];
             }
-#pragma unroll
-            for (int f = 0; f < T::NF; ++f) acc[f] = fma(rx[ft.x[f]] * ry[ft.y[f]], rz[ft.z[f]], acc[f]);
+            if constexpr (T::NCH == 1)
+                team_contract<LC, LD, 0, T::NF>(rx, ry, rz, acc);
+            else if (chunk == 0)
+                team_contract<LC, LD, 0, T::FCH>(rx, ry, rz, acc);
+            else
+                team_contract<LC, LD, T::FCH, T::NF>(rx, ry, rz, acc);
         }
         __syncthreads();
     }
@@ -268,23 +299,25 @@ __device__ __forceinline__ bool eri_team_block(const PairData& pd, const DevTabl
     // sum the parts (lane bits above EP) into part 0, which stores its row
     // (this round's rows: the tables of a later round sit after all rows)
 #pragma unroll
-    for (int f = 0; f < T::NF; ++f) {
+    for (int f = 0; f < T::FCH; ++f) {
         double v = acc[f];
 #pragma unroll
         for (int m = T::EP; m < 64; m <<= 1) v += __shfl_xor(v, m, 64);
         acc[f] = v;
     }
-    if (part == 0 && e_own < T::NE) {
+    if (part == 0 && e_own < T::NE && chunk < T::NCH) {
+        const int f0 = chunk * T::FCH;
 #pragma unroll
-        for (int f = 0; f < T::NF; ++f) ef[e_own * T::NF + f] = acc[f];
+        for (int f = 0; f < T::FCH; ++f)
+            if (f0 + f < T::NF) ef[e_own * T::NF + f0 + f] = acc[f];
     }
     }  // rounds
     __syncthreads();
 
     // ---- phase C: HRR from [e|f] to the unit-diagonal Cartesian block, ket
     // side first ([e|f] -> [e|cd] for every e), then bra side
-    double* xk = lds + T::R0;     // NX, [e][c][d]
-    double* blk = lds;            // NOUT, [a][b][c][d] (over [e|f], read before)
+    double* xk = work + T::R0;    // NX, [e][c][d]
+    double* blk = work;           // NOUT, [a][b][c][d] (over [e|f], read before)
     const double AB[3] = {pd.AB[3 * P], pd.AB[3 * P + 1], pd.AB[3 * P + 2]};
     const double CD[3] = {pd.AB[3 * Q], pd.AB[3 * Q + 1], pd.AB[3 * Q + 2]};
     auto hw = [](int n, int t, double X) { return hrr_weight(n, t, X); };
@@ -328,13 +361,13 @@ __device__ __forceinline__ bool eri_team_block(const PairData& pd, const DevTabl
 // Phase D: digestion of the block in lds[0, NOUT) (J and K element sums),
 // the contract of the lane evaluator (eri_jk_task).
 template <int LA, int LB, int LC, int LD>
-__device__ __forceinline__ void eri_team_digest(const JKArgs& a, double* __restrict__ lds, int P, int Q,
+__device__ __forceinline__ void eri_team_digest(const JKArgs& a, double* __restrict__ work, int P, int Q,
                                                 double* __restrict__ Jm, double* __restrict__ Km,
                                                 const double* __restrict__ Dm, long M) {
     using T = TeamShape<LA, LB, LC, LD>;
     const int tid = threadIdx.x;
     const PairData& pd = a.pd;
-    double* blk = lds;
+    double* blk = work;
     double s = 1.0;
     if (pd.diag[P]) s *= 0.5;
     if (pd.diag[Q]) s *= 0.5;
@@ -421,6 +454,7 @@ template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__((TeamShape<LA, LB, LC, LD>::NT)) void k_eri_team(JKArgs a) {
     using T = TeamShape<LA, LB, LC, LD>;
     __shared__ double lds[T::LDS];
+    double* const work = T::GWORK ? a.gwork + static_cast<long long>(blockIdx.x) * T::WORK : lds;
     const long long w0 = blockIdx.x;
     const int ns = a.team_slices > 1 ? a.team_slices : 1;
     const long long nwork = (a.tquart ? a.ntquart : static_cast<long long>(a.ntasks) * 64) * ns;
@@ -489,8 +523,8 @@ __global__ __launch_bounds__((TeamShape<LA, LB, LC, LD>::NT)) void k_eri_team(JK
             atomicAdd(cnt, 1ull);
             atomicAdd(cnt + 1, static_cast<unsigned long long>(a.pd.pn[Q]) * a.pd.pn[P]);
         }
-        if (eri_team_block<LA, LB, LC, LD>(a.pd, a.tb, a.fs_min, lds, P, Q, slice, ns))
-            eri_team_digest<LA, LB, LC, LD>(a, lds, P, Q, Jm, Km, Dm, M);
+        if (eri_team_block<LA, LB, LC, LD>(a.pd, a.tb, a.fs_min, lds, work, P, Q, slice, ns))
+            eri_team_digest<LA, LB, LC, LD>(a, work, P, Q, Jm, Km, Dm, M);
     }
 }
 
@@ -502,13 +536,14 @@ template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__((TeamShape<LA, LB, LC, LD>::NT)) void k_eri_team_store(StoreArgs a) {
     using T = TeamShape<LA, LB, LC, LD>;
     __shared__ double lds[T::LDS];
+    double* const work = T::GWORK ? a.gwork + static_cast<long long>(blockIdx.x) * T::WORK : lds;
     for (long long i = blockIdx.x; i < a.n; i += gridDim.x) {
         const int2 pq = a.quartets[i];
         const int P = __builtin_amdgcn_readfirstlane(pq.x);
         const int Q = __builtin_amdgcn_readfirstlane(pq.y);
-        eri_team_block<LA, LB, LC, LD>(a.pd, a.tb, 0.0, lds, P, Q, 0, 1);
+        eri_team_block<LA, LB, LC, LD>(a.pd, a.tb, 0.0, lds, work, P, Q, 0, 1);
         double* out = a.out + (a.off ? a.off[i] : i * a.stride);
-        for (int o = threadIdx.x; o < T::NOUT; o += T::NT) out[o] = lds[o];
+        for (int o = threadIdx.x; o < T::NOUT; o += T::NT) out[o] = work[o];
         __syncthreads();  // the LDS block is reused by the next quartet
     }
 }

@@ -217,8 +217,8 @@ void launch_onee(const PairData& pd, const DevTables& tb, const int* dL, const d
                  const double* dZ, int nnuc, const int2* dlist, const int* class_begin, double vthr, int M,
                  double* S, double* T, double* V, hipStream_t st) {
     (void)dL;
-    static const int LAB[kNumClasses][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1},
-                                            {2, 2}, {3, 0}, {3, 1}, {3, 2}, {3, 3}};
+    static const int LAB[kNumClasses][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}, {3, 0}, {3, 1},
+                                            {3, 2}, {3, 3}, {4, 0}, {4, 1}, {4, 2}, {4, 3}, {4, 4}};
     for (int c = 0; c < kNumClasses; ++c) {
         const int n = class_begin[c + 1] - class_begin[c];
         if (n <= 0) continue;
@@ -248,7 +248,12 @@ void launch_onee(const PairData& pd, const DevTables& tb, const int* dL, const d
             case 6: k_onee<3, 0><<<grid, 256, 0, st>>>(a); break;
             case 7: k_onee<3, 1><<<grid, 256, 0, st>>>(a); break;
             case 8: k_onee<3, 2><<<grid, 256, 0, st>>>(a); break;
-            default: k_onee<3, 3><<<grid, 256, 0, st>>>(a); break;
+            case 9: k_onee<3, 3><<<grid, 256, 0, st>>>(a); break;
+            case 10: k_onee<4, 0><<<grid, 256, 0, st>>>(a); break;
+            case 11: k_onee<4, 1><<<grid, 256, 0, st>>>(a); break;
+            case 12: k_onee<4, 2><<<grid, 256, 0, st>>>(a); break;
+            case 13: k_onee<4, 3><<<grid, 256, 0, st>>>(a); break;
+            default: k_onee<4, 4><<<grid, 256, 0, st>>>(a); break;
         }
     }
 }

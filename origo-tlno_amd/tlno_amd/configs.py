@@ -23,6 +23,8 @@ test_molecules/README.md:3):
   organic_f  organic30's generator with an f polarization shell (1-2
              primitives, exponents in [0.3, 6]) on the heavy atoms, a few
              atoms: the f-shell parity workload.
+  organic_g  the same with an f and a g polarization shell: the g-shell
+             parity workload.
 
 Bases are drawn once per element (a synthetic basis SET) from the workload's
 seed, so equal elements share contractions as a real basis set does.  Seeds
@@ -200,6 +202,23 @@ def organic_f(seed=SEED0 + 8, natoms=5, box_A=4.5):
     for el in ("C", "N", "O"):
         eb[el] = eb[el] + _draw_shells(rng, [3], (1, 2), (0.3, 6.0))
     b = _build(atoms, eb, f"organic_f{natoms}")
+    D = _density(rng, b.n, max(1, b.n // 5))
+    D /= np.linalg.norm(D)
+    return b, D
+
+
+def organic_g(seed=SEED0 + 9, natoms=3, box_A=3.5):
+    """organic_f with a g polarization shell as well: heavy atoms 3s2p1d1f1g
+    (the f and g shells 1-2 primitives, exponents log-uniform in [0.3, 6]),
+    H 2s1p, a few atoms: the g-shell parity workload (the reference's
+    largest angular momentum, L = 4)."""
+    rng = np.random.default_rng(seed)
+    atoms = _random_atoms(rng, natoms, box_A, 1.0, probs=(0.4, 0.2, 0.2, 0.2))
+    eb = _element_bases(rng, ["C", "N", "O", "H"], [0, 0, 0, 1, 1, 2], [0, 0, 1],
+                        (1, 6), (0.08, 5000.0))
+    for el in ("C", "N", "O"):
+        eb[el] = eb[el] + _draw_shells(rng, [3, 4], (1, 2), (0.3, 6.0))
+    b = _build(atoms, eb, f"organic_g{natoms}")
     D = _density(rng, b.n, max(1, b.n // 5))
     D /= np.linalg.norm(D)
     return b, D

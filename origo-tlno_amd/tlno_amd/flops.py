@@ -18,8 +18,9 @@ integral (J only 4, K only 8).  Worked values: (ss|ss) 71, (ps|ss) 92,
 """
 from __future__ import annotations
 
-CLASS_L = [(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (2, 2), (3, 0), (3, 1), (3, 2), (3, 3)]
-CLASS_NAME = ["ss", "ps", "pp", "ds", "dp", "dd", "fs", "fp", "fd", "ff"]
+CLASS_L = [(0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (2, 2), (3, 0), (3, 1), (3, 2), (3, 3),
+           (4, 0), (4, 1), (4, 2), (4, 3), (4, 4)]
+CLASS_NAME = ["ss", "ps", "pp", "ds", "dp", "dd", "fs", "fp", "fd", "ff", "gs", "gp", "gd", "gf", "gg"]
 
 
 def ncart(L):

@@ -252,13 +252,13 @@ class DirectPlan:
         return res
 
     def pairs(self):
-        """(group_a, group_b, Schwarz bound, class_begin[11]) in plan pair order (classes ss, ps, pp,
-        ds, dp, dd, fs, fp, fd, ff)."""
+        """(group_a, group_b, Schwarz bound, class_begin[16]) in plan pair order (classes ss, ps, pp,
+        ds, dp, dd, fs, fp, fd, ff, gs, gp, gd, gf, gg)."""
         n = self.profile.pair_groups
         ga = np.zeros(n, np.int32)
         gb = np.zeros(n, np.int32)
         Q = np.zeros(n)
-        cb = np.zeros(11, np.int32)
+        cb = np.zeros(16, np.int32)
         _lib.check(_lib.load().tlno_gpu_plan_pairs(self._h, ga.ctypes.data_as(_lib.I), gb.ctypes.data_as(_lib.I),
                                                    Q.ctypes.data_as(_lib.D), cb.ctypes.data_as(_lib.I)))
         return ga, gb, Q, cb

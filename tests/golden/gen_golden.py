@@ -72,6 +72,10 @@ tests) and the reference's outputs:
                    Cartesian J/K (density D) and spherical J/K (density Dsph)
                    at Schwarz 1e-12 (python gen_golden.py fshell rewrites
                    only it)
+  gshell.npz       the same for configs.organic_g() (3 atoms, heavy
+                   3s2p1d1f1g: 120 Cartesian / 90 spherical functions), one
+                   quartet per class pair with a g shell (python gen_golden.py
+                   gshell rewrites only it)
   density.npz      4-water cluster (96 spherical functions), density
                    configs.spherical_density(96, 9, 20): the reference's
                    density-screened J, K and G (density_threshold 1e-9, 1e-7,
@@ -123,6 +127,8 @@ def main(only=None):
         return water10()
     if only == "fshell":
         return fshell(ref)
+    if only == "gshell":
+        return fshell(ref, lmax=4)
     t0 = time.time()
     # ---------------------------------------------------------- Boys / Rys
     xs = np.concatenate([[0.0, 1e-10, 1e-3, 0.5, 5.0, 11.99, 12.0, 12.5, 25.0, 39.99, 40.0, 45.0, 80.0, 200.0],
@@ -192,6 +198,7 @@ def main(only=None):
     water10()
     fingerprint(ref)
     fshell(ref)
+    fshell(ref, lmax=4)
     print(f"golden fixtures written in {time.time() - t0:.1f} s")
 
 
@@ -325,25 +332,25 @@ def fingerprint(ref):
     np.savez_compressed(os.path.join(HERE, "fingerprint.npz"), **out)
 
 
-def fshell(ref):
-    # ---------------------------------------------------------- f shells
-    b, D = configs.organic_f()
+def fshell(ref, lmax=3):
+    # ---------------------------------------------------------- f (g) shells
+    b, D = configs.organic_f() if lmax == 3 else configs.organic_g()
     out = {"basis_sha": basis_hash(b), "D": D}
     groups = ref.groups(b)
     Ls = [g[0] for g in groups]
     byL = {L: [g for g in range(len(groups)) if Ls[g] == L] for L in set(Ls)}
-    rng = np.random.default_rng(61)
+    rng = np.random.default_rng(61 if lmax == 3 else 81)
     quads = []
-    # one quartet per class pair with an f shell (bra L >= ket L per pair,
-    # bra pair class >= ket pair class), then shuffled shell orders
-    cls = [(la, lb) for la in range(4) for lb in range(la + 1)]
+    # one quartet per class pair with a shell of L = lmax (bra L >= ket L per
+    # pair, bra pair class >= ket pair class), then shuffled shell orders
+    cls = [(la, lb) for la in range(lmax + 1) for lb in range(la + 1)]
     for x in range(len(cls)):
         for y in range(x + 1):
             (la, lb), (lc, ld) = cls[x], cls[y]
-            if max(la, lb, lc, ld) == 3:
+            if max(la, lb, lc, ld) == lmax:
                 quads.append([int(rng.choice(byL[v])) for v in (la, lb, lc, ld)])
     for _ in range(12):
-        q = [int(rng.choice(byL[3]))] + [int(rng.integers(len(groups))) for _ in range(3)]
+        q = [int(rng.choice(byL[lmax]))] + [int(rng.integers(len(groups))) for _ in range(3)]
         quads.append([q[i] for i in rng.permutation(4)])
     blocks = [ref.cart_block(b, *q, groups=groups).ravel() for q in quads]
     out["quads"] = np.array(quads, dtype=np.int32)
@@ -371,7 +378,7 @@ def fshell(ref):
     for sph in (0, 1):
         S, T, V = ref.onee(b, b.nuclear_charges, b.nuclear_centers, spherical=bool(sph), screening=False)
         out[f"S{sph}"], out[f"T{sph}"], out[f"V{sph}"] = S, T, V
-    np.savez_compressed(os.path.join(HERE, "fshell.npz"), **out)
+    np.savez_compressed(os.path.join(HERE, "fshell.npz" if lmax == 3 else "gshell.npz"), **out)
 
 
 BATCH256_PICK = tuple(range(0, 256, 16))

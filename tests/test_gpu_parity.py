@@ -41,16 +41,16 @@ class ExactFallback:
         self.limit = limit
         self.used = 0
 
-    def check(self, x, ref, exact_of):
+    def check(self, x, ref, exact_of, floor=0.0):
         x = np.asarray(x)
         ref = np.asarray(ref)
-        bad = np.argwhere(np.abs(x - ref) > ERI_ABS + ERI_REL * np.abs(ref))
+        bad = np.argwhere(np.abs(x - ref) > ERI_ABS + ERI_REL * np.abs(ref) + floor)
         for idx in bad:
             idx = tuple(int(i) for i in idx)
             self.used += 1
             assert self.used <= self.limit, "too many entries differ from the oracle"
             ex = float(exact_of(idx))
-            assert abs(x[idx] - ex) <= ERI_ABS + ERI_REL * abs(ex), (idx, x[idx], ref[idx], ex)
+            assert abs(x[idx] - ex) <= ERI_ABS + ERI_REL * abs(ex) + floor, (idx, x[idx], ref[idx], ex)
 
 
 def _block_exact(b, ob, q):

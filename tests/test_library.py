@@ -48,15 +48,15 @@ def _bad_basis(kind):
         off = np.concatenate([[0], np.cumsum(nprim)])
         pk = np.concatenate([np.arange(off[i], off[i + 1]) for i in range(b.n) if keep[i]])
         centers, ang, nprim, exps, coefs = centers[keep], ang[keep], nprim[keep], exps[pk], coefs[pk]
-    elif kind == "g_shell":
-        ang[2] = [4, 0, 0]
+    elif kind == "h_shell":
+        ang[2] = [5, 0, 0]
     return centers, ang, nprim, exps, coefs
 
 
 @pytest.mark.parametrize("kind,msg", [("nan_center", "shell centers must be finite"),
                                       ("neg_exp", "primitive exponents must be finite and positive"),
                                       ("incomplete", "exactly one complete Cartesian shell"),
-                                      ("g_shell", "shell L=0..3")])
+                                      ("h_shell", "shell L=0..4")])
 def test_plan_rejects_invalid_basis_on_host(kind, msg):
     lib = _lib.load()
     c, a, n, e, f = (np.ascontiguousarray(x) for x in _bad_basis(kind))

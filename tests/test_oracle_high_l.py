@@ -1,13 +1,14 @@
-"""CPU: the oracle pinned for f shells (L = 3) and for Rys rules of 6..9 roots,
-the part of the reference's L <= 4 range beyond d that the engine covers.
+"""CPU: the oracle pinned for f and g shells (L = 3, 4) and for Rys rules of
+6..9 roots, the part of the reference's L <= 4 range beyond d.
 
   * Rys rules, N = 6..9: the moment identities sum_i w_i t_i^(2k) = F_k(T),
     k < 2N, against 40-digit Boys values (mpmath's incomplete gamma), and the
     ordering / positivity of roots and weights.  The reference's own rules
     sit about 1e-12 from those moments (its table route), so they are compared
     at that level only.
-  * f blocks: the reference's audit-route Cartesian blocks of one quartet per
-    class pair with an f shell (tests/golden/fshell.npz); entries outside the
+  * f and g blocks: the reference's audit-route Cartesian blocks of one quartet
+    per class pair with an f (a g) shell (tests/golden/fshell.npz, gshell.npz);
+    entries outside the
     strict bound |d| <= 1e-14 + 1e-12 |ref| are judged against their 40-digit
     McMurchie-Davidson values (tests/mp_integrals.py), which the oracle must
     meet.
@@ -57,10 +58,11 @@ def test_rys_high_order_moments_exact(N):
     assert worst_ref < 1e-10
 
 
-def test_f_blocks_vs_reference_audit_blocks():
+@pytest.mark.parametrize("case", ["f", "g"])
+def test_blocks_vs_reference_audit_blocks(case):
     mp.mp.dps = 40
-    g = np.load(os.path.join(GOLD, "fshell.npz"))
-    b, _ = configs.organic_f()
+    g = np.load(os.path.join(GOLD, f"{case}shell.npz"))
+    b, _ = (configs.organic_f if case == "f" else configs.organic_g)()
     ob = O.OracleBasis(b)
     rows = []
     nval = 0
@@ -76,7 +78,7 @@ def test_f_blocks_vs_reference_audit_blocks():
             idx = np.unravel_index(e, dims)
             comps = [int(ob.groups[q[m]][1][idx[m]]) for m in range(4)]
             rows.append((abs(ours[e] - ref[e]), comps, ours[e], ref[e]))
-    print(f"{len(g['quads'])} f blocks, {nval} values; {len(rows)} outside the strict bound of the reference")
+    print(f"{len(g['quads'])} {case} blocks, {nval} values; {len(rows)} outside the strict bound of the reference")
     rows.sort(key=lambda r: -r[0])
     pick = rows[:4]
     if len(rows) > 4:
@@ -137,11 +139,12 @@ def test_f_blocks_symmetry_and_translation():
         assert np.allclose(X, Y, rtol=1e-10, atol=1e-12 * max(1.0, np.abs(X).max()))
 
 
-def test_f_spherical_jk_vs_reference():
-    g = np.load(os.path.join(GOLD, "fshell.npz"))
+@pytest.mark.parametrize("case", ["f", "g"])
+def test_spherical_jk_vs_reference(case):
+    g = np.load(os.path.join(GOLD, f"{case}shell.npz"))
     Go = g["Jo_12"] - 0.5 * g["Ko_12"]
     d = np.abs(Go - g["G_12"]).max()
-    print(f"oracle against the reference's spherical G with f shells: {d:.2e} (|G| up to {np.abs(Go).max():.2f})")
+    print(f"oracle against the reference's spherical G with {case} shells: {d:.2e} (|G| up to {np.abs(Go).max():.2f})")
     assert d < 1e-10
     # symmetric to well inside the J/K tolerance (1e-10) the GPU tests use
     for X in (g["Jo_12"], g["Ko_12"], g["Jc_12"], g["Kc_12"]):
