@@ -76,10 +76,9 @@ struct TeamShape {
     // combination)
     static constexpr int EP = TW > 1 ? EPAD : (NE <= 8 ? 8 : (NE <= 16 ? 16 : (NE <= 32 ? 32 : 64)));
     static constexpr int NP = TW > 1 ? 1 : 64 / EP;
-    // more bra intermediates than threads: rounds, each round evaluating
-    // every combination again, each round's rows [e|f] waiting in LDS ahead of
-    // the tables (not reached: one wave takes at most 64, several waves EPAD)
-    static constexpr int NRND = TW > 1 ? 1 : (NE + 63) / 64;
+    // every bra intermediate has its thread: one wave takes at most 64, the
+    // multi-wave classes EPAD
+    static_assert(TW > 1 || NE <= 64, "one wave owns at most 64 bra intermediates");
     static constexpr int OPL = (NOUT + 63) / 64;
     static constexpr int GS = (LAB + 1) * (LCD + 1);       // one axis table
     // one combination's three tables, padded to an odd number of doubles:
@@ -100,30 +99,45 @@ struct TeamShape {
     static constexpr int R0 = NI > NOUT ? NI : NOUT;
     static constexpr int LDS_C = R0 + NX;
     static constexpr int LDS_D = NOUT + NA * NB + NC * ND + NA * NC + NA * ND + NB * NC + NB * ND;
-    static constexpr int A0 = NRND > 1 ? NI : 0;            // phase A tables start here
     static constexpr int LDS_CD = LDS_C > LDS_D ? LDS_C : LDS_D;
     // phases C and D work in LDS while their working set fits (gfx950 gives a
     // workgroup up to 160 KB; (ff|ff) needs 139 KB), else in a global scratch
     // slot of the workgroup (the large g classes: (gg|gg) needs 843 KB)
     static constexpr bool GWORK = LDS_CD * 8 > 144 * 1024;
     static constexpr int WORK = GWORK ? LDS_CD : 0;        // doubles per scratch slot
-    static constexpr int LDS = GWORK ? A0 + LDS_A : (A0 + LDS_A > LDS_CD ? A0 + LDS_A : LDS_CD);
+    static constexpr int LDS = GWORK ? LDS_A : (LDS_A > LDS_CD ? LDS_A : LDS_CD);
     static_assert(LDS * 8 <= 160 * 1024, "team evaluator LDS beyond 160 KB");
+    // waves per SIMD the register allocation must leave room for: one-wave
+    // classes whose LDS admits eleven or more workgroups per CU ask for three
+    // (at most 168 VGPRs); above that their registers, not their LDS, would
+    // cap the CU at eight waves ((pp|pp), (dd|pp) and (dp|ps) at 176-186
+    // VGPRs ran 18-27 % slower than at 164).  Classes that fit nine or ten
+    // workgroups gain no wave from the cap and measured a few per cent slower
+    // under it.
+    static constexpr int WPE = (TW == 1 && 160 * 1024 / (LDS * 8) >= 11) ? 3 : 1;
 };
 
 
-// binomial HRR weight C(n, t) X^(n - t) for n <= 4 (the power n - t decides
-// the coefficient: C(n, n-1) = n, C(n, n-2) = n(n-1)/2, C(n, n-3) = 4 for
-// n = 4 and 1 for n = 3); exponents are decoded arithmetically, since a
-// lane-dependent index into a constexpr table would put the table in per-lane
-// scratch memory
+// binomial HRR weight C(n, t) X^(n - t) for n <= L (the shell's angular
+// momentum, at most 4; the power n - t decides the coefficient: C(n, n-1) = n,
+// C(n, n-2) = n(n-1)/2, C(n, n-3) = 4 for n = 4 and 1 for n = 3); exponents
+// are decoded arithmetically, since a lane-dependent index into a constexpr
+// table would put the table in per-lane scratch memory.  Shells up to d take
+// the two-branch form (the general one costs the (pp|pp) quartets, whose
+// transfer and digestion weigh as much as their contraction, about 25 %).
+template <int L>
 __device__ __forceinline__ double hrr_weight(int n, int t, double X) {
+    static_assert(L <= 4, "HRR weights for shells up to g");
     const int k = n - t;
-    if (k == 0) return 1.0;
-    if (k == 1) return n * X;
-    if (k == 2) return ((n * (n - 1)) >> 1) * (X * X);
-    if (k == 3) return (n == 4 ? 4.0 : 1.0) * (X * X * X);
-    return (X * X) * (X * X);
+    if constexpr (L <= 2) {
+        return k == 0 ? 1.0 : (k == 1 ? n * X : X * X);
+    } else {
+        if (k == 0) return 1.0;
+        if (k == 1) return n * X;
+        if (k == 2) return ((n * (n - 1)) >> 1) * (X * X);
+        if (k == 3) return (n == 4 ? 4.0 : 1.0) * (X * X * X);
+        return (X * X) * (X * X);
+    }
 }
 
 // the triples of the list (|f| from LMIN to LMAX, by |f| then descending
@@ -202,11 +216,9 @@ __device__ __forceinline__ bool eri_team_block(const PairData& pd, const DevTabl
     const int ncomb = min(ntot, cbeg + per);
     if (cbeg >= ncomb) return false;  // the whole block leaves together
 
-    double* const tab = lds + T::A0;  // phase A tables (after the rows of earlier rounds)
-    double* ef = work;                // NI, [e][f]
-#pragma unroll 1
-    for (int rnd = 0; rnd < T::NRND; ++rnd) {
-    const int e_own = rnd * T::NT + tid % T::EP;
+    double* const tab = lds;  // phase A tables
+    double* ef = work;        // NI, [e][f]
+    const int e_own = tid % T::EP;
     int ex, ey, ez;
     tri_decode(LA, e_own < T::NE ? e_own : 0, ex, ey, ez);
     const int rx0 = ex * (T::LCD + 1), ry0 = T::GS + ey * (T::LCD + 1), rz0 = 2 * T::GS + ez * (T::LCD + 1);
@@ -297,7 +309,6 @@ __device__ __forceinline__ bool eri_team_block(const PairData& pd, const DevTabl
     }
 
     // sum the parts (lane bits above EP) into part 0, which stores its row
-    // (this round's rows: the tables of a later round sit after all rows)
 #pragma unroll
     for (int f = 0; f < T::FCH; ++f) {
         double v = acc[f];
@@ -311,7 +322,6 @@ __device__ __forceinline__ bool eri_team_block(const PairData& pd, const DevTabl
         for (int f = 0; f < T::FCH; ++f)
             if (f0 + f < T::NF) ef[e_own * T::NF + f0 + f] = acc[f];
     }
-    }  // rounds
     __syncthreads();
 
     // ---- phase C: HRR from [e|f] to the unit-diagonal Cartesian block, ket
@@ -320,7 +330,7 @@ __device__ __forceinline__ bool eri_team_block(const PairData& pd, const DevTabl
     double* blk = work;           // NOUT, [a][b][c][d] (over [e|f], read before)
     const double AB[3] = {pd.AB[3 * P], pd.AB[3 * P + 1], pd.AB[3 * P + 2]};
     const double CD[3] = {pd.AB[3 * Q], pd.AB[3 * Q + 1], pd.AB[3 * Q + 2]};
-    auto hw = [](int n, int t, double X) { return hrr_weight(n, t, X); };
+
     // [e|c d) = sum_u C(d, u) CD^(d - u) [e|c + u]
     for (int i = tid; i < T::NX; i += T::NT) {
         const int d = i % T::ND, c = (i / T::ND) % T::NC, e = i / (T::ND * T::NC);
@@ -332,7 +342,7 @@ __device__ __forceinline__ bool eri_team_block(const PairData& pd, const DevTabl
         for (int ux = 0; ux <= dx; ++ux)
             for (int uy = 0; uy <= dy; ++uy)
                 for (int uz = 0; uz <= dz; ++uz)
-                    u = fma(hw(dx, ux, CD[0]) * hw(dy, uy, CD[1]) * hw(dz, uz, CD[2]),
+                    u = fma(hrr_weight<LD>(dx, ux, CD[0]) * hrr_weight<LD>(dy, uy, CD[1]) * hrr_weight<LD>(dz, uz, CD[2]),
                             row[tri_index(LC, cxc + ux, cyc + uy, czc + uz)], u);
         xk[i] = u;
     }
@@ -350,7 +360,7 @@ __device__ __forceinline__ bool eri_team_block(const PairData& pd, const DevTabl
         for (int tx = 0; tx <= bx; ++tx)
             for (int ty = 0; ty <= by; ++ty)
                 for (int tz = 0; tz <= bz; ++tz)
-                    v = fma(hw(bx, tx, AB[0]) * hw(by, ty, AB[1]) * hw(bz, tz, AB[2]),
+                    v = fma(hrr_weight<LB>(bx, tx, AB[0]) * hrr_weight<LB>(by, ty, AB[1]) * hrr_weight<LB>(bz, tz, AB[2]),
                             xk[tri_index(LA, axc + tx, ayc + ty, azc + tz) * (T::NC * T::ND) + cd_], v);
         blk[o] = v * (nang(axc, ayc, azc) * nang(bx, by, bz) * nang(cxc, cyc, czc) * nang(dx, dy, dz));
     }
@@ -451,7 +461,8 @@ __device__ __forceinline__ void eri_team_digest(const JKArgs& a, double* __restr
 // Same task formats, counters, molecule offsets and persistent stride (over
 // work items) as the lane evaluator; FP64 only.
 template <int LA, int LB, int LC, int LD>
-__global__ __launch_bounds__((TeamShape<LA, LB, LC, LD>::NT)) void k_eri_team(JKArgs a) {
+__global__ __launch_bounds__((TeamShape<LA, LB, LC, LD>::NT))
+__attribute__((amdgpu_waves_per_eu(TeamShape<LA, LB, LC, LD>::WPE))) void k_eri_team(JKArgs a) {
     using T = TeamShape<LA, LB, LC, LD>;
     __shared__ double lds[T::LDS];
     double* const work = T::GWORK ? a.gwork + static_cast<long long>(blockIdx.x) * T::WORK : lds;
@@ -491,7 +502,10 @@ __global__ __launch_bounds__((TeamShape<LA, LB, LC, LD>::NT)) void k_eri_team(JK
                 Q = qb + k;
             }
         }
-        if (a.tquart && (a.sblk || a.sdecay)) {
+        // (quartet lists with screens serve the f and g classes only, whose
+        // bra pair holds the f or g shell; the code stays out of the
+        // lower classes' kernels)
+        if constexpr (LA >= 3) if (a.tquart && (a.sblk || a.sdecay)) {
             // the routing kernel's density and decay rules, per quartet
             bool keep = true;
             if (a.sblk) {
