@@ -102,7 +102,8 @@ int tlno_gpu_build_jk_device(tlno_gpu_plan* plan, const double* d_density,
  * process ends), so the plan may have at most 1024 Cartesian functions
  * (larger plans are rejected with an error).  The call allocates nothing:
  * the plan's work lists (per Schwarz threshold), routing buffers, team
- * quartet lists and the device's side streams must already exist, i.e. one
+ * quartet lists, the team working buffer of plans with g shells and the
+ * device's side streams must already exist, i.e. one
  * tlno_gpu_build_jk_device (or tlno_gpu_build_jk) call on this plan with the
  * same threshold and settings has run; otherwise it returns an error and
  * does no work.  All calls on one GPU share its array: calls that overlap in
