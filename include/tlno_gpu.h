@@ -250,7 +250,7 @@ int tlno_gpu_eri_cart_blocks(tlno_gpu_plan* plan, int n, const int* groups4,
 #define TLNO_GPU_BOYS_MMAX 128
 int tlno_gpu_boys(int n, const double* T, int mmax, double* out);
 
-/* Rys roots t^2 and weights for N = 1..5 at n arguments: roots[i*N+k]; every
+/* Rys roots t^2 and weights for N = 1..9 at n arguments: roots[i*N+k]; every
  * T finite and >= 0 (checked).
  * Counterpart of rys_roots_weights (rys_engine.ipp:3820-3837). */
 int tlno_gpu_rys(int N, int n, const double* T, double* roots, double* weights);

@@ -73,8 +73,13 @@ struct TeamShape {
     // phase B ownership: thread = (bra intermediate e < EP, part); a thread
     // holds its f-chunk of the intermediates [e|f] of its e and takes every
     // NP-th combination (several waves: thread = (f-chunk, e), every
-    // combination)
-    static constexpr int EP = TW > 1 ? EPAD : (NE <= 8 ? 8 : (NE <= 16 ? 16 : (NE <= 32 ? 32 : 64)));
+    // combination).  One wave: blocks of EP = NE lanes (EXACT) where that
+    // keeps more lanes busy than blocks of the next power of two ((pp| has 9
+    // intermediates: 7 parts on 63 lanes, where blocks of 16 left 7 of every
+    // 16 idle); their partial rows are summed through LDS
+    static constexpr int EP2 = NE <= 8 ? 8 : (NE <= 16 ? 16 : (NE <= 32 ? 32 : 64));
+    static constexpr bool EXACT = TW == 1 && 64 / EP2 < 64 / NE;
+    static constexpr int EP = TW > 1 ? EPAD : (EXACT ? NE : EP2);
     static constexpr int NP = TW > 1 ? 1 : 64 / EP;
     // every bra intermediate has its thread: one wave takes at most 64, the
     // multi-wave classes EPAD
@@ -105,7 +110,10 @@ struct TeamShape {
     // slot of the workgroup (the large g classes: (gg|gg) needs 843 KB)
     static constexpr bool GWORK = LDS_CD * 8 > 144 * 1024;
     static constexpr int WORK = GWORK ? LDS_CD : 0;        // doubles per scratch slot
-    static constexpr int LDS = GWORK ? LDS_A : (LDS_A > LDS_CD ? LDS_A : LDS_CD);
+    // the parts' partial rows [p][e][f] after [e|f] (EXACT blocks)
+    static constexpr int LDS_P = EXACT ? (NP + 1) * NI : 0;
+    static constexpr int LDS_ACD = LDS_A > LDS_CD ? LDS_A : LDS_CD;
+    static constexpr int LDS = GWORK ? LDS_A : (LDS_ACD > LDS_P ? LDS_ACD : LDS_P);
     static_assert(LDS * 8 <= 160 * 1024, "team evaluator LDS beyond 160 KB");
     // waves per SIMD the register allocation must leave room for: one-wave
     // classes whose LDS admits eleven or more workgroups per CU ask for three
@@ -287,7 +295,7 @@ __device__ __forceinline__ bool eri_team_block(const PairData& pd, const DevTabl
         __syncthreads();
         // ---- phase B: accumulate the owned intermediates over this part's
         // combinations of the batch (three LDS rows per combination)
-        const int nb = (T::TW == 1 || chunk < T::NCH) ? min(T::CB, ncomb - c0) : 0;
+        const int nb = (T::TW == 1 ? part < T::NP : chunk < T::NCH) ? min(T::CB, ncomb - c0) : 0;
 #pragma unroll 2
         for (int j = part; j < nb; j += T::NP) {
             const double* g = tab + j * T::CS;
@@ -308,19 +316,39 @@ __device__ __forceinline__ bool eri_team_block(const PairData& pd, const DevTabl
         __syncthreads();
     }
 
-    // sum the parts (lane bits above EP) into part 0, which stores its row
+    if constexpr (T::EXACT) {
+        // the parts' rows go to LDS after [e|f] (the tables are consumed);
+        // part 0 adds them in part order
+        double* prow = ef + T::NI;
+        if (part < T::NP) {
 #pragma unroll
-    for (int f = 0; f < T::FCH; ++f) {
-        double v = acc[f];
+            for (int f = 0; f < T::NF; ++f) prow[(part * T::NE + e_own) * T::NF + f] = acc[f];
+        }
+        __syncthreads();
+        if (part == 0) {
 #pragma unroll
-        for (int m = T::EP; m < 64; m <<= 1) v += __shfl_xor(v, m, 64);
-        acc[f] = v;
-    }
-    if (part == 0 && e_own < T::NE && chunk < T::NCH) {
-        const int f0 = chunk * T::FCH;
+            for (int f = 0; f < T::NF; ++f) {
+                double sum = prow[e_own * T::NF + f];
 #pragma unroll
-        for (int f = 0; f < T::FCH; ++f)
-            if (f0 + f < T::NF) ef[e_own * T::NF + f0 + f] = acc[f];
+                for (int pp = 1; pp < T::NP; ++pp) sum += prow[(pp * T::NE + e_own) * T::NF + f];
+                ef[e_own * T::NF + f] = sum;
+            }
+        }
+    } else {
+        // sum the parts (lane bits above EP) into part 0, which stores its row
+#pragma unroll
+        for (int f = 0; f < T::FCH; ++f) {
+            double v = acc[f];
+#pragma unroll
+            for (int m = T::EP; m < 64; m <<= 1) v += __shfl_xor(v, m, 64);
+            acc[f] = v;
+        }
+        if (part == 0 && e_own < T::NE && chunk < T::NCH) {
+            const int f0 = chunk * T::FCH;
+#pragma unroll
+            for (int f = 0; f < T::FCH; ++f)
+                if (f0 + f < T::NF) ef[e_own * T::NF + f0 + f] = acc[f];
+        }
     }
     __syncthreads();
 
