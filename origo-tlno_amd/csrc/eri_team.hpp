@@ -102,7 +102,9 @@ struct TeamShape {
     // [e|cd] sits after them; phase D: the block, then six density blocks
     static constexpr int NX = NE * NC * ND;
     static constexpr int R0 = NI > NOUT ? NI : NOUT;
-    static constexpr int LDS_C = R0 + NX;
+    // after [e|cd]: the unit-diagonal norms of the bra pairs (a, b) and of
+    // the ket pairs (c, d), once per quartet
+    static constexpr int LDS_C = R0 + NX + NA * NB + NC * ND;
     static constexpr int LDS_D = NOUT + NA * NB + NC * ND + NA * NC + NA * ND + NB * NC + NB * ND;
     static constexpr int LDS_CD = LDS_C > LDS_D ? LDS_C : LDS_D;
     // phases C and D work in LDS while their working set fits (gfx950 gives a
@@ -359,6 +361,25 @@ __device__ __forceinline__ bool eri_team_block(const PairData& pd, const DevTabl
     const double AB[3] = {pd.AB[3 * P], pd.AB[3 * P + 1], pd.AB[3 * P + 2]};
     const double CD[3] = {pd.AB[3 * Q], pd.AB[3 * Q + 1], pd.AB[3 * Q + 2]};
 
+    // norms of the pairs: the block element (ab|cd) takes nab[ab] ncd[cd]
+    double* nab = xk + T::NX;
+    double* ncd = nab + T::NA * T::NB;
+    {
+        auto nang = [](int x, int y, int z) { return inv_sqrt_small(dfact2m1(x) * dfact2m1(y) * dfact2m1(z)); };
+        for (int t = tid; t < T::NA * T::NB + T::NC * T::ND; t += T::NT) {
+            int x0, y0, z0, x1, y1, z1;
+            if (t < T::NA * T::NB) {
+                tri_decode(LA, t / T::NB, x0, y0, z0);
+                tri_decode(LB, t % T::NB, x1, y1, z1);
+            } else {
+                const int u = t - T::NA * T::NB;
+                tri_decode(LC, u / T::ND, x0, y0, z0);
+                tri_decode(LD, u % T::ND, x1, y1, z1);
+            }
+            nab[t] = nang(x0, y0, z0) * nang(x1, y1, z1);  // ncd[u] = nab[NA NB + u]
+        }
+    }
+
     // [e|c d) = sum_u C(d, u) CD^(d - u) [e|c + u]
     for (int i = tid; i < T::NX; i += T::NT) {
         const int d = i % T::ND, c = (i / T::ND) % T::NC, e = i / (T::ND * T::NC);
@@ -376,21 +397,18 @@ __device__ __forceinline__ bool eri_team_block(const PairData& pd, const DevTabl
     }
     __syncthreads();
     // (a b| = sum_t C(b, t) AB^(b - t) (a + t|, then the unit-diagonal norms
-    auto nang = [](int x, int y, int z) { return inv_sqrt_small(dfact2m1(x) * dfact2m1(y) * dfact2m1(z)); };
     for (int o = tid; o < T::NOUT; o += T::NT) {
         const int cd_ = o % (T::NC * T::ND), b = (o / (T::NC * T::ND)) % T::NB, a0 = o / (T::NC * T::ND * T::NB);
-        int axc, ayc, azc, bx, by, bz, cxc, cyc, czc, dx, dy, dz;
+        int axc, ayc, azc, bx, by, bz;
         tri_decode(LA, a0, axc, ayc, azc);
         tri_decode(LB, b, bx, by, bz);
-        tri_decode(LC, cd_ / T::ND, cxc, cyc, czc);
-        tri_decode(LD, cd_ % T::ND, dx, dy, dz);
         double v = 0.0;
         for (int tx = 0; tx <= bx; ++tx)
             for (int ty = 0; ty <= by; ++ty)
                 for (int tz = 0; tz <= bz; ++tz)
                     v = fma(hrr_weight<LB>(bx, tx, AB[0]) * hrr_weight<LB>(by, ty, AB[1]) * hrr_weight<LB>(bz, tz, AB[2]),
                             xk[tri_index(LA, axc + tx, ayc + ty, azc + tz) * (T::NC * T::ND) + cd_], v);
-        blk[o] = v * (nang(axc, ayc, azc) * nang(bx, by, bz) * nang(cxc, cyc, czc) * nang(dx, dy, dz));
+        blk[o] = v * (nab[a0 * T::NB + b] * ncd[cd_]);
     }
     __syncthreads();
     return true;
