@@ -2933,6 +2933,16 @@ struct AutotuneScope {
     }
 };
 
+// A lane class's FP32 share runs the FP32 kernels only where they beat the
+// FP64 kernels on the whole class by this fraction: routed builds then split
+// the class into two task lists, which fill their tasks worse than one (on
+// water40's screened build, classes whose FP32 kernels were 3-5 % faster ran
+// 3-6 % slower routed than kept in FP64).
+#ifndef TLNO_FP32_MIN_GAIN
+#define TLNO_FP32_MIN_GAIN 0.10
+#endif
+constexpr double kFp32MinGain = TLNO_FP32_MIN_GAIN;
+
 std::string plan_autotune(tlno_gpu_plan* pl, double thr, int repeats) {
     HIP_OK(hipSetDevice(pl->device));
     const long mo2 = static_cast<long>(pl->Mout) * pl->Mout;
@@ -2980,7 +2990,7 @@ std::string plan_autotune(tlno_gpu_plan* pl, double thr, int repeats) {
             const bool team = has_team && ms[1][X][Y] < ms[0][X][Y];
             pl->team_sel[X][Y] = static_cast<signed char>(team ? 1 : 0);
             pl->cls_cost[X][Y] = team ? ms[1][X][Y] : ms[0][X][Y];
-            pl->fp32_slower[X][Y] = ms[2][X][Y] > 0.0 && ms[2][X][Y] >= ms[0][X][Y];
+            pl->fp32_slower[X][Y] = ms[2][X][Y] > 0.0 && ms[2][X][Y] >= (1.0 - kFp32MinGain) * ms[0][X][Y];
             if (ms[0][X][Y] == 0.0) continue;
             char buf[256];
             if (has_team)

@@ -20,8 +20,14 @@
 #ifndef TLNO_CHUNK_MAXOUT
 #define TLNO_CHUNK_MAXOUT 108
 #endif
+// runs of bra components up to 54 outputs: (dp|pp)'s 162 values as three
+// chunks of 54 hold 255 VGPRs (two waves per SIMD) where two chunks of 81
+// took 390 with spills (one wave); water40's (dp|pp) 206 -> 185 ms in the
+// FP64 build and 81 -> 65 ms in the screened one.  (dd|ds) runs as six
+// chunks of 36; on organic30 its lane kernel is slower that way, and the
+// team evaluator serves it there.
 #ifndef TLNO_CHUNK_RUN_MAXOUT
-#define TLNO_CHUNK_RUN_MAXOUT 81
+#define TLNO_CHUNK_RUN_MAXOUT 54
 #endif
 #ifndef TLNO_CHUNK_GROUP
 #define TLNO_CHUNK_GROUP 2
