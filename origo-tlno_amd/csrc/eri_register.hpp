@@ -1,7 +1,7 @@
 // Compile-time registration of the per-class ERI kernels.  Each class pair
 // (bra X >= ket Y) is split into output chunks of at most TLNO_CHUNK_MAXOUT
 // values:
-//   mode 0: the full block (one chunk);
+//   mode 0: the full block (one chunk) of at most TLNO_CHUNK_SINGLE_MAXOUT;
 //   mode 1: one chunk per run of AN consecutive bra components a (as many
 //           as fit TLNO_CHUNK_RUN_MAXOUT; the last run may be shorter);
 //   mode 2: one chunk per bra component a and group of CN ket components c
@@ -19,6 +19,13 @@
 
 #ifndef TLNO_CHUNK_MAXOUT
 #define TLNO_CHUNK_MAXOUT 108
+#endif
+// blocks of 55-108 values run as bra-component runs of at most 54: (dp|ds)'s
+// and (dd|ps)'s 108 values held 423-430 VGPRs in one chunk (one wave per
+// SIMD); water40's FP64 build 1391 -> 1362 ms, its screened build 454 ->
+// 438 ms
+#ifndef TLNO_CHUNK_SINGLE_MAXOUT
+#define TLNO_CHUNK_SINGLE_MAXOUT 54
 #endif
 // runs of bra components up to 54 outputs: (dp|pp)'s 162 values as three
 // chunks of 54 hold 255 VGPRs (two waves per SIMD) where two chunks of 81
@@ -42,7 +49,7 @@ struct ChunkPolicy {
     static constexpr int NA = ncart(LA), NC = ncart(LC);
     static constexpr int nc = ncart(LA) * ncart(LB) * ncart(LC) * ncart(LD);
     static constexpr int per_c = nc / NA / NC;  // outputs per (a, c)
-    static constexpr int mode = nc <= TLNO_CHUNK_MAXOUT ? 0 : (nc / NA <= TLNO_CHUNK_MAXOUT ? 1 : 2);
+    static constexpr int mode = nc <= TLNO_CHUNK_SINGLE_MAXOUT ? 0 : (nc / NA <= TLNO_CHUNK_MAXOUT ? 1 : 2);
     static constexpr int pick_cn() {
         for (int g = NC; g > 1; --g)
             if (NC % g == 0 && per_c * g <= TLNO_CHUNK_MAXOUT) return g;
