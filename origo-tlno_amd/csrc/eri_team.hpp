@@ -108,8 +108,8 @@ struct TeamShape {
     static constexpr int LDS_D = NOUT + NA * NB + NC * ND + NA * NC + NA * ND + NB * NC + NB * ND;
     static constexpr int LDS_CD = LDS_C > LDS_D ? LDS_C : LDS_D;
     // phases C and D work in LDS while their working set fits (gfx950 gives a
-    // workgroup up to 160 KB; (ff|ff) needs 139 KB), else in a global scratch
-    // slot of the workgroup (the large g classes: (gg|gg) needs 843 KB)
+    // workgroup up to 160 KiB; (ff|ff) needs 138 KiB), else in a global
+    // scratch slot of the workgroup (the large g classes: (gg|gg) needs 654 KiB)
     static constexpr bool GWORK = LDS_CD * 8 > 144 * 1024;
     static constexpr int WORK = GWORK ? LDS_CD : 0;        // doubles per scratch slot
     // the parts' partial rows [p][e][f] after [e|f] (EXACT blocks)
