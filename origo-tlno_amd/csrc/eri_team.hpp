@@ -125,6 +125,10 @@ struct TeamShape {
     // workgroups gain no wave from the cap and measured a few per cent slower
     // under it.
     static constexpr int WPE = (TW == 1 && 160 * 1024 / (LDS * 8) >= 11) ? 3 : 1;
+    // phase-B unroll: four combinations per iteration where LDS, not
+    // registers, caps the wave count and the rows are short ((dp|ds) 2.5 %
+    // faster); two elsewhere (four cost (dd|ds) its third wave per SIMD)
+    static constexpr int BU = (TW == 1 && WPE == 1 && NF <= 9) ? 4 : 2;
 };
 
 
@@ -298,7 +302,7 @@ __device__ __forceinline__ bool eri_team_block(const PairData& pd, const DevTabl
         // ---- phase B: accumulate the owned intermediates over this part's
         // combinations of the batch (three LDS rows per combination)
         const int nb = (T::TW == 1 ? part < T::NP : chunk < T::NCH) ? min(T::CB, ncomb - c0) : 0;
-#pragma unroll 2
+#pragma unroll T::BU
         for (int j = part; j < nb; j += T::NP) {
             const double* g = tab + j * T::CS;
             double rx[T::LCD + 1], ry[T::LCD + 1], rz[T::LCD + 1];
