@@ -222,37 +222,113 @@ EriDirectJKResult build_jk_direct_for_G_rys(const EriPackedPlan& plan, const std
 }
 
 // ---------------------------------------------------- incremental state --
+// The state is a three-step machine over the cached (D, G) pair: skip (D
+// unchanged), exact (G(D) from scratch) or incremental (G += G(D - D_prev),
+// screened by the delta-density threshold). Options are checked once by a
+// rule table; each update scans D, picks its step, builds into locals and
+// commits by swapping, so a throw leaves the state as it was
+// (direct_build.ipp:562-763 for the caller-visible rules and messages).
+namespace {
+
+using IncOpts = EriDirectJKIncrementalOptions;
+
+struct IncRule {
+    bool (*holds)(const IncOpts&);
+    const char* message;
+};
+
+bool delta_screen_on(const IncOpts& o) {
+    return o.delta_density_threshold > 0.0 ||
+           (o.use_first_delta_density_threshold && o.first_delta_density_threshold > 0.0);
+}
+
+// first failing rule names the error
+const IncRule kIncRules[] = {
+    {[](const IncOpts& o) { return o.exact_rebuild_interval != 0u; },
+     "incremental exact_rebuild_interval must be >= 1."},
+    {[](const IncOpts& o) { return finite_nonneg(o.base_options.schwarz_threshold); },
+     "incremental schwarz_threshold must be finite and >= 0."},
+    {[](const IncOpts& o) { return finite_nonneg(o.base_options.primitive_fastschwarz_threshold); },
+     "incremental primitive_fastschwarz_threshold must be finite and >= 0."},
+    {[](const IncOpts& o) { return finite_nonneg(o.base_options.decay_threshold); },
+     "incremental decay_threshold must be finite and >= 0."},
+    {[](const IncOpts& o) { return finite_nonneg(o.base_options.density_threshold); },
+     "incremental base density_threshold must be finite and >= 0."},
+    {[](const IncOpts& o) { return o.base_options.schwarz_threshold == 0.0 || o.base_options.decay_threshold == 0.0; },
+     "incremental base options cannot combine Schwarz and decay screening."},
+    {[](const IncOpts& o) { return finite_nonneg(o.delta_density_threshold); },
+     "incremental delta_density_threshold must be finite and >= 0."},
+    {[](const IncOpts& o) { return finite_nonneg(o.first_delta_density_threshold); },
+     "incremental first_delta_density_threshold must be finite and >= 0."},
+    {[](const IncOpts& o) { return !(delta_screen_on(o) && o.base_options.decay_threshold > 0.0); },
+     "incremental delta-density screening cannot be combined with the heuristic decay metric."},
+};
+
+enum class IncStep { skip, exact, incremental };
+
+// what one step does to the status and the counters
+struct IncStepEffect {
+    EriDirectJKUpdateStatus status;
+    bool counts_exact, counts_incremental, counts_skipped;
+};
+
+IncStepEffect effect_of(IncStep s) {
+    switch (s) {
+        case IncStep::skip: return {EriDirectJKUpdateStatus::skipped_unchanged, false, false, true};
+        case IncStep::exact: return {EriDirectJKUpdateStatus::exact_rebuild, true, false, false};
+        default: return {EriDirectJKUpdateStatus::incremental_update, false, true, false};
+    }
+}
+
+// an unchanged density skips unless forced; the first update, a forced one
+// and every exact_rebuild_interval-th since the last exact one rebuild
+IncStep choose_step(bool have_state, bool unchanged, bool forced, std::uint64_t since_exact,
+                    std::uint64_t interval) {
+    if (have_state && unchanged && !forced) return IncStep::skip;
+    if (!have_state || forced || since_exact + 1u >= interval) return IncStep::exact;
+    return IncStep::incremental;
+}
+
+// D - D_prev (or D itself before the first update), its largest magnitude and
+// whether it is identically zero
+struct DensityScan {
+    std::vector<double> delta;
+    double max_abs = 0.0;
+    bool unchanged = false;
+};
+
+DensityScan scan_density(const std::vector<double>& d, const std::vector<double>& prev, bool have_prev) {
+    DensityScan r;
+    r.unchanged = have_prev;
+    if (have_prev) r.delta.resize(d.size());
+    for (std::size_t i = 0; i < d.size(); ++i) {
+        if (!std::isfinite(d[i]))
+            throw std::runtime_error("eri_ao_tensor_hgp: incremental density contains a non-finite value.");
+        const double x = have_prev ? d[i] - prev[i] : d[i];
+        if (!std::isfinite(x)) throw std::runtime_error("eri_ao_tensor_hgp: incremental density delta overflowed.");
+        if (have_prev) {
+            r.delta[i] = x;
+            if (x != 0.0) r.unchanged = false;
+        }
+        r.max_abs = std::max(r.max_abs, std::abs(x));
+    }
+    return r;
+}
+
+void require_finite(const std::vector<double>& v, const char* message) {
+    if (!std::all_of(v.begin(), v.end(), [](double x) { return std::isfinite(x); }))
+        throw std::runtime_error(message);
+}
+
+}  // namespace
+
 EriDirectJKIncrementalState::EriDirectJKIncrementalState(const EriPackedPlan& plan,
                                                          const EriDirectJKIncrementalOptions& options)
     : plan_(plan), options_(options) {
     M_ = gpu_of(plan_).M;
-    if (options_.exact_rebuild_interval == 0u)
-        throw std::runtime_error("eri_ao_tensor_hgp: incremental exact_rebuild_interval must be >= 1.");
-    const EriPackedBuildOptions& b = options_.base_options;
-    if (!finite_nonneg(b.schwarz_threshold))
-        throw std::runtime_error("eri_ao_tensor_hgp: incremental schwarz_threshold must be finite and >= 0.");
-    if (!finite_nonneg(b.primitive_fastschwarz_threshold))
-        throw std::runtime_error(
-            "eri_ao_tensor_hgp: incremental primitive_fastschwarz_threshold must be finite and >= 0.");
-    if (!finite_nonneg(b.decay_threshold))
-        throw std::runtime_error("eri_ao_tensor_hgp: incremental decay_threshold must be finite and >= 0.");
-    if (!finite_nonneg(b.density_threshold))
-        throw std::runtime_error("eri_ao_tensor_hgp: incremental base density_threshold must be finite and >= 0.");
-    if (b.schwarz_threshold != 0.0 && b.decay_threshold != 0.0)
-        throw std::runtime_error(
-            "eri_ao_tensor_hgp: incremental base options cannot combine Schwarz and decay screening.");
-    if (!finite_nonneg(options_.delta_density_threshold))
-        throw std::runtime_error("eri_ao_tensor_hgp: incremental delta_density_threshold must be finite and >= 0.");
-    if (!finite_nonneg(options_.first_delta_density_threshold))
-        throw std::runtime_error(
-            "eri_ao_tensor_hgp: incremental first_delta_density_threshold must be finite and >= 0.");
-    if ((options_.delta_density_threshold > 0.0 ||
-         (options_.use_first_delta_density_threshold && options_.first_delta_density_threshold > 0.0)) &&
-        b.decay_threshold > 0.0)
-        throw std::runtime_error(
-            "eri_ao_tensor_hgp: incremental delta-density screening cannot be combined with the heuristic decay "
-            "metric.");
-    // exact rebuilds never screen with the (full) density
+    for (const IncRule& r : kIncRules)
+        if (!r.holds(options_)) throw std::runtime_error(std::string("eri_ao_tensor_hgp: ") + r.message);
+    // an exact rebuild never screens with the full density
     options_.base_options.density_threshold = 0.0;
 }
 
@@ -261,73 +337,51 @@ EriDirectJKUpdateStatus EriDirectJKIncrementalState::update(const std::vector<do
     const std::size_t n = static_cast<std::size_t>(M_) * M_;
     if (density.size() != n)
         throw std::runtime_error("eri_ao_tensor_hgp: incremental density must be spherical M x M row-major.");
-    std::vector<double> delta(initialized_ ? n : 0);
-    double max_abs = 0.0;
-    bool unchanged = initialized_;
-    for (std::size_t i = 0; i < n; ++i) {
-        if (!std::isfinite(density[i]))
-            throw std::runtime_error("eri_ao_tensor_hgp: incremental density contains a non-finite value.");
-        const double d = initialized_ ? density[i] - density_[i] : density[i];
-        if (!std::isfinite(d)) throw std::runtime_error("eri_ao_tensor_hgp: incremental density delta overflowed.");
-        if (initialized_) {
-            delta[i] = d;
-            unchanged = unchanged && d == 0.0;
+    DensityScan scan = scan_density(density, density_, initialized_);
+    const IncStep step = choose_step(initialized_, scan.unchanged, force_exact_rebuild,
+                                     updates_since_exact_rebuild_, options_.exact_rebuild_interval);
+
+    double threshold = 0.0;
+    std::vector<double> next_g, next_density;
+    if (step != IncStep::skip) {
+        EriPackedBuildOptions bo = options_.base_options;
+        if (step == IncStep::incremental) {
+            const bool first = options_.use_first_delta_density_threshold && updates_since_exact_rebuild_ == 0u;
+            threshold = first ? options_.first_delta_density_threshold : options_.delta_density_threshold;
+            bo.density_threshold = threshold;
         }
-        max_abs = std::max(max_abs, std::abs(d));
-    }
-    if (unchanged && !force_exact_rebuild) {
-        ++counters_.updates;
-        ++counters_.skipped_updates;
-        last_status_ = EriDirectJKUpdateStatus::skipped_unchanged;
-        last_max_abs_density_delta_ = 0.0;
-        last_effective_density_threshold_ = 0.0;
-        return last_status_;
-    }
-    const bool exact = !initialized_ || force_exact_rebuild ||
-                       updates_since_exact_rebuild_ + 1u >= options_.exact_rebuild_interval;
-    EriPackedBuildOptions bo = options_.base_options;
-    double eff = 0.0;
-    if (!exact) {
-        eff = (options_.use_first_delta_density_threshold && updates_since_exact_rebuild_ == 0u)
-                  ? options_.first_delta_density_threshold
-                  : options_.delta_density_threshold;
-        bo.density_threshold = eff;
-    }
-    std::vector<double> next_density(density);
-    EriDirectJKResult built = build_jk_direct_for_G_rys(plan_, exact ? density : delta, bo);
-    if (built.M != M_ || built.g.size() != n)
-        throw std::runtime_error("eri_ao_tensor_hgp: incremental direct J/K builder returned an inconsistent shape.");
-    std::vector<double> next_g;
-    if (exact) {
-        next_g = std::move(built.g);
-        for (double v : next_g)
-            if (!std::isfinite(v))
-                throw std::runtime_error("eri_ao_tensor_hgp: exact direct J/K rebuild produced a non-finite value.");
-    } else {
-        next_g = g_;
-        for (std::size_t i = 0; i < n; ++i) {
-            if (!std::isfinite(built.g[i]))
-                throw std::runtime_error("eri_ao_tensor_hgp: incremental dG build produced a non-finite value.");
-            next_g[i] += built.g[i];
-            if (!std::isfinite(next_g[i]))
-                throw std::runtime_error("eri_ao_tensor_hgp: incremental cached G overflowed.");
+        next_density = density;
+        EriDirectJKResult built =
+            build_jk_direct_for_G_rys(plan_, step == IncStep::exact ? density : scan.delta, bo);
+        if (built.M != M_ || built.g.size() != n)
+            throw std::runtime_error(
+                "eri_ao_tensor_hgp: incremental direct J/K builder returned an inconsistent shape.");
+        if (step == IncStep::exact) {
+            require_finite(built.g, "eri_ao_tensor_hgp: exact direct J/K rebuild produced a non-finite value.");
+            next_g = std::move(built.g);
+        } else {
+            require_finite(built.g, "eri_ao_tensor_hgp: incremental dG build produced a non-finite value.");
+            next_g = g_;
+            for (std::size_t i = 0; i < n; ++i) next_g[i] += built.g[i];
+            require_finite(next_g, "eri_ao_tensor_hgp: incremental cached G overflowed.");
         }
     }
-    density_.swap(next_density);
-    g_.swap(next_g);
-    initialized_ = true;
+
+    // commit: nothing below throws
+    const IncStepEffect fx = effect_of(step);
+    if (step != IncStep::skip) {
+        density_.swap(next_density);
+        g_.swap(next_g);
+        initialized_ = true;
+        updates_since_exact_rebuild_ = fx.counts_exact ? 0u : updates_since_exact_rebuild_ + 1u;
+    }
     ++counters_.updates;
-    last_max_abs_density_delta_ = max_abs;
-    last_effective_density_threshold_ = eff;
-    if (exact) {
-        ++counters_.exact_rebuilds;
-        updates_since_exact_rebuild_ = 0;
-        last_status_ = EriDirectJKUpdateStatus::exact_rebuild;
-    } else {
-        ++counters_.incremental_updates;
-        ++updates_since_exact_rebuild_;
-        last_status_ = EriDirectJKUpdateStatus::incremental_update;
-    }
+    counters_.exact_rebuilds += fx.counts_exact;
+    counters_.incremental_updates += fx.counts_incremental;
+    counters_.skipped_updates += fx.counts_skipped;
+    last_status_ = fx.status;
+    last_max_abs_density_delta_ = step == IncStep::skip ? 0.0 : scan.max_abs;
+    last_effective_density_threshold_ = threshold;
     return last_status_;
 }
 

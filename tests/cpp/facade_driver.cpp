@@ -111,6 +111,33 @@ int main(int argc, char** argv) {
         } catch (const std::runtime_error& e) {
             std::fprintf(out, "error %s\n", e.what());
         }
+        // incremental options: the first broken rule names the error; a
+        // failed update leaves the state untouched
+        {
+            twoe::EriDirectJKIncrementalOptions zero = io, mixed = io;
+            zero.exact_rebuild_interval = 0;
+            mixed.delta_density_threshold = 1e-9;
+            mixed.base_options.schwarz_threshold = 0.0;
+            mixed.base_options.decay_threshold = 1e-6;
+            int raised = 0;
+            for (const auto* o : {&zero, &mixed}) {
+                try {
+                    twoe::EriDirectJKIncrementalState bad_state(eplan, *o);
+                } catch (const std::runtime_error& e) {
+                    ++raised;
+                    std::fprintf(out, "inc_error %s\n", e.what());
+                }
+            }
+            const std::uint64_t before = state.counters().updates;
+            try {
+                (void)state.update(std::vector<double>(3, 0.0));
+            } catch (const std::runtime_error&) {
+                ++raised;
+            }
+            const bool kept = state.counters().updates == before &&
+                              state.last_status() == twoe::EriDirectJKUpdateStatus::exact_rebuild;
+            put(out, "inc_errors", {double(raised), kept ? 1.0 : 0.0});
+        }
         // one-electron matrices
         const onee::OneElectronMatrices mc = onee::build_one_electron_matrices(build);
         put(out, "S_cart", mc.overlap.values);

@@ -35,8 +35,8 @@ def run(b, D0, D1, workdir):
         with open(out) as f:
             for line in f:
                 name, rest = line.split(" ", 1)
-                if name == "error":
-                    res["error"] = rest.strip()
+                if name in ("error", "inc_error"):
+                    res.setdefault(name, []).append(rest.strip())
                     continue
                 vals = rest.split()
                 res[name] = np.array([float(x) for x in vals[1:]])

@@ -63,7 +63,13 @@ def test_facade_matches_reference_and_mirror(name, tmp_path):
     assert res["counters"].tolist() == [6, 3, 2, 1]
     assert np.abs(res["G_incremental"] - G1).max() < 1e-12
     assert np.abs(res["G_final"] - G1).max() < 1e-12
-    assert "schwarz_threshold must be finite" in res["error"]
+    assert "schwarz_threshold must be finite" in res["error"][0]
+    # incremental options: interval 0, then delta-density screening with decay;
+    # a wrongly shaped density raises and leaves the state as it was
+    assert res["inc_error"] == [
+        "eri_ao_tensor_hgp: incremental exact_rebuild_interval must be >= 1.",
+        "eri_ao_tensor_hgp: incremental delta-density screening cannot be combined with the heuristic decay metric."]
+    assert res["inc_errors"].tolist() == [3.0, 1.0]
     # one-electron matrices
     n = len(b.nprim)
     assert np.abs(res["S_cart"] - g["S0"].ravel()).max() < 1e-10
